@@ -1,0 +1,229 @@
+/*
+ * spp_engine.h -- C ABI of the MI355X particle engine for 2-D self-propelled particles.
+ *
+ * The reference (hamidallaei/SPP) has no FFI: its "operator API" is the header-only C++ Box /
+ * Particle class surface.  Every entry point below replaces one of those methods; the comment on
+ * each names the reference member it stands in for (file:line in the reference tree).  The C++
+ * facade in spp_amd/csrc/box.hpp keeps the reference's method names on top of these calls.
+ *
+ * Conventions
+ *   - Plain C types only; all arrays are host pointers in ORIGINAL particle order (index i is the
+ *     i-th particle the caller handed to spp_set_state).  The engine keeps its device copy sorted
+ *     by cell and un-permutes on every getter.
+ *   - Box geometry follows the reference: Lx, Ly are HALF widths, the box is [-Lx, Lx) x [-Ly, Ly)
+ *     (shared/parameters.h:68-74, shared/c2dvector.h:67-73).
+ *   - Every call returns SPP_OK (0) or a negative error code; spp_last_error() gives the text.
+ *     Nothing exits the process (the reference calls exit(0), e.g. shared/cell.h:45-49).
+ *   - Calls are ordered on the engine's HIP stream; getters synchronise.  One engine per host
+ *     thread; several engines may coexist on one GPU.
+ */
+#ifndef SPP_ENGINE_H
+#define SPP_ENGINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPP_ABI_VERSION 1
+
+/* ---- error codes ---------------------------------------------------------------------------- */
+enum {
+    SPP_OK = 0,
+    SPP_ERR_ARG = -1,         /* bad argument / parameter combination                              */
+    SPP_ERR_HIP = -2,         /* HIP runtime error (message has the HIP error string)              */
+    SPP_ERR_STATE = -3,       /* call not valid in the engine's current state                      */
+    SPP_ERR_CAPACITY = -4,    /* caller buffer too small (size needed is reported)                 */
+    SPP_ERR_UNSUPPORTED = -5, /* model / integrator combination not implemented                    */
+    SPP_ERR_NO_DEVICE = -6    /* no HIP device visible                                             */
+};
+
+/* ---- particle models ------------------------------------------------------------------------ */
+enum {
+    /* RepulsiveParticle (shared/particle.h:499-786): f = v0*vhat + sum F_rep(d < rc),
+       tau = sum g*sin(theta_j - theta_i)/(pi*R_align^2) over d < R_align.                       */
+    SPP_MODEL_REPULSIVE = 0,
+    /* ContinuousParticle torque law (shared/particle.h:296-338): tau = g*sum[(1-alpha)
+       sin(theta_j - theta_i)/pi - alpha*(dr x vhat_i)/(pi d)] over d^2 < R_align^2,
+       f = v0*vhat (no pair force).                                                              */
+    SPP_MODEL_CONTINUOUS = 1,
+    /* Single-bead ActiveBrownianChain swimmers plus an optional closed ring membrane
+       (shared/particle.h:792-1057, parallel/beadbox.h:185-196).                                 */
+    SPP_MODEL_ABP = 2
+};
+
+/* ---- integrators (Box::Move / Move_Runge_Kutta2_* / Move_Runge_Kutta4_*) --------------------- */
+enum { SPP_EULER = 1, SPP_RK2 = 2, SPP_RK4 = 4 };
+
+/* ---- pair repulsion laws (shared/force-fields.h) ------------------------------------------- */
+enum {
+    SPP_REP_NONE = 0,
+    SPP_REP_R12_TRUNCATED = 1, /* R12_Repulsive_Truncated, force-fields.h:14-39                   */
+    SPP_REP_HARMONIC = 2,      /* Spring(dr, d, rc, k) gated by d < rc: k (rc - d) dr/d            */
+    SPP_REP_R12 = 3,           /* R12_Repulsive, force-fields.h:41-58 (gated by d < rc)            */
+    SPP_REP_YUKAWA_TRUNCATED = 4, /* Yukawa_Truncated, force-fields.h:60-73                        */
+    SPP_REP_YUKAWA = 5         /* Yukawa, force-fields.h:75-85 (gated by d < rc)                   */
+};
+
+/* ---- cell binning ---------------------------------------------------------------------------- */
+enum {
+    SPP_BIN_EXACT = 0,     /* cell = floor((x + Lx) * ncell_x / (2 Lx))                            */
+    SPP_BIN_REF_QUIRK = 1  /* cell = (int)((int)(x + Lx) * ncell_x / (2 Lx)), serial/box.h:116-117 */
+};
+
+/* ---- neighbour-list rebuild policy ------------------------------------------------------------ */
+enum {
+    SPP_REBUILD_DISPLACEMENT = 0, /* rebuild when any particle moved more than (r_list - r_cut)/2   */
+    SPP_REBUILD_PERIOD = 1        /* rebuild every rebuild_period steps (Multi_Step(cell_update_period)
+                                     then Update_Cells, parallel/box.h:394-439)                   */
+};
+
+/* ---- angular noise source --------------------------------------------------------------------- */
+enum {
+    SPP_NOISE_STREAM = 0,  /* Philox4x32-10 stream addressed by (seed, particle id, step), Box-Muller */
+    SPP_NOISE_INJECTED = 1 /* caller supplies d_theta for each step with spp_inject_noise            */
+};
+
+typedef struct spp_params {
+    int32_t abi_version;    /* must be SPP_ABI_VERSION                                              */
+    int32_t model;          /* SPP_MODEL_*                                                          */
+    int32_t integrator;     /* SPP_EULER / SPP_RK2 / SPP_RK4                                        */
+    int32_t repulsion;      /* SPP_REP_*                                                            */
+    int32_t n;              /* number of particles (ring beads first for SPP_MODEL_ABP)             */
+    int32_t n_membrane;     /* SPP_MODEL_ABP: number of ring beads (0 = no membrane)                */
+    int32_t binning;        /* SPP_BIN_*                                                            */
+    int32_t rebuild;        /* SPP_REBUILD_*                                                        */
+    int32_t rebuild_period; /* steps between rebuilds for SPP_REBUILD_PERIOD                        */
+    int32_t ncell_x;        /* cells along x (reference divisor_x); 0 = floor(2 Lx / r_list)        */
+    int32_t ncell_y;        /* cells along y (reference divisor_y); 0 = floor(2 Ly / r_list)        */
+    int32_t noise_mode;     /* SPP_NOISE_*                                                          */
+    uint64_t seed;          /* noise stream key                                                     */
+    double Lx, Ly;          /* half widths of the periodic box                                      */
+    double dt;              /* time step                                                            */
+    double v0;              /* self-propulsion: f starts at v0 * (cos theta, sin theta)             */
+    double g;               /* alignment coupling                                                   */
+    double R_align;         /* alignment radius                                                     */
+    double alpha;           /* SPP_MODEL_CONTINUOUS self-torque fraction (0 = pure alignment)       */
+    double A_rep;           /* repulsion amplitude (k for SPP_REP_HARMONIC)                         */
+    double sigma_rep;       /* repulsion length scale sigma                                         */
+    double rc_rep;          /* repulsion cutoff radius                                              */
+    double D_rot;           /* rotational diffusion: d_theta ~ N(0, sqrt(2 D_rot dt))               */
+    double r_list;          /* Verlet list radius (reference rv), >= every interaction cutoff       */
+    /* SPP_MODEL_ABP only ---------------------------------------------------------------------- */
+    double mob_par;         /* mobility along the heading (reference m_parallel, 0.5 for nb = 1)   */
+    double mob_perp;        /* mobility across the heading (reference m_perpendicular)             */
+    double mob_rot;         /* rotational mobility (reference k_perpendicular, 1 for nb = 1)       */
+    double torque0;         /* intrinsic torque (reference torque0)                                 */
+    double k_bond;          /* ring bond stiffness                                                  */
+    double l_bond;          /* ring bond rest length                                                */
+    double kappa_bend;      /* ring bending stiffness (0 = none)                                    */
+    double A_mm;            /* membrane-membrane R12 amplitude                                      */
+    double sigma_mm;        /* membrane-membrane R12 sigma (reference sigma_m)                      */
+    double rc_mm;           /* membrane-membrane R12 cutoff (reference repulsion_radius_m)          */
+    int32_t ell_capacity;   /* neighbour slots per particle on the device (0 = default 32)          */
+    int32_t reserved_i[7];
+    double reserved_d[8];
+} spp_params;
+
+typedef struct spp_engine spp_engine;
+
+typedef struct spp_info {
+    int64_t steps;          /* completed steps (also the noise-stream step counter)                 */
+    int64_t rebuilds;       /* neighbour-list rebuilds so far                                       */
+    int64_t rollbacks;      /* chunks re-run because a displacement check failed late              */
+    int64_t host_syncs;     /* host round trips taken by spp_multi_step                            */
+    double t;               /* simulated time                                                       */
+    int32_t ncell_x, ncell_y;
+    int32_t ell_capacity;
+    int32_t max_neighbors;  /* longest full neighbour list at the last rebuild                     */
+    double mean_neighbors;  /* mean full neighbour-list length at the last rebuild                 */
+} spp_info;
+
+/* ---- lifetime ----------------------------------------------------------------------------------- */
+
+/* Fill *p with the defaults of a model (reference statics in shared/particle.h:774-786,
+   1048-1056 and globals in shared/parameters.h:60-99).                                             */
+int spp_params_default(spp_params* p, int32_t model);
+
+/* Box::Box + Box::Init (parallel/box.h:69-151): validates parameters, allocates device state.     */
+int spp_create(const spp_params* p, spp_engine** out);
+void spp_destroy(spp_engine* e);
+/* Text of the last error on this engine (e may be NULL for errors raised by spp_create).          */
+const char* spp_last_error(const spp_engine* e);
+int spp_get_info(spp_engine* e, spp_info* info);
+
+/* ---- state ------------------------------------------------------------------------------------- */
+
+/* Particle positions and headings, host -> device.  xw/yw are wrapped positions (reference r),
+   xu/yu unwrapped (reference r_original), theta the heading.  Any of xu/yu may be NULL: they are
+   then taken equal to xw/yw.  Also rebuilds cells and the neighbour list (Box::Load,
+   parallel/box.h:222-243, which ends in Full_Update_Cells).                                       */
+int spp_set_state(spp_engine* e, const double* xw, const double* yw, const double* xu,
+                  const double* yu, const double* theta, int32_t n);
+/* Device -> host in original order.  Any pointer may be NULL.                                     */
+int spp_get_state(spp_engine* e, double* xw, double* yw, double* xu, double* yu, double* theta,
+                  int32_t n);
+
+/* ---- neighbour structures (Box::Update_Cells + Update_Neighbor_List, serial/box.h:107-179) ---- */
+
+int spp_build_neighbors(spp_engine* e);
+/* Full (symmetric) Verlet list in original ids: offsets[n+1] CSR, idx[nnz].  Order inside a row
+   is unspecified.  Returns SPP_ERR_CAPACITY and sets *nnz when cap is too small.                 */
+int spp_get_neighbors(spp_engine* e, int32_t* offsets, int32_t* idx, int64_t cap, int64_t* nnz);
+/* Cell index of every particle as the last build binned it (original order).                     */
+int spp_get_cells(spp_engine* e, int32_t* cell_x, int32_t* cell_y, int32_t n);
+
+/* ---- per-stage operations (the reference's Box / Particle methods) ----------------------------- */
+
+/* Box::Interact (serial/box.h:322-371): forces and torques at the current stage state.           */
+int spp_interact(spp_engine* e);
+/* f (including the self-propulsion term) and tau of the last spp_interact, original order.       */
+int spp_get_forces(spp_engine* e, double* fx, double* fy, double* tau, int32_t n);
+/* Noise for the next step (original order): replaces Noise_Gen (particle.h:566-569) when
+   noise_mode == SPP_NOISE_INJECTED.                                                              */
+int spp_inject_noise(spp_engine* e, const double* dtheta, int32_t n);
+/* Box::Move (stage 0, Euler), Move_Runge_Kutta2_{1,2} (stage 1..2) or Move_Runge_Kutta4_{1..4}
+   (stage 1..4), using the forces of the last spp_interact.  The last stage of a step advances the
+   step counter.                                                                                   */
+int spp_move_stage(spp_engine* e, int32_t stage);
+/* Box::One_Step (serial/box.h:427-455) through the fused interact+move kernels.                   */
+int spp_one_step(spp_engine* e);
+/* Box::Multi_Step (serial/box.h:469-506): steps through the fused kernels, rebuilding the
+   neighbour list by the configured policy.                                                        */
+int spp_multi_step(spp_engine* e, int32_t steps);
+
+/* ---- observables and snapshots ----------------------------------------------------------------- */
+
+/* Box::Compute_Polarization (serial/box.h:531-538): P = |sum vhat| / N, plus the summed vector.   */
+int spp_polarization(spp_engine* e, double* p, double* sum_x, double* sum_y);
+/* Box::Save / Box::Load (parallel/box.h:222-267) into device-resident slots, including the noise
+   step counter (the counter-based stream makes replay exact).                                    */
+int spp_snapshot(spp_engine* e, int32_t slot);
+int spp_restore(spp_engine* e, int32_t slot);
+
+/* ---- timing -------------------------------------------------------------------------------------- */
+
+int spp_synchronize(spp_engine* e);
+/* Per-kernel HIP-event timing on the engine's stream: enable, run work, read averages.
+   names: '\n'-separated kernel names written into buf (cap bytes).  ms_avg/launches have room
+   for max_kernels entries; *nk receives the number of kernels.                                    */
+int spp_profile_enable(spp_engine* e, int32_t on);
+int spp_profile_read(spp_engine* e, char* names, int32_t cap, double* ms_avg, int64_t* launches,
+                     int32_t max_kernels, int32_t* nk);
+
+/* ---- helpers ------------------------------------------------------------------------------------ */
+
+/* Uniform positions in [-Lx, Lx) x [-Ly, Ly) and headings in [0, 2 pi) from a seed
+   (Random_Formation, shared/set-up.h:152-163, with a counter-based generator).                   */
+int spp_random_state(uint64_t seed, int32_t n, double Lx, double Ly, double* x, double* y,
+                     double* theta);
+/* Standard-normal variate of the angular-noise stream for (seed, particle id, step); the engine's
+   d_theta is sqrt(2 D_rot dt) times this value.                                                   */
+double spp_noise_normal(uint64_t seed, int32_t id, int64_t step);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPP_ENGINE_H */

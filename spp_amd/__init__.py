@@ -1,0 +1,307 @@
+"""spp_amd -- Python binding of the MI355X SPP particle engine's C ABI (include/spp_engine.h).
+
+The engine itself is C++/HIP (spp_amd/csrc); this module only loads the in-tree
+``libspp_engine.so`` with ctypes and exposes it with numpy arrays, for tests and the benchmark.
+There is no fallback: if the library is missing, importing the binding raises.
+
+Method names follow the reference's Box surface (serial/box.h:40-57, parallel/box.h:40-53):
+``interact`` = Box::Interact, ``move_stage`` = Move / Move_Runge_Kutta*_k, ``one_step`` =
+One_Step, ``multi_step`` = Multi_Step, ``snapshot``/``restore`` = Save/Load.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libspp_engine.so")
+
+ABI_VERSION = 1
+MODEL_REPULSIVE, MODEL_CONTINUOUS, MODEL_ABP = 0, 1, 2
+EULER, RK2, RK4 = 1, 2, 4
+REP_NONE, REP_R12_TRUNCATED, REP_HARMONIC, REP_R12, REP_YUKAWA_TRUNCATED, REP_YUKAWA = range(6)
+BIN_EXACT, BIN_REF_QUIRK = 0, 1
+REBUILD_DISPLACEMENT, REBUILD_PERIOD = 0, 1
+NOISE_STREAM, NOISE_INJECTED = 0, 1
+
+OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_CAPACITY, ERR_UNSUPPORTED, ERR_NO_DEVICE = 0, -1, -2, -3, -4, -5, -6
+
+
+class Params(C.Structure):
+    """Mirror of ``spp_params``."""
+
+    _fields_ = [
+        ("abi_version", C.c_int32),
+        ("model", C.c_int32),
+        ("integrator", C.c_int32),
+        ("repulsion", C.c_int32),
+        ("n", C.c_int32),
+        ("n_membrane", C.c_int32),
+        ("binning", C.c_int32),
+        ("rebuild", C.c_int32),
+        ("rebuild_period", C.c_int32),
+        ("ncell_x", C.c_int32),
+        ("ncell_y", C.c_int32),
+        ("noise_mode", C.c_int32),
+        ("seed", C.c_uint64),
+        ("Lx", C.c_double),
+        ("Ly", C.c_double),
+        ("dt", C.c_double),
+        ("v0", C.c_double),
+        ("g", C.c_double),
+        ("R_align", C.c_double),
+        ("alpha", C.c_double),
+        ("A_rep", C.c_double),
+        ("sigma_rep", C.c_double),
+        ("rc_rep", C.c_double),
+        ("D_rot", C.c_double),
+        ("r_list", C.c_double),
+        ("mob_par", C.c_double),
+        ("mob_perp", C.c_double),
+        ("mob_rot", C.c_double),
+        ("torque0", C.c_double),
+        ("k_bond", C.c_double),
+        ("l_bond", C.c_double),
+        ("kappa_bend", C.c_double),
+        ("A_mm", C.c_double),
+        ("sigma_mm", C.c_double),
+        ("rc_mm", C.c_double),
+        ("ell_capacity", C.c_int32),
+        ("reserved_i", C.c_int32 * 7),
+        ("reserved_d", C.c_double * 8),
+    ]
+
+    def update(self, **kw) -> "Params":
+        for k, v in kw.items():
+            if not hasattr(self, k):
+                raise AttributeError(f"spp_params has no field {k!r}")
+            setattr(self, k, v)
+        return self
+
+    def copy(self) -> "Params":
+        p = Params()
+        C.memmove(C.byref(p), C.byref(self), C.sizeof(Params))
+        return p
+
+
+class Info(C.Structure):
+    """Mirror of ``spp_info``."""
+
+    _fields_ = [
+        ("steps", C.c_int64),
+        ("rebuilds", C.c_int64),
+        ("rollbacks", C.c_int64),
+        ("host_syncs", C.c_int64),
+        ("t", C.c_double),
+        ("ncell_x", C.c_int32),
+        ("ncell_y", C.c_int32),
+        ("ell_capacity", C.c_int32),
+        ("max_neighbors", C.c_int32),
+        ("mean_neighbors", C.c_double),
+    ]
+
+
+# Exported symbols and their signatures (every declaration of include/spp_engine.h).
+_D = C.POINTER(C.c_double)
+_I32 = C.POINTER(C.c_int32)
+_I64 = C.POINTER(C.c_int64)
+SIGNATURES = {
+    "spp_params_default": (C.c_int, [C.POINTER(Params), C.c_int32]),
+    "spp_create": (C.c_int, [C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    "spp_destroy": (None, [C.c_void_p]),
+    "spp_last_error": (C.c_char_p, [C.c_void_p]),
+    "spp_get_info": (C.c_int, [C.c_void_p, C.POINTER(Info)]),
+    "spp_set_state": (C.c_int, [C.c_void_p, _D, _D, _D, _D, _D, C.c_int32]),
+    "spp_get_state": (C.c_int, [C.c_void_p, _D, _D, _D, _D, _D, C.c_int32]),
+    "spp_build_neighbors": (C.c_int, [C.c_void_p]),
+    "spp_get_neighbors": (C.c_int, [C.c_void_p, _I32, _I32, C.c_int64, _I64]),
+    "spp_get_cells": (C.c_int, [C.c_void_p, _I32, _I32, C.c_int32]),
+    "spp_interact": (C.c_int, [C.c_void_p]),
+    "spp_get_forces": (C.c_int, [C.c_void_p, _D, _D, _D, C.c_int32]),
+    "spp_inject_noise": (C.c_int, [C.c_void_p, _D, C.c_int32]),
+    "spp_move_stage": (C.c_int, [C.c_void_p, C.c_int32]),
+    "spp_one_step": (C.c_int, [C.c_void_p]),
+    "spp_multi_step": (C.c_int, [C.c_void_p, C.c_int32]),
+    "spp_polarization": (C.c_int, [C.c_void_p, _D, _D, _D]),
+    "spp_snapshot": (C.c_int, [C.c_void_p, C.c_int32]),
+    "spp_restore": (C.c_int, [C.c_void_p, C.c_int32]),
+    "spp_synchronize": (C.c_int, [C.c_void_p]),
+    "spp_profile_enable": (C.c_int, [C.c_void_p, C.c_int32]),
+    "spp_profile_read": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, _D, _I64, C.c_int32, _I32]),
+    "spp_random_state": (C.c_int, [C.c_uint64, C.c_int32, C.c_double, C.c_double, _D, _D, _D]),
+    "spp_noise_normal": (C.c_double, [C.c_uint64, C.c_int32, C.c_int64]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load the in-tree engine library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+            )
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class SppError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"spp error {code}: {msg}")
+        self.code = code
+
+
+def _ptr(a, t=_D):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def default_params(model: int = MODEL_REPULSIVE, **kw) -> Params:
+    p = Params()
+    rc = lib().spp_params_default(C.byref(p), model)
+    if rc:
+        raise SppError(rc, "spp_params_default")
+    return p.update(**kw)
+
+
+def random_state(seed: int, n: int, Lx: float, Ly: float):
+    x = np.empty(n)
+    y = np.empty(n)
+    th = np.empty(n)
+    rc = lib().spp_random_state(seed, n, Lx, Ly, _ptr(x), _ptr(y), _ptr(th))
+    if rc:
+        raise SppError(rc, "spp_random_state")
+    return x, y, th
+
+
+def noise_normal(seed: int, pid: int, step: int) -> float:
+    return lib().spp_noise_normal(seed, pid, step)
+
+
+class Engine:
+    """One engine (device state + stream) for one box of particles."""
+
+    def __init__(self, params: Params):
+        self._lib = lib()
+        self.params = params.copy()
+        h = C.c_void_p()
+        rc = self._lib.spp_create(C.byref(self.params), C.byref(h))
+        if rc:
+            raise SppError(rc, self._lib.spp_last_error(None).decode())
+        self._h = h
+        self.n = params.n
+
+    # -- plumbing ------------------------------------------------------------------------------
+    def _check(self, rc: int):
+        if rc:
+            raise SppError(rc, self._lib.spp_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.spp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- state ---------------------------------------------------------------------------------
+    def set_state(self, xw, yw, theta, xu=None, yu=None):
+        f = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        xw, yw, theta, xu, yu = f(xw), f(yw), f(theta), f(xu), f(yu)
+        self._check(self._lib.spp_set_state(self._h, _ptr(xw), _ptr(yw), _ptr(xu), _ptr(yu), _ptr(theta), self.n))
+
+    def get_state(self):
+        a = [np.empty(self.n) for _ in range(5)]
+        self._check(self._lib.spp_get_state(self._h, *[_ptr(x) for x in a], self.n))
+        return dict(zip(("xw", "yw", "xu", "yu", "theta"), a))
+
+    def info(self) -> Info:
+        i = Info()
+        self._check(self._lib.spp_get_info(self._h, C.byref(i)))
+        return i
+
+    # -- neighbours ------------------------------------------------------------------------------
+    def build_neighbors(self):
+        self._check(self._lib.spp_build_neighbors(self._h))
+
+    def get_neighbors(self):
+        nnz = C.c_int64(0)
+        rc = self._lib.spp_get_neighbors(self._h, None, None, 0, C.byref(nnz))
+        if rc not in (OK, ERR_CAPACITY):
+            self._check(rc)
+        off = np.empty(self.n + 1, dtype=np.int32)
+        idx = np.empty(max(nnz.value, 1), dtype=np.int32)
+        self._check(self._lib.spp_get_neighbors(self._h, _ptr(off, _I32), _ptr(idx, _I32), idx.size, C.byref(nnz)))
+        return off, idx[: nnz.value]
+
+    def get_cells(self):
+        cx = np.empty(self.n, dtype=np.int32)
+        cy = np.empty(self.n, dtype=np.int32)
+        self._check(self._lib.spp_get_cells(self._h, _ptr(cx, _I32), _ptr(cy, _I32), self.n))
+        return cx, cy
+
+    # -- Box surface ---------------------------------------------------------------------------
+    def interact(self):
+        self._check(self._lib.spp_interact(self._h))
+
+    def get_forces(self):
+        a = [np.empty(self.n) for _ in range(3)]
+        self._check(self._lib.spp_get_forces(self._h, *[_ptr(x) for x in a], self.n))
+        return a
+
+    def inject_noise(self, dtheta):
+        d = np.ascontiguousarray(dtheta, dtype=np.float64)
+        self._check(self._lib.spp_inject_noise(self._h, _ptr(d), self.n))
+
+    def move_stage(self, stage: int):
+        self._check(self._lib.spp_move_stage(self._h, stage))
+
+    def one_step(self):
+        self._check(self._lib.spp_one_step(self._h))
+
+    def multi_step(self, steps: int):
+        self._check(self._lib.spp_multi_step(self._h, steps))
+
+    def polarization(self):
+        p, sx, sy = C.c_double(), C.c_double(), C.c_double()
+        self._check(self._lib.spp_polarization(self._h, C.byref(p), C.byref(sx), C.byref(sy)))
+        return p.value, sx.value, sy.value
+
+    def snapshot(self, slot: int = 0):
+        self._check(self._lib.spp_snapshot(self._h, slot))
+
+    def restore(self, slot: int = 0):
+        self._check(self._lib.spp_restore(self._h, slot))
+
+    def synchronize(self):
+        self._check(self._lib.spp_synchronize(self._h))
+
+    # -- timing --------------------------------------------------------------------------------
+    def profile(self, on: bool):
+        self._check(self._lib.spp_profile_enable(self._h, 1 if on else 0))
+
+    def profile_read(self):
+        buf = C.create_string_buffer(4096)
+        ms = np.zeros(64)
+        cnt = np.zeros(64, dtype=np.int64)
+        nk = C.c_int32(0)
+        self._check(self._lib.spp_profile_read(self._h, buf, 4096, _ptr(ms), _ptr(cnt, _I64), 64, C.byref(nk)))
+        names = buf.value.decode().split("\n")[: nk.value]
+        return {n: (float(ms[i]), int(cnt[i])) for i, n in enumerate(names)}

@@ -1,0 +1,64 @@
+"""C-ABI checks that need no GPU: the library loads, exports every declared entry point, and its
+host-side helpers agree with the oracle bit for bit."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import spp_amd as S
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "spp_engine.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\*?\s+\*?(spp_[a-z_0-9]+)\s*\(", src, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_declares_what_binding_binds():
+    assert set(declared_functions()) == set(S.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol(engine_mod):
+    L = engine_mod.lib()
+    for name in declared_functions():
+        assert hasattr(L, name), name
+
+
+def test_params_default_and_abi_version(engine_mod):
+    p = engine_mod.default_params(S.MODEL_REPULSIVE)
+    assert p.abi_version == S.ABI_VERSION
+    # reference globals and statics (shared/parameters.h:60-99, particle.h:774-786)
+    assert p.Lx == 64 and p.dt == 1.0 / 1024 and p.r_list == 1.0625
+    assert p.A_rep == 2 and p.sigma_rep == 1 and p.rc_rep == 1.2 and p.R_align == 1.1
+    c = engine_mod.default_params(S.MODEL_CONTINUOUS)
+    assert c.R_align == 1 and c.g == 4 and c.repulsion == S.REP_NONE
+
+
+def test_create_rejects_bad_parameters(engine_mod):
+    p = engine_mod.default_params(S.MODEL_REPULSIVE, n=100, Lx=2.0, Ly=2.0, r_list=1.5)
+    with pytest.raises(S.SppError) as ei:
+        S.Engine(p)  # two cells per side: rejected before any device call
+    assert ei.value.code == S.ERR_ARG
+    p = engine_mod.default_params(S.MODEL_REPULSIVE, n=100, r_list=0.5)
+    with pytest.raises(S.SppError):
+        S.Engine(p)  # list radius below the cutoff
+    p = engine_mod.default_params(S.MODEL_REPULSIVE, n=100)
+    p.abi_version = 99
+    with pytest.raises(S.SppError):
+        S.Engine(p)
+
+
+def test_host_noise_matches_oracle(engine_mod, oracle_mod):
+    for seed, i, step in [(1, 0, 0), (10, 12345, 678), (2**40 + 3, 262143, 9999), (7, 5, 2**33)]:
+        assert S.noise_normal(seed, i, step) == oracle_mod.noise_normal(seed, i, step)
+
+
+def test_host_random_state_matches_oracle(engine_mod, oracle_mod):
+    a = S.random_state(12, 1000, 32.0, 16.0)
+    b = oracle_mod.random_state(12, 1000, 32.0, 16.0)
+    for u, v in zip(a, b):
+        assert np.array_equal(u, v)
