@@ -1,0 +1,182 @@
+// box.hpp -- C++ host facade with the reference's Box call surface over the engine's C ABI.
+//
+// The reference drives its particle path through Box methods (serial/box.h:40-57,
+// parallel/box.h:40-53): Interact, Move / Move_Runge_Kutta{2,4}_k, One_Step, Multi_Step,
+// Update_Cells, Save / Load, Save_Polarization and the binary frame operators.  This class keeps
+// those names and meanings; every call forwards to include/spp_engine.h, which runs the HIP
+// kernels.  Errors throw spp_gpu::Error (the reference prints and calls exit(0)).
+#pragma once
+
+#include <cmath>
+#include <fstream>
+#include <istream>
+#include <ostream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/spp_engine.h"
+#include "trajectory.hpp"
+
+namespace spp_gpu {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+class Box {
+public:
+    double t = 0;             // simulated time (reference Box::t)
+    int N = 0, Ns = 0, Nm = 0;
+    double polarization = 0;  // last Compute_Polarization
+
+    explicit Box(const spp_params& p) : params_(p)
+    {
+        check(spp_create(&params_, &e_), nullptr);
+        N = p.n;
+        Nm = p.n_membrane;
+        Ns = N - Nm;
+    }
+    ~Box() { spp_destroy(e_); }
+    Box(const Box&) = delete;
+    Box& operator=(const Box&) = delete;
+
+    spp_engine* engine() { return e_; }
+    const spp_params& params() const { return params_; }
+
+    // Box::Init with given positions and headings (the reference places them with
+    // Random_Formation / Triangle_Lattice_Formation, shared/set-up.h); unwrapped = wrapped.
+    void Init(const std::vector<double>& x, const std::vector<double>& y, const std::vector<double>& theta)
+    {
+        check(spp_set_state(e_, x.data(), y.data(), nullptr, nullptr, theta.data(), N));
+        set_time_origin(0.0);
+    }
+
+    // Box::Positioning_Particles (serial/box.h:211-272): resume from the last healthy frame of a
+    // trajectory file (truncated tails and unhealthy frames are skipped, as fix-file does).
+    // Returns false if the file holds no usable frame.
+    bool Positioning_Particles(const std::string& name)
+    {
+        std::ifstream is(name, std::ios::binary);
+        if (!is) return false;
+        auto frames = spp_traj::read_healthy_frames(is, true);
+        if (frames.empty()) return false;
+        const spp_traj::Frame& f = frames.back();
+        if (f.Ns() + f.Nm() != N) return false;
+        std::vector<double> xu(N), yu(N), xw(N), yw(N), th(N, 0.0);
+        for (int i = 0; i < f.Nm(); ++i) {
+            xu[i] = f.mx[i];
+            yu[i] = f.my[i];
+        }
+        for (int i = 0; i < f.Ns(); ++i) {
+            xu[f.Nm() + i] = f.x[i];
+            yu[f.Nm() + i] = f.y[i];
+            th[f.Nm() + i] = f.theta.empty() ? 0.0 : f.theta[i];
+        }
+        for (int i = 0; i < N; ++i) {  // r = r_original; r.Periodic_Transform() (:245-258)
+            xw[i] = xu[i] - 2 * params_.Lx * (double)(int)std::floor(xu[i] / (2 * params_.Lx) + 0.5);
+            yw[i] = yu[i] - 2 * params_.Ly * (double)(int)std::floor(yu[i] / (2 * params_.Ly) + 0.5);
+        }
+        check(spp_set_state(e_, xw.data(), yw.data(), xu.data(), yu.data(), th.data(), N));
+        set_time_origin(f.t);
+        return true;
+    }
+
+    void Update_Cells() { check(spp_build_neighbors(e_)); }        // serial/box.h:107-134
+    void Update_Neighbor_List() { check(spp_build_neighbors(e_)); } // serial/box.h:137-179
+    void Interact() { check(spp_interact(e_)); }                    // serial/box.h:322-371
+    void Move() { stage(0); }                                       // :374-378
+    void Move_Runge_Kutta2_1() { stage(1); }                        // :382-386
+    void Move_Runge_Kutta2_2() { stage(2); }                        // :389-393
+    void Move_Runge_Kutta4_1() { stage(1); }                        // :398-402
+    void Move_Runge_Kutta4_2() { stage(2); }
+    void Move_Runge_Kutta4_3() { stage(3); }
+    void Move_Runge_Kutta4_4() { stage(4); }                        // :419-423
+    void One_Step() { check(spp_one_step(e_)); sync_time(); }       // :427-455
+    void Multi_Step(int steps) { check(spp_multi_step(e_, steps)); sync_time(); }  // :469-506
+    // :509-515 runs steps/interval chunks then the remainder
+    void Multi_Step(int steps, int interval)
+    {
+        for (int i = 0; interval > 0 && i < steps / interval; ++i) Multi_Step(interval);
+        Multi_Step(interval > 0 ? steps % interval : steps);
+    }
+
+    void Save(int slot) { check(spp_snapshot(e_, slot)); }          // parallel/box.h:246-267
+    void Load(int slot) { check(spp_restore(e_, slot)); sync_time(); }  // :222-243
+
+    void Compute_Polarization()                                     // serial/box.h:531-538
+    {
+        check(spp_polarization(e_, &polarization, nullptr, nullptr));
+    }
+    void Save_Polarization(std::ostream& os)                        // :541-546
+    {
+        Compute_Polarization();
+        os << t << "\t" << polarization << std::endl;
+    }
+
+    // Current state in caller order.
+    void Get_State(std::vector<double>& xw, std::vector<double>& yw, std::vector<double>& xu,
+                   std::vector<double>& yu, std::vector<double>& th)
+    {
+        xw.resize(N);
+        yw.resize(N);
+        xu.resize(N);
+        yu.resize(N);
+        th.resize(N);
+        check(spp_get_state(e_, xw.data(), yw.data(), xu.data(), yu.data(), th.data(), N));
+    }
+
+    // operator<<(ostream&, Box*), serial/box.h:549-568: one binary frame.
+    friend std::ostream& operator<<(std::ostream& os, Box* box)
+    {
+        std::vector<double> xw, yw, xu, yu, th;
+        box->Get_State(xw, yw, xu, yu, th);
+        spp_traj::Frame f;
+        f.t = box->t;
+        f.Lx = box->params_.Lx;
+        f.Ly = box->params_.Ly;
+        f.nb = 1;
+        // continuous particles write the wrapped r (BasicDynamicParticle::Write), the others
+        // r_original (RepulsiveParticle::Write, ActiveBrownianChain::Write)
+        bool wrapped = box->params_.model == SPP_MODEL_CONTINUOUS;
+        for (int i = 0; i < box->Nm; ++i) {
+            f.mx.push_back((float)xu[i]);
+            f.my.push_back((float)yu[i]);
+        }
+        for (int i = box->Nm; i < box->N; ++i) {
+            f.x.push_back((float)(wrapped ? xw[i] : xu[i]));
+            f.y.push_back((float)(wrapped ? yw[i] : yu[i]));
+            f.theta.push_back((float)th[i]);
+        }
+        spp_traj::write_frame(os, f);
+        return os;
+    }
+
+private:
+    void stage(int s) { check(spp_move_stage(e_, s)); sync_time(); }
+    void set_time_origin(double t0)
+    {
+        spp_info inf;
+        check(spp_get_info(e_, &inf));
+        t0_ = t0;
+        base_ = inf.t;
+        t = t0;
+    }
+    void sync_time()
+    {
+        spp_info inf;
+        if (spp_get_info(e_, &inf) == SPP_OK) t = t0_ + (inf.t - base_);
+    }
+    void check(int rc, spp_engine* e) const
+    {
+        if (rc != SPP_OK) throw Error(rc, spp_last_error(e));
+    }
+    void check(int rc) const { check(rc, e_); }
+
+    spp_params params_;
+    spp_engine* e_ = nullptr;
+    double t0_ = 0, base_ = 0;
+};
+
+}  // namespace spp_gpu

@@ -1,17 +1,24 @@
 // engine.hip -- MI355X (gfx950) engine for the per-time-step SPP particle path, behind the C ABI
 // declared in include/spp_engine.h.
 //
-// Device layout (all arrays in CELL-SORTED slot order; id[slot] is the caller's particle index):
-//   pos0  P4[N]      state being advanced: wrapped x, y and theta (reference r, theta)
-//   pos1/pos2 P4[N]  RK stage states (pos1 = RK2 midpoint)
-//   U     double2[N] unwrapped positions (reference r_original)
-//   dth   double[N]  this step's angular noise
-//   kacc  P4[N]      RK4 accumulator k1 + 2 k2 + 2 k3 (fx, fy, tau)
-//   nbr   int[cap*N] full Verlet list, ELL, slot-major (nbr[k*N + i]) so a wavefront's k-th
-//                    neighbour ids are one coalesced 256-B read
-//   build float2[N]  positions at the last build, for the displacement trigger
-// Cells are keyed in 4x4-cell tiles so consecutive slots cover compact 2-D patches and a
-// wavefront's neighbour gathers stay inside a few 128-B lines of L2.
+// Device layout (all per-particle arrays in CELL-SORTED slot order; id[slot] = caller's index):
+//   state set (x3)  pos P4[N] (x, y, cos th, sin th), th double[N], U double2[N] (unwrapped
+//                   positions, reference r_original), id int[N]
+//   pos1/th1, pos2/th2  RK stage states (pos1 = RK2 midpoint)
+//   dth    double[N]   this step's angular noise
+//   kacc   P4[N]       RK4 accumulator k1 + 2 k2 + 2 k3 (fx, fy, tau)
+//   nbr    int[cap*N]  full Verlet list, ELL, slot-major (nbr[k*N + i]): a wavefront's k-th
+//                      neighbour ids are one coalesced 256-B read
+//   build  float2[N]   positions at the last build, for the displacement trigger
+// Cells are row-major (key = cy*ncx + cx), so each stencil row of three cells is one contiguous
+// slot range, and each XCD (a contiguous 1/8 of the slots) owns a horizontal band of the box.
+//
+// Step scheduling (displacement-triggered rebuilds): work is enqueued in intervals, each a
+// rebuild followed by m fused steps.  The kernels flag the first step whose input moved farther
+// than the half skin; the host checks an interval's flags while the next one already runs, so
+// the GPU never waits for the host.  The state at an interval's start is kept untouched (its
+// first step writes into another of three state sets), so a late flag rolls back by switching
+// sets and replaying only the valid steps -- the counter-based noise makes the replay exact.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -20,6 +27,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <string>
 #include <vector>
@@ -34,14 +42,20 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kScanItems = 4;  // items per thread in the cell-count scan
 constexpr int kScanTile = kBlock * kScanItems;
-constexpr int kTileBits = 2;   // 4x4 cells per key tile
+
+struct Flags {
+    int bad_step;  // first step index (within the interval) whose stage input left the skin
+    int max_len;   // longest full neighbour list of the interval's rebuild
+    int nnz;       // total entries of that list
+    int pad;
+};
 
 // ------------------------------------------------------------------------------------------------
 // Small device utilities
 // ------------------------------------------------------------------------------------------------
 
 // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of
-// slots (a contiguous region of space) so its L2 keeps that region's particles.
+// slots (a band of the box) so its L2 keeps that band's particles.
 __device__ __forceinline__ int xcd_block()
 {
     int nblk = gridDim.x, b = blockIdx.x;
@@ -59,15 +73,9 @@ __device__ __forceinline__ int cell_coord(double x, double L, double L2, int nc,
     return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
 }
 
-__device__ __forceinline__ int cell_key(int cx, int cy, int ntx)
-{
-    return (((cy >> kTileBits) * ntx + (cx >> kTileBits)) << (2 * kTileBits)) +
-           ((cy & ((1 << kTileBits) - 1)) << kTileBits) + (cx & ((1 << kTileBits) - 1));
-}
-
 struct Grid {
-    double Lx, Ly, Lx2, Ly2;
-    int ncx, ncy, ntx, quirk;
+    double Lx, Ly, Lx2, Ly2, invLx2, invLy2;
+    int ncx, ncy, quirk;
 };
 
 template <typename T>
@@ -97,14 +105,20 @@ __device__ __forceinline__ double wave_max(double v)
 // ------------------------------------------------------------------------------------------------
 
 __global__ __launch_bounds__(kBlock) void k_bin(int N, Grid G, const P4* __restrict__ pos,
-                                                int* __restrict__ key, int* __restrict__ count)
+                                                int* __restrict__ key, int* __restrict__ count,
+                                                Flags* flags)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && flags) {
+        flags->bad_step = INT_MAX;
+        flags->max_len = 0;
+        flags->nnz = 0;
+    }
     if (i >= N) return;
     P4 p = pos[i];
     int cx = cell_coord(p.x, G.Lx, G.Lx2, G.ncx, G.quirk);
     int cy = cell_coord(p.y, G.Ly, G.Ly2, G.ncy, G.quirk);
-    int k = cell_key(cx, cy, G.ntx);
+    int k = cy * G.ncx + cx;
     key[i] = k;
     atomicAdd(&count[k], 1);
 }
@@ -160,7 +174,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_partial(const int* __restrict__
 }
 
 // out[0..n] = exclusive scan of in[0..n), out[n] = total.  Each block first sums the partial
-// sums of the blocks before it (there are few: n / 1024).
+// sums of the blocks before it (few: n / 1024).
 __global__ __launch_bounds__(kBlock) void k_scan_final(const int* __restrict__ in, int n,
                                                        const int* __restrict__ block_sums,
                                                        int* __restrict__ out)
@@ -186,18 +200,20 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(const int* __restrict__ i
     if (b == gridDim.x - 1 && threadIdx.x == 0) out[n] = prefix + total;
 }
 
+// Counting-sort scatter.  Counting down returns every cell count to zero, ready for the next
+// rebuild without a clearing pass.
 __global__ __launch_bounds__(kBlock) void k_scatter(int N, const int* __restrict__ key,
                                                     const int* __restrict__ start,
-                                                    int* __restrict__ cursor, int* __restrict__ perm)
+                                                    int* __restrict__ count, int* __restrict__ perm)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     int k = key[i];
-    perm[start[k] + atomicAdd(&cursor[k], 1)] = i;
+    perm[start[k] + atomicSub(&count[k], 1) - 1] = i;
 }
 
 // Order each cell's particles by caller index, so the sorted layout is a pure function of the
-// positions (the atomic scatter leaves it in arrival order).
+// positions (the atomic scatter leaves arrival order).
 __global__ __launch_bounds__(kBlock) void k_cell_sort(int nkeys, const int* __restrict__ start,
                                                       int* __restrict__ perm,
                                                       const int* __restrict__ id)
@@ -216,16 +232,18 @@ __global__ __launch_bounds__(kBlock) void k_cell_sort(int nkeys, const int* __re
     }
 }
 
+struct SetPtrs {
+    P4* pos;
+    double* th;
+    double2* U;
+    int* id;
+};
+
 struct PermuteArgs {
     int N;
     const int* perm;
-    const P4* pos_in;
-    P4* pos_out;
-    const double2* U_in;
-    double2* U_out;
-    const int* id_in;
-    int* id_out;
-    // per-stage buffers of the unfused path (null when the permutation happens between steps)
+    SetPtrs in, out;
+    // per-stage buffers of the split-call path (null when permuting between steps)
     const P4* a_in[3];
     P4* a_out[3];
     const double* d_in;
@@ -237,9 +255,10 @@ __global__ __launch_bounds__(kBlock) void k_permute(PermuteArgs a)
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.N) return;
     int o = a.perm[k];
-    a.pos_out[k] = a.pos_in[o];
-    a.U_out[k] = a.U_in[o];
-    a.id_out[k] = a.id_in[o];
+    a.out.pos[k] = a.in.pos[o];
+    a.out.th[k] = a.in.th[o];
+    a.out.U[k] = a.in.U[o];
+    a.out.id[k] = a.in.id[o];
     if (a.d_in) {
         a.d_out[k] = a.d_in[o];
 #pragma unroll
@@ -250,18 +269,19 @@ __global__ __launch_bounds__(kBlock) void k_permute(PermuteArgs a)
 struct NlistArgs {
     int N, cap;
     Grid G;
-    double rv;
+    double T_list;  // sqrt(d2) < r_list  <=>  d2 < T_list
     const P4* pos;
     const int* start;
     int* nbr;
     int* ncount;
     float2* build;
     int* cellxy;
-    int* stats;  // [0] max list length, [1] total entries (low 31 bits are enough per build)
+    Flags* flags;
 };
 
 // Cell::Neighbor_List (shared/cell.h:74-103) over the 3x3 periodic stencil, full list, predicate
-// sqrt(d2) < rv exactly as cell.h:84-85.
+// sqrt(d2) < rv (cell.h:84-85) evaluated as d2 < T(rv).  Each stencil row is scanned as one or
+// two contiguous slot ranges.
 __global__ __launch_bounds__(kBlock) void k_nlist(NlistArgs a)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -272,21 +292,36 @@ __global__ __launch_bounds__(kBlock) void k_nlist(NlistArgs a)
         P4 me = a.pos[i];
         int cx = cell_coord(me.x, G.Lx, G.Lx2, G.ncx, G.quirk);
         int cy = cell_coord(me.y, G.Ly, G.Ly2, G.ncy, G.quirk);
+        // x ranges of the three stencil columns: [x0, x1] plus an optional wrapped cell xw
+        int x0 = cx - 1, x1 = cx + 1, xw = -1;
+        if (x0 < 0) {
+            xw = G.ncx - 1;
+            x0 = 0;
+        } else if (x1 >= G.ncx) {
+            xw = 0;
+            x1 = G.ncx - 1;
+        }
         for (int oy = -1; oy <= 1; ++oy) {
             int y = cy + oy;
             y = y < 0 ? y + G.ncy : (y >= G.ncy ? y - G.ncy : y);
-            for (int ox = -1; ox <= 1; ++ox) {
-                int x = cx + ox;
-                x = x < 0 ? x + G.ncx : (x >= G.ncx ? x - G.ncx : x);
-                int k = cell_key(x, y, G.ntx);
-                int s = a.start[k], e = a.start[k + 1];
+            int row = y * G.ncx;
+            for (int seg = 0; seg < 2; ++seg) {
+                int s, e;
+                if (seg == 0) {
+                    s = a.start[row + x0];
+                    e = a.start[row + x1 + 1];
+                } else {
+                    if (xw < 0) break;
+                    s = a.start[row + xw];
+                    e = a.start[row + xw + 1];
+                }
                 for (int j = s; j < e; ++j) {
                     if (j == i) continue;
                     P4 pj = a.pos[j];
-                    double dx = spp::min_image(me.x - pj.x, G.Lx, G.Lx2);
-                    double dy = spp::min_image(me.y - pj.y, G.Ly, G.Ly2);
-                    double d = sqrt(dx * dx + dy * dy);
-                    if (d < a.rv) {
+                    double dx = spp::min_image(me.x - pj.x, G.Lx2, G.invLx2);
+                    double dy = spp::min_image(me.y - pj.y, G.Ly2, G.invLy2);
+                    double d2 = dx * dx + dy * dy;
+                    if (d2 < a.T_list) {
                         if (c < a.cap) a.nbr[(size_t)c * a.N + i] = j;
                         ++c;
                     }
@@ -302,8 +337,8 @@ __global__ __launch_bounds__(kBlock) void k_nlist(NlistArgs a)
     for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
     int t = wave_sum(c);
     if ((threadIdx.x & 63) == 0) {
-        atomicMax(&a.stats[0], m);
-        atomicAdd(&a.stats[1], t);
+        atomicMax(&a.flags->max_len, m);
+        atomicAdd(&a.flags->nnz, t);
     }
 }
 
@@ -313,59 +348,68 @@ __global__ __launch_bounds__(kBlock) void k_nlist(NlistArgs a)
 
 struct StageArgs {
     int N;
-    int cap;
     Phys P;
-    const P4* gather;   // stage input state (also this particle's own stage state)
-    P4* out;            // stage output state
-    const P4* pos0;     // step-start state (theta_old) for RK stages >= 2
-    double2* U;         // unwrapped positions
-    double* dth;        // angular noise of this step
-    P4* kacc;           // RK4 accumulator
+    const P4* gather;      // stage input: neighbours' (and own) position and heading
+    const double* th_in;   // own heading angle of the stage input
+    P4* out;               // stage output
+    double* th_out;
+    const double* th0;     // step-start heading (theta_old) for RK stages >= 2
+    const double2* U_in;   // step-start unwrapped position
+    double2* U_out;        // written by the final stage
+    double* dth;           // angular noise of this step
+    P4* kacc;              // RK4 accumulator
     const int* nbr;
     const int* ncount;
     const int* id;
-    const double* inj;  // injected noise, slot order (null = stream)
+    int* id_out;           // first step of an interval: the output set's copy of id
+    const double* inj;     // injected noise, slot order (null = stream)
     const float2* build;
-    int64_t step;       // noise-stream step
-    int kstep;          // step index inside the current chunk
-    double thresh2;     // displacement trigger squared
-    int* bad_step;
-    unsigned long long* maxdisp;
-    int record;         // record the max displacement (last step of a chunk)
+    int64_t step;          // noise-stream step
+    int kstep;             // step index inside the current interval
+    double thresh2;        // displacement trigger, squared
+    Flags* flags;
+    double* blockmax;      // per-block max displacement^2 (last step of an interval)
+    int record;
 };
 
 template <int MODEL, int REP>
-__device__ __forceinline__ void gather_forces(const StageArgs& a, int i, const P4& me, double vix,
-                                              double viy, double& fx, double& fy, double& tq)
+__device__ __forceinline__ void gather_forces(const StageArgs& a, int i, const P4& me, double& fx,
+                                              double& fy, double& tq)
 {
     double s1 = 0, s2 = 0;
-    fx = vix * a.P.v0;  // Reset: f = v (speed v0), particle.h:559-564
-    fy = viy * a.P.v0;
+    fx = me.c * a.P.v0;  // Reset: f = v (speed v0), particle.h:559-564
+    fy = me.s * a.P.v0;
     int cnt = a.ncount[i];
     const int* nb = a.nbr + i;
     for (int k = 0; k < cnt; ++k) {
         int j = nb[(size_t)k * a.N];
         P4 pj = a.gather[j];
-        spp::pair_accumulate<MODEL, REP>(a.P, me.x, me.y, me.th, vix, viy, pj, fx, fy, s1, s2);
+        spp::pair_accumulate<MODEL, REP>(a.P, me, pj, fx, fy, s1, s2);
     }
     tq = spp::drift_torque<MODEL>(a.P, s1, s2);
 }
 
-__device__ __forceinline__ void check_disp(const StageArgs& a, int i, bool live, double x,
+__device__ __forceinline__ void check_disp(const StageArgs& a, bool live, int i, double x,
                                            double y, int mark)
 {
+    __shared__ double red[kBlock / 64];
     double d2 = 0;
     if (live) {
         float2 b = a.build[i];
-        double dx = spp::min_image(x - (double)b.x, a.P.Lx, a.P.Lx2);
-        double dy = spp::min_image(y - (double)b.y, a.P.Ly, a.P.Ly2);
+        double dx = spp::min_image(x - (double)b.x, a.P.Lx2, a.P.invLx2);
+        double dy = spp::min_image(y - (double)b.y, a.P.Ly2, a.P.invLy2);
         d2 = dx * dx + dy * dy;
-        if (d2 > a.thresh2) atomicMin(a.bad_step, mark);
     }
+    if (__any(d2 > a.thresh2) && (threadIdx.x & 63) == 0) atomicMin(&a.flags->bad_step, mark);
     if (a.record) {
         double m = wave_max(d2);
-        if ((threadIdx.x & 63) == 0)
-            atomicMax(a.maxdisp, (unsigned long long)__double_as_longlong(m));
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double r = red[0];
+            for (int k = 1; k < kBlock / 64; ++k) r = fmax(r, red[k]);
+            a.blockmax[blockIdx.x] = r;
+        }
     }
 }
 
@@ -380,24 +424,26 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageArgs a)
     if (live) {
         const Phys& P = a.P;
         P4 me = a.gather[i];
-        double vix = cos(me.th), viy = sin(me.th);
         double fx, fy, tq;
-        gather_forces<MODEL, REP>(a, i, me, vix, viy, fx, fy, tq);
+        gather_forces<MODEL, REP>(a, i, me, fx, fy, tq);
         double th;
         if (STAGE <= 1) {
-            double dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)a.id[i], a.step);
+            int pid = a.id[i];
+            if (a.id_out) a.id_out[i] = pid;
+            double dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)pid, a.step);
+            double th_in = a.th_in[i];
             if (INTEG == SPP_EULER) {
-                th = me.th + P.dt * (tq);
+                th = th_in + P.dt * (tq);
                 th += dn;
-                double2 u = a.U[i];
+                double2 u = a.U_in[i];
                 u.x += fx * P.dt;
                 u.y += fy * P.dt;
-                a.U[i] = u;
+                a.U_out[i] = u;
                 x = spp::periodic_wrap(u.x, P.Lx2);
                 y = spp::periodic_wrap(u.y, P.Ly2);
             } else {
                 a.dth[i] = dn;
-                th = me.th;
+                th = th_in;
                 th += P.half_dt * tq;
                 th += dn / 2;
                 th = spp::wrap_angle(th);
@@ -406,13 +452,13 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageArgs a)
                 if (INTEG == SPP_RK4) a.kacc[i] = P4{fx, fy, tq, 0.0};
             }
         } else {
-            double thold = a.pos0[i].th;
+            double thold = a.th0[i];
             double dn = a.dth[i];
-            double2 u = a.U[i];
+            double2 u = a.U_in[i];
             double h = (INTEG == SPP_RK4 && STAGE == 2) ? P.half_dt : P.dt;
             if (INTEG == SPP_RK4 && STAGE == 4) {
                 P4 k = a.kacc[i];
-                th = thold + (k.th + tq) * P.dt6;
+                th = thold + (k.c + tq) * P.dt6;
                 th += dn;
                 u.x = u.x + (k.x + fx) * P.dt6;
                 u.y = u.y + (k.y + fy) * P.dt6;
@@ -423,23 +469,26 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageArgs a)
                 u.y = u.y + fy * h;
                 if (INTEG == SPP_RK4) {
                     P4 k = a.kacc[i];
-                    a.kacc[i] = P4{k.x + fx * 2, k.y + fy * 2, k.th + tq * 2, 0.0};
+                    a.kacc[i] = P4{k.x + fx * 2, k.y + fy * 2, k.c + tq * 2, 0.0};
                 }
             }
             th = spp::wrap_angle(th);
-            bool final_stage = (INTEG == SPP_RK2) || (STAGE == 4);
-            if (final_stage) a.U[i] = u;
+            if (INTEG == SPP_RK2 || STAGE == 4) a.U_out[i] = u;
             x = spp::periodic_wrap(u.x, P.Lx2);
             y = spp::periodic_wrap(u.y, P.Ly2);
         }
-        a.out[i] = P4{x, y, th, 0.0};
+        double sn, cs;
+        sincos(th, &sn, &cs);
+        a.out[i] = P4{x, y, cs, sn};
+        a.th_out[i] = th;
     }
-    bool final_stage = (INTEG == SPP_EULER) || (INTEG == SPP_RK2 && STAGE == 2) || (STAGE == 4);
-    check_disp(a, i, live, x, y, final_stage ? a.kstep + 1 : a.kstep);
+    constexpr bool final_stage = (INTEG == SPP_EULER) || (INTEG == SPP_RK2 && STAGE == 2) || (STAGE == 4);
+    check_disp(a, live, i, x, y, final_stage ? a.kstep + 1 : a.kstep);
 }
 
 // ------------------------------------------------------------------------------------------------
-// Unfused path: Box::Interact and Move_* as separate launches (drop-in granularity).
+// Split-call path: Box::Interact and Move_* as separate launches (drop-in granularity), on the
+// current state set in place like the reference's particle array.
 // ------------------------------------------------------------------------------------------------
 
 template <int MODEL, int REP>
@@ -448,9 +497,8 @@ __global__ __launch_bounds__(kBlock) void k_interact(StageArgs a, P4* force)
     int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.N) return;
     P4 me = a.gather[i];
-    double vix = cos(me.th), viy = sin(me.th);
     double fx, fy, tq;
-    gather_forces<MODEL, REP>(a, i, me, vix, viy, fx, fy, tq);
+    gather_forces<MODEL, REP>(a, i, me, fx, fy, tq);
     force[i] = P4{fx, fy, tq, 0.0};
 }
 
@@ -458,13 +506,11 @@ struct MoveArgs {
     int N;
     Phys P;
     int integ, stage;
-    P4* pos;            // state, updated in place like the reference's particle array
-    double2* U;
-    P4* old;            // r_old.x, r_old.y, theta_old
+    SetPtrs cur;
+    P4* old;     // r_old.x, r_old.y, theta_old
     P4* kacc;
     double* dth;
     const P4* force;
-    const int* id;
     const double* inj;
     int64_t step;
 };
@@ -474,54 +520,73 @@ __global__ __launch_bounds__(kBlock) void k_move(MoveArgs a)
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.N) return;
     const Phys& P = a.P;
-    P4 me = a.pos[i];
+    P4 me = a.cur.pos[i];
+    double th_in = a.cur.th[i];
     P4 f = a.force[i];
-    double2 u = a.U[i];
-    double th;
+    double2 u = a.cur.U[i];
+    double th, x, y;
     if (a.stage <= 1) {
-        double dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)a.id[i], a.step);
+        double dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)a.cur.id[i], a.step);
         a.dth[i] = dn;
         if (a.integ == SPP_EULER) {
-            th = me.th + P.dt * (f.th);
+            th = th_in + P.dt * (f.c);
             th += dn;
             u.x += f.x * P.dt;
             u.y += f.y * P.dt;
-            a.U[i] = u;
-            a.pos[i] = P4{spp::periodic_wrap(u.x, P.Lx2), spp::periodic_wrap(u.y, P.Ly2), th, 0.0};
-            return;
+            a.cur.U[i] = u;
+            x = spp::periodic_wrap(u.x, P.Lx2);
+            y = spp::periodic_wrap(u.y, P.Ly2);
+        } else {
+            a.old[i] = P4{u.x, u.y, th_in, 0.0};
+            th = th_in;
+            th += P.half_dt * f.c;
+            th += dn / 2;
+            th = spp::wrap_angle(th);
+            if (a.integ == SPP_RK4) a.kacc[i] = f;
+            x = spp::periodic_wrap(me.x + f.x * P.half_dt, P.Lx2);
+            y = spp::periodic_wrap(me.y + f.y * P.half_dt, P.Ly2);
         }
-        a.old[i] = P4{u.x, u.y, me.th, 0.0};
-        th = me.th;
-        th += P.half_dt * f.th;
-        th += dn / 2;
-        th = spp::wrap_angle(th);
-        if (a.integ == SPP_RK4) a.kacc[i] = f;
-        a.pos[i] = P4{spp::periodic_wrap(me.x + f.x * P.half_dt, P.Lx2),
-                      spp::periodic_wrap(me.y + f.y * P.half_dt, P.Ly2), th, 0.0};
-        return;
-    }
-    P4 o = a.old[i];
-    double dn = a.dth[i];
-    if (a.integ == SPP_RK4 && a.stage == 4) {
-        P4 k = a.kacc[i];
-        th = o.th + (k.th + f.th) * P.dt6;
-        th += dn;
-        u.x = o.x + (k.x + f.x) * P.dt6;
-        u.y = o.y + (k.y + f.y) * P.dt6;
     } else {
-        double h = (a.integ == SPP_RK4 && a.stage == 2) ? P.half_dt : P.dt;
-        th = o.th + h * f.th;
-        th += (a.integ == SPP_RK4 && a.stage == 2) ? dn / 2 : dn;
-        u.x = o.x + f.x * h;
-        u.y = o.y + f.y * h;
-        if (a.integ == SPP_RK4) {
+        P4 o = a.old[i];
+        double dn = a.dth[i];
+        if (a.integ == SPP_RK4 && a.stage == 4) {
             P4 k = a.kacc[i];
-            a.kacc[i] = P4{k.x + f.x * 2, k.y + f.y * 2, k.th + f.th * 2, 0.0};
+            th = o.c + (k.c + f.c) * P.dt6;
+            th += dn;
+            u.x = o.x + (k.x + f.x) * P.dt6;
+            u.y = o.y + (k.y + f.y) * P.dt6;
+        } else {
+            double h = (a.integ == SPP_RK4 && a.stage == 2) ? P.half_dt : P.dt;
+            th = o.c + h * f.c;
+            th += (a.integ == SPP_RK4 && a.stage == 2) ? dn / 2 : dn;
+            u.x = o.x + f.x * h;
+            u.y = o.y + f.y * h;
+            if (a.integ == SPP_RK4) {
+                P4 k = a.kacc[i];
+                a.kacc[i] = P4{k.x + f.x * 2, k.y + f.y * 2, k.c + f.c * 2, 0.0};
+            }
         }
+        th = spp::wrap_angle(th);
+        a.cur.U[i] = u;  // the reference writes r_original in every stage >= 2
+        x = spp::periodic_wrap(u.x, P.Lx2);
+        y = spp::periodic_wrap(u.y, P.Ly2);
     }
-    th = spp::wrap_angle(th);
-    a.U[i] = u;
-    a.pos[i] = P4{spp::periodic_wrap(u.x, P.Lx2), spp::periodic_wrap(u.y, P.Ly2), th, 0.0};
+    double sn, cs;
+    sincos(th, &sn, &cs);
+    a.cur.pos[i] = P4{x, y, cs, sn};
+    a.cur.th[i] = th;
+}
+
+// Headings -> (cos, sin) after a host upload, computed by the same device sincos as the stages.
+__global__ __launch_bounds__(kBlock) void k_init_heading(int N, P4* __restrict__ pos,
+                                                         const double* __restrict__ th)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double sn, cs;
+    sincos(th[i], &sn, &cs);
+    pos[i].c = cs;
+    pos[i].s = sn;
 }
 
 // Scatter caller-order values into slot order: dst[slot] = src[id[slot]].
@@ -532,6 +597,16 @@ __global__ void k_gather_by_id(int N, const int* __restrict__ id, const double* 
     if (i < N) dst[i] = src[id[i]];
 }
 
+__global__ __launch_bounds__(kBlock) void k_copy_set(int N, SetPtrs src, SetPtrs dst)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    dst.pos[i] = src.pos[i];
+    dst.th[i] = src.th[i];
+    dst.U[i] = src.U[i];
+    dst.id[i] = src.id[i];
+}
+
 // Polarization partial sums (Box::Compute_Polarization, serial/box.h:531-538), two-pass.
 __global__ __launch_bounds__(kBlock) void k_polar(int N, const P4* __restrict__ pos,
                                                   double2* __restrict__ part)
@@ -540,9 +615,8 @@ __global__ __launch_bounds__(kBlock) void k_polar(int N, const P4* __restrict__ 
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double vx = 0, vy = 0;
     if (i < N) {
-        double th = pos[i].th;
-        vx = cos(th);
-        vy = sin(th);
+        vx = pos[i].c;
+        vy = pos[i].s;
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -621,6 +695,26 @@ const char* stage_name(int integ, int stage)
 
 thread_local std::string g_create_error;
 
+// Smallest double T with fl(sqrt(T)) >= R: then sqrt(d2) < R  <=>  d2 < T for every d2 >= 0
+// (correctly rounded sqrt is monotone), so the reference's predicates need no sqrt.
+double sqrt_threshold(double R)
+{
+    if (!(R > 0)) return 0.0;
+    double x = R * R;
+    if (std::sqrt(x) >= R) {
+        for (;;) {
+            double p = std::nextafter(x, 0.0);
+            if (std::sqrt(p) >= R)
+                x = p;
+            else
+                break;
+        }
+    } else {
+        while (std::sqrt(x) < R) x = std::nextafter(x, INFINITY);
+    }
+    return x;
+}
+
 }  // namespace
 
 // ==================================================================================================
@@ -628,13 +722,9 @@ thread_local std::string g_create_error;
 // ==================================================================================================
 
 struct Snapshot {
-    P4* pos = nullptr;
-    double2* U = nullptr;
-    int* id = nullptr;
+    SetPtrs s{nullptr, nullptr, nullptr, nullptr};
     int64_t step = 0;
     double t = 0;
-    int chunk_hint = 0;
-    double rate_hint = 0;
     bool valid = false;
 };
 
@@ -643,45 +733,61 @@ struct ProfRec {
     hipEvent_t a, b;
 };
 
+// One enqueued interval awaiting verification.
+struct Interval {
+    int start_set;   // set holding the state at the interval's start (kept untouched)
+    int m;           // steps
+    int slot;        // flag slot
+    int64_t step0;
+    int64_t since0;  // steps since the last rebuild at the interval's start (after its rebuild)
+    double t0;
+    bool rebuilt;
+    hipEvent_t done;
+};
+
 struct spp_engine {
     spp_params p{};
     int N = 0;
     Grid G{};
-    int nkeys = 0, nscan_blocks = 0;
+    int nkeys = 0, nscan_blocks = 0, nblk = 0;
     int cap = 32;
     Phys P{};
-    double r_cut = 0, half_skin = 0, thresh = 0;
+    double r_cut = 0, half_skin = 0, thresh = 0, T_list = 0;
     hipStream_t stream = nullptr;
     int device = 0;
 
-    P4 *pos0 = nullptr, *pos1 = nullptr, *pos2 = nullptr, *pos_alt = nullptr;
-    double2 *U = nullptr, *U_alt = nullptr;
-    int *id = nullptr, *id_alt = nullptr;
+    SetPtrs set[3] = {};
+    int cur = 0;
+    P4 *pos1 = nullptr, *pos2 = nullptr;
+    double *th1 = nullptr, *th2 = nullptr;
     double *dth = nullptr, *dth_alt = nullptr, *inj_host_order = nullptr, *inj = nullptr;
     P4 *kacc = nullptr, *force = nullptr, *old = nullptr;
-    P4 *aux_alt[3] = {nullptr, nullptr, nullptr};
+    P4* aux_alt[3] = {nullptr, nullptr, nullptr};
     int *nbr = nullptr, *ncount = nullptr, *cellxy = nullptr;
     float2* build = nullptr;
     int *key = nullptr, *perm = nullptr, *cell_count = nullptr, *cell_start = nullptr,
-        *cell_cursor = nullptr, *block_sums = nullptr;
-    int* dflags = nullptr;              // [0] bad_step, [1] max list length, [2] total entries
-    unsigned long long* dmaxdisp = nullptr;
-    int* hflags = nullptr;              // pinned mirror: 4 ints + maxdisp (8 B)
+        *block_sums = nullptr;
+    Flags* dflags = nullptr;     // 2 slots
+    double* dblockmax = nullptr; // 2 slots x nblk
+    Flags* hflags = nullptr;     // pinned mirrors
+    double* hblockmax = nullptr;
     double2* dpolar = nullptr;
-    Snapshot rollback;
+    hipEvent_t slot_event[2] = {nullptr, nullptr};
     std::map<int, Snapshot> slots;
 
     int64_t step = 0;
     double t = 0;
     int64_t rebuilds = 0, rollbacks = 0, host_syncs = 0;
     int64_t steps_since_build = 0;
-    bool have_state = false, list_valid = false, need_rebuild = false;
+    int64_t intervals = 0;
+    bool have_state = false, list_valid = false;
     bool inj_pending = false;
-    int chunk_hint = 8;
-    double rate_hint = 0;   // displacement per step seen over the last build interval
+    double T_cross = 0;          // estimated steps from a rebuild to the displacement trigger
+    double T_hist[3] = {0, 0, 0};
+    int force_m = 0;             // length of the next interval after a rollback (0 = predict)
     int max_nbrs = 0;
     double mean_nbrs = 0;
-    int unfused_stage = 0;  // next expected stage of the unfused path (0 = at a step boundary)
+    int unfused_stage = 0;       // next expected stage of the split-call path (0 = step boundary)
 
     bool profiling = false;
     std::vector<ProfRec> prof;
@@ -710,6 +816,12 @@ int fail(spp_engine* e, int code, const char* fmt, ...)
     do {                                                                                           \
         hipError_t _st = (call);                                                                   \
         if (_st != hipSuccess) return fail((e), SPP_ERR_HIP, "%s: %s", #call, hipGetErrorString(_st)); \
+    } while (0)
+
+#define TRY(x)                         \
+    do {                               \
+        int _rc = (x);                 \
+        if (_rc != SPP_OK) return _rc; \
     } while (0)
 
 inline int nblocks(int n) { return (n + kBlock - 1) / kBlock; }
@@ -746,12 +858,6 @@ int launch(spp_engine* e, const char* name, F kernel, int grid, Args... args)
     return SPP_OK;
 }
 
-#define TRY(x)                  \
-    do {                        \
-        int _rc = (x);          \
-        if (_rc != SPP_OK) return _rc; \
-    } while (0)
-
 template <typename T>
 int dalloc(spp_engine* e, T** p, size_t n)
 {
@@ -778,33 +884,52 @@ int alloc_nbr(spp_engine* e)
     return dalloc(e, &e->nbr, (size_t)e->cap * e->N);
 }
 
-// ------------------------------------------------------------------------------------------------
-// Rebuild: bin, scan, scatter, per-cell order, permute, Verlet list.
-// ------------------------------------------------------------------------------------------------
-int launch_rebuild(spp_engine* e, bool aux)
+int alloc_set(spp_engine* e, SetPtrs& s)
 {
-    int N = e->N, nk = e->nkeys;
-    TRY(launch(e, "fill", k_fill<int>, nblocks(nk), e->cell_count, 0, nk));
-    TRY(launch(e, "fill", k_fill<int>, nblocks(nk), e->cell_cursor, 0, nk));
-    TRY(launch(e, "fill", k_fill<int>, 1, e->dflags + 1, 0, 2));
-    TRY(launch(e, "bin", k_bin, nblocks(N), N, e->G, (const P4*)e->pos0, e->key, e->cell_count));
+    TRY(dalloc(e, &s.pos, e->N));
+    TRY(dalloc(e, &s.th, e->N));
+    TRY(dalloc(e, &s.U, e->N));
+    TRY(dalloc(e, &s.id, e->N));
+    return SPP_OK;
+}
+
+void free_set(SetPtrs& s)
+{
+    dfree(s.pos);
+    dfree(s.th);
+    dfree(s.U);
+    dfree(s.id);
+    s = SetPtrs{nullptr, nullptr, nullptr, nullptr};
+}
+
+// ------------------------------------------------------------------------------------------------
+// Rebuild: bin, scan, scatter, per-cell order, permute (set src -> set dst), Verlet list.
+// ------------------------------------------------------------------------------------------------
+int launch_nlist(spp_engine* e, int slot)
+{
+    NlistArgs na{e->N, e->cap, e->G, e->T_list, e->set[e->cur].pos, e->cell_start, e->nbr,
+                 e->ncount, e->build, e->cellxy, e->dflags + slot};
+    return launch(e, "nlist", k_nlist, nblocks(e->N), na);
+}
+
+int launch_rebuild(spp_engine* e, int dst, int slot, bool aux)
+{
+    int N = e->N, nk = e->nkeys, src = e->cur;
+    TRY(launch(e, "bin", k_bin, nblocks(N), N, e->G, (const P4*)e->set[src].pos, e->key,
+               e->cell_count, e->dflags + slot));
     TRY(launch(e, "scan_partial", k_scan_partial, e->nscan_blocks, (const int*)e->cell_count, nk,
                e->block_sums));
     TRY(launch(e, "scan_final", k_scan_final, e->nscan_blocks, (const int*)e->cell_count, nk,
                (const int*)e->block_sums, e->cell_start));
     TRY(launch(e, "scatter", k_scatter, nblocks(N), N, (const int*)e->key,
-               (const int*)e->cell_start, e->cell_cursor, e->perm));
+               (const int*)e->cell_start, e->cell_count, e->perm));
     TRY(launch(e, "cell_sort", k_cell_sort, nblocks(nk), nk, (const int*)e->cell_start, e->perm,
-               (const int*)e->id));
+               (const int*)e->set[src].id));
     PermuteArgs pa{};
     pa.N = N;
     pa.perm = e->perm;
-    pa.pos_in = e->pos0;
-    pa.pos_out = e->pos_alt;
-    pa.U_in = e->U;
-    pa.U_out = e->U_alt;
-    pa.id_in = e->id;
-    pa.id_out = e->id_alt;
+    pa.in = e->set[src];
+    pa.out = e->set[dst];
     if (aux) {
         pa.a_in[0] = e->kacc;
         pa.a_in[1] = e->force;
@@ -814,160 +939,125 @@ int launch_rebuild(spp_engine* e, bool aux)
         pa.d_out = e->dth_alt;
     }
     TRY(launch(e, "permute", k_permute, nblocks(N), pa));
-    std::swap(e->pos0, e->pos_alt);
-    std::swap(e->U, e->U_alt);
-    std::swap(e->id, e->id_alt);
     if (aux) {
         std::swap(e->kacc, e->aux_alt[0]);
         std::swap(e->force, e->aux_alt[1]);
         std::swap(e->old, e->aux_alt[2]);
         std::swap(e->dth, e->dth_alt);
     }
-    NlistArgs na{N, e->cap, e->G, e->p.r_list, e->pos0, e->cell_start, e->nbr, e->ncount,
-                 e->build, e->cellxy, e->dflags + 1};
-    TRY(launch(e, "nlist", k_nlist, nblocks(N), na));
+    e->cur = dst;
+    TRY(launch_nlist(e, slot));
     e->rebuilds++;
     e->steps_since_build = 0;
     e->list_valid = true;
-    e->need_rebuild = false;
     return SPP_OK;
 }
 
-// Re-run only the list kernel (after growing the capacity; cells and order are unchanged).
-int launch_nlist_only(spp_engine* e)
+int free_set_other_than(int a, int b)
 {
-    TRY(launch(e, "fill", k_fill<int>, 1, e->dflags + 1, 0, 2));
-    NlistArgs na{e->N, e->cap, e->G, e->p.r_list, e->pos0, e->cell_start, e->nbr, e->ncount,
-                 e->build, e->cellxy, e->dflags + 1};
-    return launch(e, "nlist", k_nlist, nblocks(e->N), na);
+    for (int k = 0; k < 3; ++k)
+        if (k != a && k != b) return k;
+    return -1;
 }
 
-int read_flags(spp_engine* e)
+// Synchronous rebuild with a capacity check (used outside the interval scheduler).
+int rebuild_sync(spp_engine* e, bool aux)
 {
-    HIPC(e, hipMemcpyAsync(e->hflags, e->dflags, 4 * sizeof(int), hipMemcpyDeviceToHost, e->stream));
-    HIPC(e, hipMemcpyAsync(e->hflags + 4, e->dmaxdisp, sizeof(unsigned long long),
-                           hipMemcpyDeviceToHost, e->stream));
-    HIPC(e, hipStreamSynchronize(e->stream));
-    e->host_syncs++;
-    return SPP_OK;
-}
-
-// Synchronously verify the list of the last rebuild fitted; grow and rebuild the list if not.
-int ensure_capacity(spp_engine* e)
-{
+    int dst = free_set_other_than(e->cur, -1);
+    TRY(launch_rebuild(e, dst, 0, aux));
     for (;;) {
-        TRY(read_flags(e));
-        int need = e->hflags[1];
+        HIPC(e, hipMemcpyAsync(e->hflags, e->dflags, sizeof(Flags), hipMemcpyDeviceToHost, e->stream));
+        HIPC(e, hipStreamSynchronize(e->stream));
+        e->host_syncs++;
+        int need = e->hflags[0].max_len;
         e->max_nbrs = need;
-        e->mean_nbrs = (double)(unsigned)e->hflags[2] / e->N;
+        e->mean_nbrs = (double)(unsigned)e->hflags[0].nnz / e->N;
         if (need <= e->cap) return SPP_OK;
         e->cap = ((need + 7) / 8) * 8 + 8;
         TRY(alloc_nbr(e));
-        TRY(launch_nlist_only(e));
+        TRY(launch(e, "fill", k_fill<int>, 1, (int*)e->dflags, 0, 4));
+        TRY(launch_nlist(e, 0));
     }
 }
 
-// Injected noise arrives in caller order; put it in slot order with the current permutation.
 int take_injected(spp_engine* e)
 {
-    TRY(launch(e, "gather_noise", k_gather_by_id, nblocks(e->N), e->N, (const int*)e->id,
-               (const double*)e->inj_host_order, e->inj));
-    return SPP_OK;
+    // injected noise arrives in caller order; put it in slot order with the current permutation
+    return launch(e, "gather_noise", k_gather_by_id, nblocks(e->N), e->N, (const int*)e->set[e->cur].id,
+                  (const double*)e->inj_host_order, e->inj);
 }
 
-StageArgs stage_args(spp_engine* e)
+// Launch one fused step.  S = set holding the step's input state, W = set receiving its output
+// (S == W except for the first step of an interval).
+int launch_step(spp_engine* e, int S, int W, int kstep, int slot, bool record)
 {
+    int integ = e->p.integrator;
     StageArgs a{};
     a.N = e->N;
-    a.cap = e->cap;
     a.P = e->P;
-    a.U = e->U;
     a.dth = e->dth;
     a.kacc = e->kacc;
     a.nbr = e->nbr;
     a.ncount = e->ncount;
-    a.id = e->id;
-    a.inj = nullptr;
+    a.id = e->set[S].id;
     a.build = e->build;
     a.step = e->step;
-    a.thresh2 = e->thresh * e->thresh;
-    a.bad_step = e->dflags;
-    a.maxdisp = e->dmaxdisp;
-    return a;
-}
-
-// Launch one fused step (no host synchronisation).
-int launch_step(spp_engine* e, int kstep, bool record)
-{
-    int integ = e->p.integrator;
-    StageArgs a = stage_args(e);
     a.kstep = kstep;
+    a.thresh2 = e->thresh * e->thresh;
+    a.flags = e->dflags + slot;
+    a.blockmax = e->dblockmax + (size_t)slot * e->nblk;
+    a.th0 = e->set[S].th;
+    a.U_in = e->set[S].U;
+    a.U_out = e->set[W].U;
     if (e->inj_pending) {
         TRY(take_injected(e));
         a.inj = e->inj;
     }
+    // the id array does not change within an interval; the output set gets its own copy
+    a.id_out = S != W ? e->set[W].id : nullptr;
     int nst = integ == SPP_EULER ? 1 : integ;
     for (int s = 1; s <= nst; ++s) {
         int stage = integ == SPP_EULER ? 0 : s;
-        bool last = s == nst;
-        a.record = (record && last) ? 1 : 0;
+        a.record = (record && s == nst) ? 1 : 0;
         if (integ == SPP_EULER) {
-            a.gather = e->pos0;
+            a.gather = e->set[S].pos;
+            a.th_in = e->set[S].th;
             a.out = e->pos1;
+            a.th_out = e->th1;
         } else if (integ == SPP_RK2) {
-            a.gather = s == 1 ? e->pos0 : e->pos1;
-            a.out = s == 1 ? e->pos1 : e->pos0;
+            a.gather = s == 1 ? e->set[S].pos : e->pos1;
+            a.th_in = s == 1 ? e->set[S].th : e->th1;
+            a.out = s == 1 ? e->pos1 : e->set[W].pos;
+            a.th_out = s == 1 ? e->th1 : e->set[W].th;
         } else {
-            const P4* g[5] = {nullptr, e->pos0, e->pos1, e->pos2, e->pos1};
-            P4* o[5] = {nullptr, e->pos1, e->pos2, e->pos1, e->pos0};
+            const P4* g[5] = {nullptr, e->set[S].pos, e->pos1, e->pos2, e->pos1};
+            const double* gt[5] = {nullptr, e->set[S].th, e->th1, e->th2, e->th1};
+            P4* o[5] = {nullptr, e->pos1, e->pos2, e->pos1, e->set[W].pos};
+            double* ot[5] = {nullptr, e->th1, e->th2, e->th1, e->set[W].th};
             a.gather = g[s];
+            a.th_in = gt[s];
             a.out = o[s];
+            a.th_out = ot[s];
         }
-        a.pos0 = e->pos0;
         StageFn fn = pick_stage(e->p.model, e->p.repulsion, integ, stage);
-        TRY(launch(e, stage_name(integ, stage), fn, nblocks(e->N), a));
-        if (s == 1) a.inj = nullptr;
+        TRY(launch(e, stage_name(integ, stage), fn, e->nblk, a));
+        a.inj = nullptr;
+        a.id_out = nullptr;
     }
-    if (integ == SPP_EULER) std::swap(e->pos0, e->pos1);
+    if (integ == SPP_EULER) {
+        std::swap(e->set[W].pos, e->pos1);
+        std::swap(e->set[W].th, e->th1);
+    }
+    e->cur = W;
     e->step++;
     e->t += e->p.dt;
     e->steps_since_build++;
     return SPP_OK;
 }
 
-int copy_state(spp_engine* e, Snapshot& s, bool save)
+int copy_set(spp_engine* e, const SetPtrs& src, const SetPtrs& dst)
 {
-    size_t N = e->N;
-    if (!s.pos) {
-        TRY(dalloc(e, &s.pos, N));
-        TRY(dalloc(e, &s.U, N));
-        TRY(dalloc(e, &s.id, N));
-    }
-    if (save) {
-        HIPC(e, hipMemcpyAsync(s.pos, e->pos0, N * sizeof(P4), hipMemcpyDeviceToDevice, e->stream));
-        HIPC(e, hipMemcpyAsync(s.U, e->U, N * sizeof(double2), hipMemcpyDeviceToDevice, e->stream));
-        HIPC(e, hipMemcpyAsync(s.id, e->id, N * sizeof(int), hipMemcpyDeviceToDevice, e->stream));
-        s.step = e->step;
-        s.t = e->t;
-        s.chunk_hint = e->chunk_hint;
-        s.rate_hint = e->rate_hint;
-        s.valid = true;
-    } else {
-        HIPC(e, hipMemcpyAsync(e->pos0, s.pos, N * sizeof(P4), hipMemcpyDeviceToDevice, e->stream));
-        HIPC(e, hipMemcpyAsync(e->U, s.U, N * sizeof(double2), hipMemcpyDeviceToDevice, e->stream));
-        HIPC(e, hipMemcpyAsync(e->id, s.id, N * sizeof(int), hipMemcpyDeviceToDevice, e->stream));
-        e->step = s.step;
-        e->t = s.t;
-    }
-    return SPP_OK;
-}
-
-void free_snapshot(Snapshot& s)
-{
-    dfree(s.pos);
-    dfree(s.U);
-    dfree(s.id);
-    s = Snapshot{};
+    return launch(e, "copy_set", k_copy_set, nblocks(e->N), e->N, src, dst);
 }
 
 int check_ready(spp_engine* e)
@@ -977,83 +1067,130 @@ int check_ready(spp_engine* e)
     return SPP_OK;
 }
 
-// Steps under SPP_REBUILD_PERIOD: the reference's Multi_Step(cell_update_period) + Update_Cells.
-int multi_step_period(spp_engine* e, int steps)
+// Enqueue one interval: [rebuild] + m steps + flag read-back.  Returns the interval record.
+int enqueue_interval(spp_engine* e, int m, bool rebuild, int protect, Interval& iv)
 {
-    int P = std::max(1, e->p.rebuild_period);
-    for (int k = 0; k < steps; ++k) {
-        TRY(launch_step(e, 0, false));
-        e->inj_pending = false;
-        if (e->steps_since_build >= P) {
-            TRY(launch_rebuild(e, false));
-            TRY(ensure_capacity(e));
-        }
+    int slot = (int)(e->intervals++ & 1);
+    iv.slot = slot;
+    iv.m = m;
+    iv.rebuilt = rebuild;
+    if (rebuild) {
+        int dst = free_set_other_than(e->cur, protect);
+        TRY(launch_rebuild(e, dst, slot, false));
+    } else {
+        // flags are reset by k_bin on a rebuild; otherwise reset them here
+        TRY(launch(e, "fill", k_fill<int>, 1, (int*)(e->dflags + slot), 0, 4));
+        TRY(launch(e, "fill", k_fill<int>, 1, (int*)(e->dflags + slot), INT_MAX, 1));
     }
+    iv.start_set = e->cur;
+    iv.step0 = e->step;
+    iv.since0 = e->steps_since_build;
+    iv.t0 = e->t;
+    int W = free_set_other_than(e->cur, protect);
+    for (int k = 0; k < m; ++k) {
+        TRY(launch_step(e, e->cur, k == 0 ? W : e->cur, k, slot, k == m - 1));
+        if (k == 0) e->inj_pending = false;
+    }
+    HIPC(e, hipMemcpyAsync(e->hflags + slot, e->dflags + slot, sizeof(Flags), hipMemcpyDeviceToHost, e->stream));
+    HIPC(e, hipMemcpyAsync(e->hblockmax + (size_t)slot * e->nblk, e->dblockmax + (size_t)slot * e->nblk,
+                           e->nblk * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIPC(e, hipEventRecord(e->slot_event[slot], e->stream));
+    iv.done = e->slot_event[slot];
     return SPP_OK;
 }
 
-// Steps under SPP_REBUILD_DISPLACEMENT.  Chunks of steps run without host round trips; the stage
-// kernels flag the first step whose input state moved farther than the half skin.  At the chunk's
-// end the host reads the flag: a late trigger rolls the chunk back to its start and replays only
-// the valid steps (the counter-based noise makes the replay exact), then rebuilds.
-int multi_step_displacement(spp_engine* e, int steps)
+int predict_m(spp_engine* e, int remaining)
 {
-    int done = 0;
-    while (done < steps) {
-        bool rebuilt = false;
-        if (e->need_rebuild || !e->list_valid) {
-            if (e->steps_since_build > 0 && e->steps_since_build < 1000000)
-                e->rate_hint = std::max(e->rate_hint * 0.5, e->thresh / (double)e->steps_since_build);
-            TRY(launch_rebuild(e, false));
-            rebuilt = true;
-        }
-        int m = std::min(steps - done, std::max(1, e->chunk_hint));
-        TRY(copy_state(e, e->rollback, true));
-        TRY(launch(e, "fill", k_fill<int>, 1, e->dflags, INT_MAX, 1));
-        TRY(launch(e, "fill", k_fill<unsigned long long>, 1, e->dmaxdisp, 0ull, 1));
-        for (int k = 0; k < m; ++k) TRY(launch_step(e, k, k == m - 1));
-        TRY(read_flags(e));
-        if (rebuilt) {
-            e->max_nbrs = e->hflags[1];
-            e->mean_nbrs = (double)(unsigned)e->hflags[2] / e->N;
-            if (e->hflags[1] > e->cap) {  // list overflowed: restore, grow, redo the chunk
-                TRY(copy_state(e, e->rollback, false));
-                e->steps_since_build = 0;
-                e->cap = ((e->hflags[1] + 7) / 8) * 8 + 8;
-                TRY(alloc_nbr(e));
-                TRY(launch_nlist_only(e));
-                e->rollbacks++;
-                continue;
-            }
-        }
-        int bad = e->hflags[0];
-        int64_t since0 = e->steps_since_build - m;
-        if (bad < m) {
-            // steps [0, bad) were computed with a valid list; replay exactly those
-            TRY(copy_state(e, e->rollback, false));
-            e->steps_since_build = since0;
-            for (int k = 0; k < bad; ++k) TRY(launch_step(e, k, false));
-            if (bad > 0) e->inj_pending = false;
-            done += bad;
-            e->rollbacks++;
-            e->need_rebuild = true;
-            e->chunk_hint = std::max(1, (int)(0.7 * (since0 + bad)));
+    int m;
+    if (e->force_m > 0) {
+        m = e->force_m;
+        e->force_m = 0;
+    } else if (e->p.rebuild == SPP_REBUILD_PERIOD) {
+        m = std::max(1, e->p.rebuild_period - (int)e->steps_since_build);
+    } else if (e->T_cross <= 0) {
+        m = 4;  // no history yet: a short interval to measure the displacement rate
+    } else {
+        m = std::max(1, (int)(0.85 * e->T_cross));
+    }
+    return std::min(std::min(m, remaining), 4096);
+}
+
+// Steps through the interval scheduler (both rebuild policies).
+int run_intervals(spp_engine* e, int steps)
+{
+    std::deque<Interval> q;
+    int to_enqueue = steps;
+    bool period = e->p.rebuild == SPP_REBUILD_PERIOD;
+    while (to_enqueue > 0 || !q.empty()) {
+        if (to_enqueue > 0 && q.size() < 2) {
+            int m = predict_m(e, to_enqueue);
+            bool rebuild = !e->list_valid || e->steps_since_build > 0;
+            if (period) rebuild = !e->list_valid || e->steps_since_build >= e->p.rebuild_period;
+            int protect = q.empty() ? -1 : q.front().start_set;
+            Interval iv{};
+            TRY(enqueue_interval(e, m, rebuild, protect, iv));
+            q.push_back(iv);
+            to_enqueue -= m;
             continue;
         }
-        done += m;
-        e->inj_pending = false;
-        double disp2;
-        std::memcpy(&disp2, e->hflags + 4, sizeof disp2);
-        double disp = std::sqrt(disp2);
-        double since = (double)e->steps_since_build;
-        double rate = std::max(disp / std::max(since, 1.0), e->rate_hint * 0.25);
-        double remaining = rate > 0 ? (e->thresh - disp) / rate : 1e9;
-        if (remaining < 1.5) {
-            e->need_rebuild = true;
-            e->chunk_hint = std::max(1, (int)(0.8 * since));
-        } else {
-            e->chunk_hint = std::max(1, std::min(256, (int)(0.8 * remaining)));
+        Interval iv = q.front();
+        HIPC(e, hipEventSynchronize(iv.done));
+        e->host_syncs++;
+        const Flags f = e->hflags[iv.slot];
+        bool overflow = iv.rebuilt && f.max_len > e->cap;
+        int bad = period ? INT_MAX : f.bad_step;
+        if (iv.rebuilt) {
+            e->max_nbrs = f.max_len;
+            e->mean_nbrs = (double)(unsigned)f.nnz / e->N;
         }
+        if (!overflow && bad >= iv.m) {
+            q.pop_front();
+            if (!period && iv.rebuilt) {
+                // linear extrapolation of the largest displacement to the trigger
+                const double* bm = e->hblockmax + (size_t)iv.slot * e->nblk;
+                double d2 = 0;
+                for (int b = 0; b < e->nblk; ++b) d2 = std::max(d2, bm[b]);
+                double disp = std::sqrt(d2);
+                double est = disp > 0 ? e->thresh / disp * iv.m : 4096.0;
+                e->T_hist[2] = e->T_hist[1];
+                e->T_hist[1] = e->T_hist[0];
+                e->T_hist[0] = est;
+                double T = est;
+                for (double h : e->T_hist)
+                    if (h > 0) T = std::min(T, h);
+                e->T_cross = T;
+            }
+            continue;
+        }
+        // Roll back to the interval's start; it and everything enqueued after it are redone.
+        HIPC(e, hipStreamSynchronize(e->stream));
+        int redo = 0;
+        for (auto& x : q) redo += x.m;
+        q.clear();
+        e->cur = iv.start_set;
+        e->step = iv.step0;
+        e->t = iv.t0;
+        e->steps_since_build = iv.since0;
+        e->rollbacks++;
+        if (e->p.noise_mode == SPP_NOISE_INJECTED) e->inj_pending = true;  // replay uses it again
+        to_enqueue += redo;
+        e->list_valid = false;  // the device list may belong to a later interval
+        if (overflow) {
+            // same rebuild point (the interval started with one), larger capacity
+            e->cap = ((f.max_len + 7) / 8) * 8 + 8;
+            TRY(alloc_nbr(e));
+            e->force_m = iv.m;
+            continue;
+        }
+        if (bad == 0 && iv.rebuilt)
+            return fail(e, SPP_ERR_STATE,
+                        "a particle moves farther than the half skin (%g) within one step: "
+                        "raise r_list or lower dt", e->half_skin);
+        // The rebuild at the interval's start (identical, the state there is already sorted) is
+        // redone; the valid prefix [0, bad) is replayed as its own interval and verified again.
+        e->force_m = bad > 0 ? bad : 0;
+        e->T_cross = std::max(1.0, 0.8 * (iv.since0 + std::max(bad, 1)));
+        for (double& h : e->T_hist) h = h > 0 ? std::min(h, e->T_cross) : h;
     }
     return SPP_OK;
 }
@@ -1065,12 +1202,13 @@ Phys make_phys(const spp_params& p)
     P.Ly = p.Ly;
     P.Lx2 = 2 * p.Lx;
     P.Ly2 = 2 * p.Ly;
+    P.invLx2 = 1.0 / P.Lx2;
+    P.invLy2 = 1.0 / P.Ly2;
     P.dt = p.dt;
     P.half_dt = p.dt / 2;  // shared/parameters.h:78
     P.dt6 = p.dt / 6;      // shared/parameters.h:79
     P.v0 = p.v0;
     P.g = p.g;
-    P.Ra = p.R_align;
     P.Ra2 = p.R_align * p.R_align;
     if (p.model == SPP_MODEL_CONTINUOUS) {
         P.c_align = (1 - p.alpha) / spp::kPi;
@@ -1079,15 +1217,17 @@ Phys make_phys(const spp_params& p)
         P.c_align = p.g / (spp::kPi * p.R_align * p.R_align);
         P.c_self = 0;
     }
+    P.T_align = sqrt_threshold(p.R_align);
+    P.T_rep = p.repulsion != SPP_REP_NONE ? sqrt_threshold(p.rc_rep) : 0.0;
+    P.T_max = std::max(P.T_align, P.T_rep);
     P.rc = p.rc_rep;
     P.sigma = p.sigma_rep;
+    P.inv_sigma = 1.0 / p.sigma_rep;
     P.A = p.A_rep;
     double sc = p.sigma_rep / p.rc_rep, sc2 = sc * sc, sc4 = sc2 * sc2, sc8 = sc4 * sc4;
     P.rc13 = sc8 * sc4 * sc;
     P.yuk_c = std::exp(-p.rc_rep / p.sigma_rep) *
               (1. / (p.rc_rep * p.rc_rep) + 1. / (p.sigma_rep * p.rc_rep));
-    double c = max_cutoff(p);
-    P.guard2 = c * c * (1.0 + 1e-12);
     P.noise_amp = std::sqrt(2 * p.D_rot * p.dt);  // BasicDynamicParticle::Set_Dr, particle.h:106-110
     P.seed = p.seed;
     return P;
@@ -1158,6 +1298,10 @@ int spp_create(const spp_params* p, spp_engine** out)
         return fail(nullptr, SPP_ERR_ARG, "n, Lx, Ly, dt and r_list must be positive");
     if (p->noise_mode != SPP_NOISE_STREAM && p->noise_mode != SPP_NOISE_INJECTED)
         return fail(nullptr, SPP_ERR_ARG, "noise_mode %d", p->noise_mode);
+    if (p->rebuild != SPP_REBUILD_DISPLACEMENT && p->rebuild != SPP_REBUILD_PERIOD)
+        return fail(nullptr, SPP_ERR_ARG, "rebuild %d", p->rebuild);
+    if (p->rebuild == SPP_REBUILD_PERIOD && p->rebuild_period <= 0)
+        return fail(nullptr, SPP_ERR_ARG, "rebuild_period must be positive");
     double rcut = max_cutoff(*p);
     if (p->r_list < rcut) return fail(nullptr, SPP_ERR_ARG, "r_list %g below cutoff %g", p->r_list, rcut);
     int ncx = p->ncell_x > 0 ? p->ncell_x : (int)std::floor(2 * p->Lx / p->r_list);
@@ -1167,7 +1311,8 @@ int spp_create(const spp_params* p, spp_engine** out)
     if (ncx < 3 || ncy < 3 || 2 * p->Lx / ncx < p->r_list || 2 * p->Ly / ncy < p->r_list)
         return fail(nullptr, SPP_ERR_ARG, "cell grid %dx%d invalid for r_list %g in box %gx%g", ncx,
                     ncy, p->r_list, 2 * p->Lx, 2 * p->Ly);
-    if (ncx >= 32768 || ncy >= 32768) return fail(nullptr, SPP_ERR_ARG, "cell grid too large");
+    if ((int64_t)ncx * ncy >= (1 << 30) || ncx >= 32768 || ncy >= 32768)
+        return fail(nullptr, SPP_ERR_ARG, "cell grid too large");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return fail(nullptr, SPP_ERR_NO_DEVICE, "no HIP device visible");
@@ -1180,15 +1325,17 @@ int spp_create(const spp_params* p, spp_engine** out)
     e->G.Ly = p->Ly;
     e->G.Lx2 = 2 * p->Lx;
     e->G.Ly2 = 2 * p->Ly;
+    e->G.invLx2 = 1.0 / e->G.Lx2;
+    e->G.invLy2 = 1.0 / e->G.Ly2;
     e->G.ncx = ncx;
     e->G.ncy = ncy;
-    e->G.ntx = (ncx + (1 << kTileBits) - 1) >> kTileBits;
     e->G.quirk = p->binning == SPP_BIN_REF_QUIRK;
-    int nty = (ncy + (1 << kTileBits) - 1) >> kTileBits;
-    e->nkeys = e->G.ntx * nty << (2 * kTileBits);
+    e->nkeys = ncx * ncy;
     e->nscan_blocks = (e->nkeys + kScanTile - 1) / kScanTile;
+    e->nblk = nblocks(e->N);
     e->cap = p->ell_capacity > 0 ? p->ell_capacity : 32;
     e->P = make_phys(*p);
+    e->T_list = sqrt_threshold(p->r_list);
     e->r_cut = rcut;
     e->half_skin = 0.5 * (p->r_list - rcut);
     // float32 build positions: allow for their rounding (|x| <= L) when triggering
@@ -1205,14 +1352,11 @@ int spp_create(const spp_params* p, spp_engine** out)
     };
     size_t N = e->N;
     A(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) == hipSuccess ? SPP_OK : SPP_ERR_HIP);
-    A(dalloc(e, &e->pos0, N));
+    for (int k = 0; k < 3; ++k) A(alloc_set(e, e->set[k]));
     A(dalloc(e, &e->pos1, N));
     A(dalloc(e, &e->pos2, N));
-    A(dalloc(e, &e->pos_alt, N));
-    A(dalloc(e, &e->U, N));
-    A(dalloc(e, &e->U_alt, N));
-    A(dalloc(e, &e->id, N));
-    A(dalloc(e, &e->id_alt, N));
+    A(dalloc(e, &e->th1, N));
+    A(dalloc(e, &e->th2, N));
     A(dalloc(e, &e->dth, N));
     A(dalloc(e, &e->dth_alt, N));
     A(dalloc(e, &e->inj_host_order, N));
@@ -1227,15 +1371,25 @@ int spp_create(const spp_params* p, spp_engine** out)
     A(dalloc(e, &e->key, N));
     A(dalloc(e, &e->perm, N));
     A(dalloc(e, &e->cell_count, (size_t)e->nkeys));
-    A(dalloc(e, &e->cell_cursor, (size_t)e->nkeys));
     A(dalloc(e, &e->cell_start, (size_t)e->nkeys + 1));
     A(dalloc(e, &e->block_sums, (size_t)e->nscan_blocks));
-    A(dalloc(e, &e->dflags, 4));
-    A(dalloc(e, &e->dmaxdisp, 1));
-    A(dalloc(e, &e->dpolar, (size_t)nblocks(e->N)));
+    A(dalloc(e, &e->dflags, 2));
+    A(dalloc(e, &e->dblockmax, 2 * (size_t)e->nblk));
+    A(dalloc(e, &e->dpolar, (size_t)e->nblk));
     A(alloc_nbr(e));
-    if (rc == SPP_OK && hipHostMalloc((void**)&e->hflags, 8 * sizeof(int), hipHostMallocDefault) != hipSuccess)
+    if (rc == SPP_OK && hipHostMalloc((void**)&e->hflags, 2 * sizeof(Flags), hipHostMallocDefault) != hipSuccess)
         rc = SPP_ERR_HIP;
+    if (rc == SPP_OK &&
+        hipHostMalloc((void**)&e->hblockmax, 2 * (size_t)e->nblk * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        rc = SPP_ERR_HIP;
+    for (int s = 0; s < 2 && rc == SPP_OK; ++s)
+        if (hipEventCreateWithFlags(&e->slot_event[s], hipEventDisableTiming) != hipSuccess) rc = SPP_ERR_HIP;
+    if (rc == SPP_OK) {
+        // cell counts start at zero and every rebuild's scatter returns them to zero
+        if (launch(e, "fill", k_fill<int>, nblocks(e->nkeys), e->cell_count, 0, e->nkeys) != SPP_OK ||
+            hipStreamSynchronize(e->stream) != hipSuccess)
+            rc = SPP_ERR_HIP;
+    }
     if (rc != SPP_OK) {
         std::string msg = e->err.empty() ? std::string("device allocation failed") : e->err;
         spp_destroy(e);
@@ -1249,20 +1403,22 @@ void spp_destroy(spp_engine* e)
 {
     if (!e) return;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
-    void* bufs[] = {e->pos0, e->pos1, e->pos2, e->pos_alt, e->U, e->U_alt, e->id, e->id_alt,
-                    e->dth, e->dth_alt, e->inj_host_order, e->inj, e->kacc, e->force, e->old,
-                    e->aux_alt[0], e->aux_alt[1], e->aux_alt[2], e->nbr, e->ncount, e->cellxy,
-                    e->build, e->key, e->perm, e->cell_count, e->cell_cursor, e->cell_start,
-                    e->block_sums, e->dflags, e->dmaxdisp, e->dpolar};
+    for (int k = 0; k < 3; ++k) free_set(e->set[k]);
+    void* bufs[] = {e->pos1, e->pos2, e->th1, e->th2, e->dth, e->dth_alt, e->inj_host_order,
+                    e->inj, e->kacc, e->force, e->old, e->aux_alt[0], e->aux_alt[1], e->aux_alt[2],
+                    e->nbr, e->ncount, e->cellxy, e->build, e->key, e->perm, e->cell_count,
+                    e->cell_start, e->block_sums, e->dflags, e->dblockmax, e->dpolar};
     for (void* b : bufs) dfree(b);
-    free_snapshot(e->rollback);
-    for (auto& kv : e->slots) free_snapshot(kv.second);
+    for (auto& kv : e->slots) free_set(kv.second.s);
     for (auto& r : e->prof) {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
     }
     for (auto ev : e->event_pool) (void)hipEventDestroy(ev);
+    for (auto ev : e->slot_event)
+        if (ev) (void)hipEventDestroy(ev);
     if (e->hflags) (void)hipHostFree(e->hflags);
+    if (e->hblockmax) (void)hipHostFree(e->hblockmax);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
@@ -1297,18 +1453,20 @@ int spp_set_state(spp_engine* e, const double* xw, const double* yw, const doubl
     std::vector<double2> U(n);
     std::vector<int> id(n);
     for (int i = 0; i < n; ++i) {
-        pos[i] = P4{xw[i], yw[i], theta[i], 0.0};
+        pos[i] = P4{xw[i], yw[i], 0.0, 0.0};
         U[i] = make_double2(xu ? xu[i] : xw[i], yu ? yu[i] : yw[i]);
         id[i] = i;
     }
-    HIPC(e, hipMemcpyAsync(e->pos0, pos.data(), n * sizeof(P4), hipMemcpyHostToDevice, e->stream));
-    HIPC(e, hipMemcpyAsync(e->U, U.data(), n * sizeof(double2), hipMemcpyHostToDevice, e->stream));
-    HIPC(e, hipMemcpyAsync(e->id, id.data(), n * sizeof(int), hipMemcpyHostToDevice, e->stream));
+    SetPtrs& s = e->set[e->cur];
+    HIPC(e, hipMemcpyAsync(s.pos, pos.data(), n * sizeof(P4), hipMemcpyHostToDevice, e->stream));
+    HIPC(e, hipMemcpyAsync(s.th, theta, n * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    HIPC(e, hipMemcpyAsync(s.U, U.data(), n * sizeof(double2), hipMemcpyHostToDevice, e->stream));
+    HIPC(e, hipMemcpyAsync(s.id, id.data(), n * sizeof(int), hipMemcpyHostToDevice, e->stream));
+    TRY(launch(e, "init_heading", k_init_heading, nblocks(n), n, s.pos, (const double*)s.th));
     e->have_state = true;
     e->unfused_stage = 0;
-    TRY(launch_rebuild(e, false));
-    TRY(ensure_capacity(e));
-    return SPP_OK;
+    e->inj_pending = false;
+    return rebuild_sync(e, false);
 }
 
 int spp_get_state(spp_engine* e, double* xw, double* yw, double* xu, double* yu, double* theta,
@@ -1317,11 +1475,14 @@ int spp_get_state(spp_engine* e, double* xw, double* yw, double* xu, double* yu,
     TRY(check_ready(e));
     if (n != e->N) return fail(e, SPP_ERR_ARG, "n = %d, engine has %d particles", n, e->N);
     std::vector<P4> pos(n);
+    std::vector<double> th(n);
     std::vector<double2> U(n);
     std::vector<int> id(n);
-    HIPC(e, hipMemcpyAsync(pos.data(), e->pos0, n * sizeof(P4), hipMemcpyDeviceToHost, e->stream));
-    HIPC(e, hipMemcpyAsync(U.data(), e->U, n * sizeof(double2), hipMemcpyDeviceToHost, e->stream));
-    HIPC(e, hipMemcpyAsync(id.data(), e->id, n * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    const SetPtrs& s = e->set[e->cur];
+    HIPC(e, hipMemcpyAsync(pos.data(), s.pos, n * sizeof(P4), hipMemcpyDeviceToHost, e->stream));
+    HIPC(e, hipMemcpyAsync(th.data(), s.th, n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIPC(e, hipMemcpyAsync(U.data(), s.U, n * sizeof(double2), hipMemcpyDeviceToHost, e->stream));
+    HIPC(e, hipMemcpyAsync(id.data(), s.id, n * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipStreamSynchronize(e->stream));
     for (int k = 0; k < n; ++k) {
         int i = id[k];
@@ -1329,7 +1490,7 @@ int spp_get_state(spp_engine* e, double* xw, double* yw, double* xu, double* yu,
         if (yw) yw[i] = pos[k].y;
         if (xu) xu[i] = U[k].x;
         if (yu) yu[i] = U[k].y;
-        if (theta) theta[i] = pos[k].th;
+        if (theta) theta[i] = th[k];
     }
     return SPP_OK;
 }
@@ -1337,8 +1498,7 @@ int spp_get_state(spp_engine* e, double* xw, double* yw, double* xu, double* yu,
 int spp_build_neighbors(spp_engine* e)
 {
     TRY(check_ready(e));
-    TRY(launch_rebuild(e, true));
-    return ensure_capacity(e);
+    return rebuild_sync(e, true);
 }
 
 int spp_get_neighbors(spp_engine* e, int32_t* offsets, int32_t* idx, int64_t cap, int64_t* nnz)
@@ -1349,7 +1509,7 @@ int spp_get_neighbors(spp_engine* e, int32_t* offsets, int32_t* idx, int64_t cap
     std::vector<int> nb((size_t)C * N), cnt(N), id(N);
     HIPC(e, hipMemcpyAsync(nb.data(), e->nbr, nb.size() * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipMemcpyAsync(cnt.data(), e->ncount, N * sizeof(int), hipMemcpyDeviceToHost, e->stream));
-    HIPC(e, hipMemcpyAsync(id.data(), e->id, N * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIPC(e, hipMemcpyAsync(id.data(), e->set[e->cur].id, N * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipStreamSynchronize(e->stream));
     std::vector<int64_t> rowlen(N + 1, 0);
     for (int k = 0; k < N; ++k) rowlen[id[k] + 1] = cnt[k];
@@ -1372,7 +1532,7 @@ int spp_get_cells(spp_engine* e, int32_t* cell_x, int32_t* cell_y, int32_t n)
     if (n != e->N) return fail(e, SPP_ERR_ARG, "n mismatch");
     std::vector<int> cxy(n), id(n);
     HIPC(e, hipMemcpyAsync(cxy.data(), e->cellxy, n * sizeof(int), hipMemcpyDeviceToHost, e->stream));
-    HIPC(e, hipMemcpyAsync(id.data(), e->id, n * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIPC(e, hipMemcpyAsync(id.data(), e->set[e->cur].id, n * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipStreamSynchronize(e->stream));
     for (int k = 0; k < n; ++k) {
         if (cell_x) cell_x[id[k]] = cxy[k] & 0xffff;
@@ -1384,10 +1544,14 @@ int spp_get_cells(spp_engine* e, int32_t* cell_x, int32_t* cell_y, int32_t n)
 int spp_interact(spp_engine* e)
 {
     TRY(check_ready(e));
-    StageArgs a = stage_args(e);
-    a.gather = e->pos0;
+    StageArgs a{};
+    a.N = e->N;
+    a.P = e->P;
+    a.gather = e->set[e->cur].pos;
+    a.nbr = e->nbr;
+    a.ncount = e->ncount;
     InteractFn fn = pick_interact(e->p.model, e->p.repulsion);
-    return launch(e, "interact", fn, nblocks(e->N), a, e->force);
+    return launch(e, "interact", fn, e->nblk, a, e->force);
 }
 
 int spp_get_forces(spp_engine* e, double* fx, double* fy, double* tau, int32_t n)
@@ -1397,13 +1561,13 @@ int spp_get_forces(spp_engine* e, double* fx, double* fy, double* tau, int32_t n
     std::vector<P4> f(n);
     std::vector<int> id(n);
     HIPC(e, hipMemcpyAsync(f.data(), e->force, n * sizeof(P4), hipMemcpyDeviceToHost, e->stream));
-    HIPC(e, hipMemcpyAsync(id.data(), e->id, n * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIPC(e, hipMemcpyAsync(id.data(), e->set[e->cur].id, n * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipStreamSynchronize(e->stream));
     for (int k = 0; k < n; ++k) {
         int i = id[k];
         if (fx) fx[i] = f[k].x;
         if (fy) fy[i] = f[k].y;
-        if (tau) tau[i] = f[k].th;
+        if (tau) tau[i] = f[k].c;
     }
     return SPP_OK;
 }
@@ -1435,13 +1599,11 @@ int spp_move_stage(spp_engine* e, int32_t stage)
     m.P = e->P;
     m.integ = integ;
     m.stage = stage;
-    m.pos = e->pos0;
-    m.U = e->U;
+    m.cur = e->set[e->cur];
     m.old = e->old;
     m.kacc = e->kacc;
     m.dth = e->dth;
     m.force = e->force;
-    m.id = e->id;
     m.inj = nullptr;
     m.step = e->step;
     if (stage <= 1 && e->inj_pending) {
@@ -1449,7 +1611,7 @@ int spp_move_stage(spp_engine* e, int32_t stage)
         m.inj = e->inj;
         e->inj_pending = false;
     }
-    TRY(launch(e, "move", k_move, nblocks(e->N), m));
+    TRY(launch(e, "move", k_move, e->nblk, m));
     int done = integ == SPP_EULER ? 1 : stage;
     if (done == nst) {
         e->unfused_stage = 0;
@@ -1470,14 +1632,7 @@ int spp_multi_step(spp_engine* e, int32_t steps)
     if (e->p.noise_mode == SPP_NOISE_INJECTED && (steps > 1 || (steps == 1 && !e->inj_pending)))
         return fail(e, SPP_ERR_STATE, "injected noise covers one step: inject before each step");
     if (steps == 0) return SPP_OK;
-    if (e->p.rebuild == SPP_REBUILD_PERIOD) {
-        if (!e->list_valid) {
-            TRY(launch_rebuild(e, false));
-            TRY(ensure_capacity(e));
-        }
-        return multi_step_period(e, steps);
-    }
-    return multi_step_displacement(e, steps);
+    return run_intervals(e, steps);
 }
 
 int spp_one_step(spp_engine* e) { return spp_multi_step(e, 1); }
@@ -1485,10 +1640,9 @@ int spp_one_step(spp_engine* e) { return spp_multi_step(e, 1); }
 int spp_polarization(spp_engine* e, double* p, double* sum_x, double* sum_y)
 {
     TRY(check_ready(e));
-    int nb = nblocks(e->N);
-    TRY(launch(e, "polar", k_polar, nb, e->N, (const P4*)e->pos0, e->dpolar));
-    std::vector<double2> part(nb);
-    HIPC(e, hipMemcpyAsync(part.data(), e->dpolar, nb * sizeof(double2), hipMemcpyDeviceToHost, e->stream));
+    TRY(launch(e, "polar", k_polar, e->nblk, e->N, (const P4*)e->set[e->cur].pos, e->dpolar));
+    std::vector<double2> part(e->nblk);
+    HIPC(e, hipMemcpyAsync(part.data(), e->dpolar, e->nblk * sizeof(double2), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipStreamSynchronize(e->stream));
     double sx = 0, sy = 0;
     for (auto& v : part) {
@@ -1509,9 +1663,14 @@ int spp_snapshot(spp_engine* e, int32_t slot)
     // Box::Save re-bins so that a later Load re-bins to the same particle order
     // (serial/box.h:313-318); the sorted layout here is a pure function of the positions, so
     // save-rebuild and restore-rebuild give the same order and list, and a replay is bitwise.
-    TRY(launch_rebuild(e, false));
-    TRY(ensure_capacity(e));
-    return copy_state(e, e->slots[slot], true);
+    TRY(rebuild_sync(e, false));
+    Snapshot& s = e->slots[slot];
+    if (!s.s.pos) TRY(alloc_set(e, s.s));
+    TRY(copy_set(e, e->set[e->cur], s.s));
+    s.step = e->step;
+    s.t = e->t;
+    s.valid = true;
+    return SPP_OK;
 }
 
 int spp_restore(spp_engine* e, int32_t slot)
@@ -1519,14 +1678,12 @@ int spp_restore(spp_engine* e, int32_t slot)
     TRY(check_ready(e));
     auto it = e->slots.find(slot);
     if (it == e->slots.end() || !it->second.valid) return fail(e, SPP_ERR_ARG, "no snapshot in slot %d", slot);
-    TRY(copy_state(e, it->second, false));
-    e->chunk_hint = it->second.chunk_hint;
-    e->rate_hint = it->second.rate_hint;
+    TRY(copy_set(e, it->second.s, e->set[e->cur]));
+    e->step = it->second.step;
+    e->t = it->second.t;
     e->unfused_stage = 0;
-    // Box::Load ends in a cell update (parallel/box.h:241); rebuilding from the restored state
-    // also makes the replay independent of whatever list was current.
-    TRY(launch_rebuild(e, false));
-    return ensure_capacity(e);
+    // Box::Load ends in a cell update (parallel/box.h:241)
+    return rebuild_sync(e, false);
 }
 
 int spp_synchronize(spp_engine* e)

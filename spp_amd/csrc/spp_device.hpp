@@ -1,8 +1,8 @@
 // spp_device.hpp -- device-side math shared by the engine's kernels (gfx950).
 //
-// Everything here is compiled with -ffp-contract=off so that products and sums round exactly as
-// the reference's x86-64 -O3 build does (it has no FMA); the Verlet predicate sqrt(d2) < rv then
-// sees bit-identical d2 on both sides and neighbour sets match exactly.
+// Compiled with -ffp-contract=off: products and sums round exactly as the reference's x86-64 -O3
+// build does (no FMA), so squared distances are bit-identical on both sides; fma() appears only
+// where the result is provably exact or where bitwise agreement is not needed.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -15,25 +15,30 @@ namespace spp {
 constexpr double kPi = 3.14159265358979323846;  // the reference's PI = M_PI
 constexpr double kTwoPi = 6.283185307179586;    // 2*M_PI as the reference evaluates it
 
-// One particle's stage state, 32 B so a neighbour gather is one 128-B line access.
+// One particle as its neighbours read it: wrapped position and heading unit vector
+// (reference r and v = (cos theta, sin theta)), 32 B so a gather is one 128-B line access.
+// The heading angle itself lives in a separate array that only its owner reads.
 struct alignas(32) P4 {
-    double x, y, th, w;
+    double x, y, c, s;
 };
 
-// Per-launch constants of the physics (kept in kernel arguments, i.e. SGPRs).
+// Per-launch physics constants (kernel arguments, i.e. SGPRs).
 struct Phys {
-    double Lx, Ly, Lx2, Ly2;
+    double Lx, Ly, Lx2, Ly2, invLx2, invLy2;
     double dt, half_dt, dt6;
     double v0;
-    double c_align;    // REPULSIVE: g / (pi R_a^2); CONTINUOUS: (1 - alpha) / pi
-    double c_self;     // CONTINUOUS: alpha / pi
-    double g;          // CONTINUOUS: drift = g * summed torque
-    double Ra, Ra2;    // alignment radius and its square (continuous predicate d2 < Ra^2)
+    double c_align;   // REPULSIVE: g / (pi R_a^2); CONTINUOUS: (1 - alpha) / pi
+    double c_self;    // CONTINUOUS: alpha / pi
+    double g;         // CONTINUOUS: drift = g * summed torque
+    double Ra2;       // CONTINUOUS predicate d2 < R_a^2 (particle.h:305)
+    // Exact squared-distance thresholds: sqrt(d2) < R  <=>  d2 < T(R) for the correctly rounded
+    // sqrt the reference uses (cell.h:84-85, particle.h:738,747), so no sqrt is needed to decide.
+    double T_align, T_rep, T_max;
     double rc, sigma, A;
-    double rc13;       // (sigma/rc)^13 for R12 truncated
-    double yuk_c;      // exp(-rc/sigma)(1/rc^2 + 1/(sigma rc)) for Yukawa truncated
-    double guard2;     // d2 >= guard2 implies sqrt(d2) >= every cutoff (skip the sqrt)
-    double noise_amp;  // sqrt(2 D dt)
+    double rc13;      // (sigma/rc)^13 for R12 truncated
+    double yuk_c;     // exp(-rc/sigma)(1/rc^2 + 1/(sigma rc)) for Yukawa truncated
+    double inv_sigma;
+    double noise_amp; // sqrt(2 D dt)
     uint64_t seed;
 };
 
@@ -45,16 +50,14 @@ __device__ __forceinline__ double periodic_wrap(double x, double L2)
     return x - L2 * (double)(int)floor(x / L2 + 0.5);
 }
 
-// Minimum image of a coordinate difference of two wrapped positions (|dx| <= 2L).  Bit-identical
-// to periodic_wrap(dx, 2L) except when dx lies within an ulp of +-L, i.e. for pairs farther apart
-// than L, beyond every cutoff and list radius (the engine requires 2L/3 >= r_list).
-__device__ __forceinline__ double min_image(double dx, double L, double L2)
+// Minimum image of a difference of two wrapped coordinates (|dx| <= 2L).  k = rint(dx / 2L) is
+// in {-1, 0, 1}, so fma(-k, 2L, dx) rounds once, exactly like the reference's dx - 2L*k; k agrees
+// with floor(dx/2L + 0.5) except within an ulp of |dx| = L, i.e. for pairs farther apart than L,
+// beyond every cutoff and list radius (the engine requires 2L/3 >= r_list).
+__device__ __forceinline__ double min_image(double dx, double L2, double invL2)
 {
-    if (dx >= L)
-        dx -= L2;
-    else if (dx < -L)
-        dx += L2;
-    return dx;
+    double k = rint(dx * invL2);
+    return fma(-k, L2, dx);
 }
 
 // Theta wrap of the RK stages, shared/particle.h:599.
@@ -97,46 +100,44 @@ __host__ __device__ __forceinline__ double noise_normal(uint64_t seed, uint32_t 
 }
 
 // ------------------------------------------------------------------------------------------------
-// Pair accumulation for particle i against neighbour j (full list: each particle sums its own
-// side of every pair, which equals the reference's Newton-3 half-list update for that particle).
-//   REPULSIVE  (particle.h:728-757): F = law(dr, d) for d < rc, tau += g sin(th_j - th_i)/(pi Ra^2)
+// Pair accumulation of particle i against neighbour j (full list: each particle sums its own side
+// of every pair, which equals what the reference's Newton-3 half-list update adds to it).
+//   REPULSIVE  (particle.h:728-757): F = law(dr, d) for d < rc; s1 += sin(th_j - th_i) for d < Ra
 //   CONTINUOUS (particle.h:296-338): for d2 < Ra^2, s1 += sin(th_j - th_i), s2 += (dr x v_i)/d
+// sin(th_j - th_i) = s_j c_i - c_j s_i from the stored headings (no transcendental per pair).
 template <int MODEL, int REP>
-__device__ __forceinline__ void pair_accumulate(const Phys& P, double xi, double yi, double thi,
-                                                double vix, double viy, const P4& pj, double& fx,
-                                                double& fy, double& s1, double& s2)
+__device__ __forceinline__ void pair_accumulate(const Phys& P, const P4& me, const P4& pj,
+                                                double& fx, double& fy, double& s1, double& s2)
 {
-    double dx = min_image(xi - pj.x, P.Lx, P.Lx2);
-    double dy = min_image(yi - pj.y, P.Ly, P.Ly2);
+    double dx = min_image(me.x - pj.x, P.Lx2, P.invLx2);
+    double dy = min_image(me.y - pj.y, P.Ly2, P.invLy2);
     double d2 = dx * dx + dy * dy;
     if (MODEL == SPP_MODEL_CONTINUOUS) {
         if (d2 < P.Ra2) {
-            s1 += sin(pj.th - thi);
-            if (P.c_self != 0.0) {
-                double d = sqrt(d2);
-                s2 += (dx * viy - dy * vix) / d;
-            }
+            s1 += pj.s * me.c - pj.c * me.s;
+            if (P.c_self != 0.0) s2 += (dx * me.s - dy * me.c) * rsqrt(d2);
         }
         return;
     } else {
-        if (d2 >= P.guard2) return;
-        double d = sqrt(d2);
-        if (REP != SPP_REP_NONE && d < P.rc) {
+        if (d2 >= P.T_max) return;
+        if (REP != SPP_REP_NONE && d2 < P.T_rep) {
+            double ri = rsqrt(d2);  // 1/d
             double s;
-            if (REP == SPP_REP_HARMONIC) {  // Spring, force-fields.h:87-97
-                s = P.A * (P.rc - d) / d;
+            if (REP == SPP_REP_HARMONIC) {  // Spring, force-fields.h:87-97: k (rc - d)/d
+                s = P.A * (P.rc * ri - 1.0);
             } else if (REP == SPP_REP_R12_TRUNCATED || REP == SPP_REP_R12) {  // :14-58
-                double sd = P.sigma / d, sd2 = sd * sd, sd4 = sd2 * sd2, sd8 = sd4 * sd4;
-                double sd13 = sd8 * sd4 * sd;
-                s = (REP == SPP_REP_R12 ? P.A * sd13 : P.A * (sd13 - P.rc13)) / d;
+                double q = P.sigma * ri, q2 = q * q, q4 = q2 * q2, q8 = q4 * q4;
+                double q13 = q8 * q4 * q;
+                s = (REP == SPP_REP_R12 ? P.A * q13 : P.A * (q13 - P.rc13)) * ri;
             } else {  // Yukawa (truncated), :60-85
-                double e = exp(-d / P.sigma) * (1. / d2 + 1. / (P.sigma * d));
-                s = (REP == SPP_REP_YUKAWA ? P.A * e : P.A * (e - P.yuk_c)) / d;
+                double d = d2 * ri;
+                double e = exp(-d * P.inv_sigma) * (ri * ri + ri * P.inv_sigma);
+                s = (REP == SPP_REP_YUKAWA ? P.A * e : P.A * (e - P.yuk_c)) * ri;
             }
             fx += s * dx;
             fy += s * dy;
         }
-        if (d < P.Ra) s1 += sin(pj.th - thi);
+        if (d2 < P.T_align) s1 += pj.s * me.c - pj.c * me.s;
     }
 }
 

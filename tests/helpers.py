@@ -43,6 +43,13 @@ def rel_err(a, b):
     return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
 
 
+def vec_rel_err(ax, ay, bx, by):
+    """||(ax, ay) - (bx, by)|| / ||(bx, by)|| over all particles (2-vectors per particle)."""
+    num = np.sqrt(np.sum((np.asarray(ax) - bx) ** 2 + (np.asarray(ay) - by) ** 2))
+    den = np.sqrt(np.sum(np.asarray(bx) ** 2 + np.asarray(by) ** 2))
+    return num / (den if den > 0 else 1.0)
+
+
 def angle_diff(a, b):
     d = np.asarray(a) - np.asarray(b)
     return np.abs(d - 2 * np.pi * np.round(d / (2 * np.pi)))
