@@ -24,11 +24,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
 
-def workload_params(S, n, L, D, steps_hint=None):
+def workload_params(S, n, L, D, variant=0):
     p = S.default_params(S.MODEL_REPULSIVE)
     p.update(n=n, Lx=L, Ly=L, dt=0.005, v0=1.0, g=1.0, R_align=1.0, repulsion=S.REP_HARMONIC,
              A_rep=10.0, sigma_rep=1.0, rc_rep=1.0, D_rot=D, integrator=S.RK2, r_list=1.3,
-             rebuild=S.REBUILD_DISPLACEMENT, seed=2)
+             rebuild=S.REBUILD_DISPLACEMENT, seed=2, kernel_variant=variant)
     return p
 
 
@@ -42,6 +42,20 @@ def stage_bytes(kbar):
     s1 = 24 + 4 + 4 * kbar + 4 + 8 + 24 + 8
     s2 = 24 + 4 + 4 * kbar + 8 + 16 + 8 + 8 + 24 + 16
     return s1, s2
+
+
+def measured_traffic(kernel, variant):
+    """HBM bytes per launch of `kernel` from the committed counter summary (rocprofv3 --pmc
+    FETCH_SIZE and WRITE_SIZE passes, each scaled by the same counter on a copy kernel of known
+    size), or None when no summary for this kernel variant exists."""
+    path = os.path.join(ROOT, "profiles", "stage_rk2_c2_n262144_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        t = d["kernels"][str(variant)][kernel]
+        return t["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(p, x, y, th, budget_s):
@@ -73,6 +87,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--variant", type=int, default=0, help="interaction-kernel variant (spp_params.kernel_variant)")
+    ap.add_argument("--time-reps", type=int, default=200, help="launches per stage-kernel timing")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -86,7 +102,7 @@ def main():
 
     n = args.n
     L = 0.5 * float(np.sqrt(n))  # density 1: side 2L = sqrt(N)
-    p = workload_params(S, n, L, args.noise)
+    p = workload_params(S, n, L, args.noise, args.variant)
     x, y, th = S.random_state(7, n, L, L)
     e = S.Engine(p)
     e.set_state(x, y, th)
@@ -112,9 +128,15 @@ def main():
     info2 = e.info()
     kbar = info2.mean_neighbors
     b1, b2 = stage_bytes(kbar)
-    cands = {k: v for k, v in prof.items() if k.startswith("stage_rk2")}
-    dom = max(cands, key=lambda k: cands[k][0])
-    dom_ms = cands[dom][0]
+    # A plain copy of known size (the snapshot copy: 60 B per particle read and written) -- the
+    # calibration kernel for hardware byte counters in profiling runs.
+    e.snapshot(99)
+    e.synchronize()
+    # Dominant kernel: each fused RK2 stage timed over back-to-back launches between two HIP
+    # events on the engine's stream (outputs to scratch, state untouched).
+    stage_us = {"stage_rk2_1": e.time_stage(1, args.time_reps), "stage_rk2_2": e.time_stage(2, args.time_reps)}
+    dom = max(stage_us, key=stage_us.get)
+    dom_ms = stage_us[dom] * 1e-3
     dom_bytes = (b1 if dom.endswith("_1") else b2) * n
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
 
@@ -138,6 +160,7 @@ def main():
             "n": n,
             "box_side": 2 * L,
             "integrator": "rk2",
+            "kernel_variant": args.variant,
             "mean_full_list": round(kbar, 3),
             "rebuilds_in_timed": info1.rebuilds - info0.rebuilds,
             "host_syncs_in_timed": info1.host_syncs - info0.host_syncs,
@@ -150,11 +173,12 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(dom, args.variant),
             "bytes_per_launch": dom_bytes,
             "avg_launch_us": dom_ms * 1e3,
         },
-        "kernel_avg_us": kernel_ms,
+        "stage_kernel_us": {k: round(v, 3) for k, v in stage_us.items()},
+        "kernel_avg_us_event_bracketed": kernel_ms,
         "step_algorithmic_bytes_per_particle": b1 + b2,
         "step_achieved_GBs": (b1 + b2) * value / 1e9,
     }

@@ -85,6 +85,13 @@ enum {
     SPP_NOISE_INJECTED = 1 /* caller supplies d_theta for each step with spp_inject_noise            */
 };
 
+/* ---- interaction-kernel variants (same results, different gather strategy) ------------------ */
+enum {
+    SPP_KERNEL_SLOT = 0,       /* one lane per particle, neighbour records gathered from L2         */
+    SPP_KERNEL_SLOT_SPLIT = 1, /* as SLOT, heading half of a record fetched only inside the cutoff  */
+    SPP_KERNEL_STRIP_LDS = 2   /* one workgroup per cell strip, 3-row neighbourhood staged in LDS  */
+};
+
 typedef struct spp_params {
     int32_t abi_version;    /* must be SPP_ABI_VERSION                                              */
     int32_t model;          /* SPP_MODEL_*                                                          */
@@ -122,7 +129,8 @@ typedef struct spp_params {
     double sigma_mm;        /* membrane-membrane R12 sigma (reference sigma_m)                      */
     double rc_mm;           /* membrane-membrane R12 cutoff (reference repulsion_radius_m)          */
     int32_t ell_capacity;   /* neighbour slots per particle on the device (0 = default 32)          */
-    int32_t reserved_i[7];
+    int32_t kernel_variant; /* SPP_KERNEL_*: how the interaction kernels gather neighbours          */
+    int32_t reserved_i[6];
     double reserved_d[8];
 } spp_params;
 
@@ -211,6 +219,12 @@ int spp_synchronize(spp_engine* e);
 int spp_profile_enable(spp_engine* e, int32_t on);
 int spp_profile_read(spp_engine* e, char* names, int32_t cap, double* ms_avg, int64_t* launches,
                      int32_t max_kernels, int32_t* nk);
+
+/* Average duration of one fused stage kernel (stage as in spp_move_stage) over `reps` back-to-back
+   launches bracketed by two HIP events on the engine's stream.  The launches read the current
+   state and write scratch buffers, so the state is unchanged.  No reference counterpart: this is
+   the measurement hook for the roofline report.                                                   */
+int spp_time_stage(spp_engine* e, int32_t stage, int32_t reps, double* avg_us);
 
 /* ---- helpers ------------------------------------------------------------------------------------ */
 

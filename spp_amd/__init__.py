@@ -25,6 +25,7 @@ REP_NONE, REP_R12_TRUNCATED, REP_HARMONIC, REP_R12, REP_YUKAWA_TRUNCATED, REP_YU
 BIN_EXACT, BIN_REF_QUIRK = 0, 1
 REBUILD_DISPLACEMENT, REBUILD_PERIOD = 0, 1
 NOISE_STREAM, NOISE_INJECTED = 0, 1
+KERNEL_SLOT, KERNEL_SLOT_SPLIT, KERNEL_STRIP_LDS = 0, 1, 2
 
 OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_CAPACITY, ERR_UNSUPPORTED, ERR_NO_DEVICE = 0, -1, -2, -3, -4, -5, -6
 
@@ -69,7 +70,8 @@ class Params(C.Structure):
         ("sigma_mm", C.c_double),
         ("rc_mm", C.c_double),
         ("ell_capacity", C.c_int32),
-        ("reserved_i", C.c_int32 * 7),
+        ("kernel_variant", C.c_int32),
+        ("reserved_i", C.c_int32 * 6),
         ("reserved_d", C.c_double * 8),
     ]
 
@@ -130,6 +132,7 @@ SIGNATURES = {
     "spp_synchronize": (C.c_int, [C.c_void_p]),
     "spp_profile_enable": (C.c_int, [C.c_void_p, C.c_int32]),
     "spp_profile_read": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, _D, _I64, C.c_int32, _I32]),
+    "spp_time_stage": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _D]),
     "spp_random_state": (C.c_int, [C.c_uint64, C.c_int32, C.c_double, C.c_double, _D, _D, _D]),
     "spp_noise_normal": (C.c_double, [C.c_uint64, C.c_int32, C.c_int64]),
 }
@@ -296,6 +299,12 @@ class Engine:
     # -- timing --------------------------------------------------------------------------------
     def profile(self, on: bool):
         self._check(self._lib.spp_profile_enable(self._h, 1 if on else 0))
+
+    def time_stage(self, stage: int, reps: int = 50) -> float:
+        """Average microseconds of one fused stage kernel over `reps` back-to-back launches."""
+        us = C.c_double()
+        self._check(self._lib.spp_time_stage(self._h, stage, reps, C.byref(us)))
+        return us.value
 
     def profile_read(self):
         buf = C.create_string_buffer(4096)

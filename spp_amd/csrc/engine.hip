@@ -43,11 +43,20 @@ constexpr int kBlock = 256;
 constexpr int kScanItems = 4;  // items per thread in the cell-count scan
 constexpr int kScanTile = kBlock * kScanItems;
 
-struct Flags {
-    int bad_step;  // first step index (within the interval) whose stage input left the skin
-    int max_len;   // longest full neighbour list of the interval's rebuild
-    int nnz;       // total entries of that list
-    int pad;
+// One interval's bookkeeping, reduced on the device from per-block partials (no same-address
+// atomics across the chip) and written straight into host-mapped memory.
+struct Report {
+    int bad_step;     // first step index (within the interval) whose stage input left the skin
+    int max_len;      // longest full neighbour list of the interval's rebuild
+    long long nnz;    // total entries of that list
+    double maxdisp2;  // largest squared displacement at the interval's end
+};
+
+struct SlotBufs {
+    int* blkbad;      // per-block first bad step (atomicMin inside a block only)
+    double* blkmax;   // per-block max squared displacement
+    int* blklen;      // per-block longest list
+    int* blknnz;      // per-block list entries
 };
 
 // ------------------------------------------------------------------------------------------------
@@ -106,14 +115,10 @@ __device__ __forceinline__ double wave_max(double v)
 
 __global__ __launch_bounds__(kBlock) void k_bin(int N, Grid G, const P4* __restrict__ pos,
                                                 int* __restrict__ key, int* __restrict__ count,
-                                                Flags* flags)
+                                                int* __restrict__ blkbad)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0 && flags) {
-        flags->bad_step = INT_MAX;
-        flags->max_len = 0;
-        flags->nnz = 0;
-    }
+    if (threadIdx.x == 0 && blkbad) blkbad[blockIdx.x] = INT_MAX;  // grid == stage grid
     if (i >= N) return;
     P4 p = pos[i];
     int cx = cell_coord(p.x, G.Lx, G.Lx2, G.ncx, G.quirk);
@@ -266,24 +271,435 @@ __global__ __launch_bounds__(kBlock) void k_permute(PermuteArgs a)
     }
 }
 
+// ------------------------------------------------------------------------------------------------
+// Strips: the unit of work of the list build and of every interaction kernel.  A strip is a run of
+// W cells of one cell row; its particles are one contiguous slot range, and every neighbour they
+// can have lies in the three rows y-1, y, y+1 over columns [x0-1, x1+1] (periodic) -- at most six
+// contiguous slot ranges ("pieces").  A workgroup copies those pieces into LDS with coalesced loads
+// once per kernel and then serves every pair gather from LDS; the Verlet list stores 16-bit LDS
+// offsets.  A strip whose neighbourhood exceeds kLdsCap records (rare dense clusters) falls back
+// to global gathers through a 32-bit slot-index list.
+// ------------------------------------------------------------------------------------------------
+constexpr int kStripThreads = 256;
+constexpr int kLdsCap = 1024;  // records (32 KB): four workgroups (16 wavefronts) per CU
+
+// The strip's LDS copy.  Declared at file scope so that every access is compiled as an LDS
+// (address space 3) access -- through a generic pointer the compiler would emit flat loads.
+__shared__ P4 s_lds[kLdsCap];
+
+struct Strip {
+    int p0, np;               // the strip's particle slots
+    int total;                // staged records; > kLdsCap means global gathers
+    int yrow[3];              // wrapped rows y-1, y, y+1
+    int lo[3], hi[3];         // main piece column range of each row
+    int main_start[3];        // slot where each row's main piece starts
+    int main_off[3];          // its LDS offset
+    int wrap_col[3];          // extra wrapped column of each row (-1 = none)
+    int wrap_off[3];          // its LDS offset
+    int pstart[6], plen[6], poff[6];  // the pieces, in LDS order
+};
+
+struct StripGeom {
+    int W, nsx, nstrips;      // cells per strip, strips per row, total strips
+};
+
+// One thread per strip: pieces and LDS offsets from the fresh cell starts.
+__global__ __launch_bounds__(kBlock) void k_strips(Grid G, StripGeom SG, const int* __restrict__ start,
+                                                   Strip* __restrict__ strips)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= SG.nstrips) return;
+    int y = s / SG.nsx, sx = s - y * SG.nsx;
+    int x0 = sx * SG.W, x1 = min(G.ncx, x0 + SG.W) - 1;
+    Strip S;
+    S.p0 = start[y * G.ncx + x0];
+    S.np = start[y * G.ncx + x1 + 1] - S.p0;
+    int off = 0, np = 0;
+    for (int r = 0; r < 3; ++r) {
+        int yr = y + r - 1;
+        yr = yr < 0 ? yr + G.ncy : (yr >= G.ncy ? yr - G.ncy : yr);
+        int lo = max(x0 - 1, 0), hi = min(x1 + 1, G.ncx - 1);
+        int wc = -1;
+        if (x0 == 0 && hi < G.ncx - 1) wc = G.ncx - 1;
+        if (x1 == G.ncx - 1 && lo > 0) wc = 0;
+        S.yrow[r] = yr;
+        S.lo[r] = lo;
+        S.hi[r] = hi;
+        int ms = start[yr * G.ncx + lo], me = start[yr * G.ncx + hi + 1];
+        S.main_start[r] = ms;
+        S.main_off[r] = off;
+        S.pstart[np] = ms;
+        S.plen[np] = me - ms;
+        S.poff[np] = off;
+        off += me - ms;
+        ++np;
+        S.wrap_col[r] = wc;
+        if (wc >= 0) {
+            int ws = start[yr * G.ncx + wc], we = start[yr * G.ncx + wc + 1];
+            S.wrap_off[r] = off;
+            S.pstart[np] = ws;
+            S.plen[np] = we - ws;
+            S.poff[np] = off;
+            off += we - ws;
+        } else {
+            S.wrap_off[r] = off;
+            S.pstart[np] = 0;
+            S.plen[np] = 0;
+            S.poff[np] = off;
+        }
+        ++np;
+    }
+    S.total = off;
+    strips[s] = S;
+}
+
+// Copy a strip's pieces of `src` into LDS (coalesced; every thread of the block participates).
+__device__ __forceinline__ void stage_strip(const Strip& S, const P4* __restrict__ src)
+{
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        int len = S.plen[q], st = S.pstart[q], of = S.poff[q];
+        for (int k = threadIdx.x; k < len; k += kStripThreads) s_lds[of + k] = src[st + k];
+    }
+}
+
+// LDS offset of the first particle of cell column c (wrapped) in stencil row r, and the slot it
+// corresponds to.
+__device__ __forceinline__ int column_lds(const Strip& S, int r, int c, int first_slot)
+{
+    if (c >= S.lo[r] && c <= S.hi[r]) return S.main_off[r] + (first_slot - S.main_start[r]);
+    return S.wrap_off[r];
+}
+
 struct NlistArgs {
     int N, cap;
     Grid G;
+    StripGeom SG;
     double T_list;  // sqrt(d2) < r_list  <=>  d2 < T_list
     const P4* pos;
     const int* start;
-    int* nbr;
+    const Strip* strips;
+    uint16_t* nbr16;
+    int* nbr32;
     int* ncount;
     float2* build;
     int* cellxy;
-    Flags* flags;
+    int* blklen;
+    int* blknnz;
 };
 
-// Cell::Neighbor_List (shared/cell.h:74-103) over the 3x3 periodic stencil, full list, predicate
-// sqrt(d2) < rv (cell.h:84-85) evaluated as d2 < T(rv).  Each stencil row is scanned as one or
-// two contiguous slot ranges.
-__global__ __launch_bounds__(kBlock) void k_nlist(NlistArgs a)
+__device__ __forceinline__ int strip_block()
 {
+    int nblk = gridDim.x, b = blockIdx.x;
+    if ((nblk & 7) == 0) return (b & 7) * (nblk >> 3) + (b >> 3);
+    return b;
+}
+
+// The candidate record of slot j (LDS offset l) for a strip in LDS (FB = false) or not.
+template <bool FB>
+__device__ __forceinline__ P4 strip_record(const P4* __restrict__ pos, int j, int l)
+{
+    if (FB) return pos[j];
+    return s_lds[l];
+}
+
+// Cell::Neighbor_List (shared/cell.h:74-103) over the 3x3 periodic stencil, full list, predicate
+// sqrt(d2) < rv (cell.h:84-85) evaluated as d2 < T(rv), candidates read from the strip's LDS copy.
+template <bool FB>
+__device__ __forceinline__ void nlist_strip(const NlistArgs& a, const Strip& S, int& mlen, int& tnnz)
+{
+    const Grid& G = a.G;
+    for (int base = 0; base < S.np; base += kStripThreads) {
+        int t = base + threadIdx.x;
+        if (t >= S.np) continue;
+        int i = S.p0 + t;
+        P4 me = strip_record<FB>(a.pos, i, S.main_off[1] + (i - S.main_start[1]));
+        int cx = cell_coord(me.x, G.Lx, G.Lx2, G.ncx, G.quirk);
+        int cy = cell_coord(me.y, G.Ly, G.Ly2, G.ncy, G.quirk);
+        int c = 0;
+        for (int r = 0; r < 3; ++r) {
+            int row = S.yrow[r] * G.ncx;
+            for (int ox = -1; ox <= 1; ++ox) {
+                int x = cx + ox;
+                x = x < 0 ? x + G.ncx : (x >= G.ncx ? x - G.ncx : x);
+                int js = a.start[row + x], je = a.start[row + x + 1];
+                int lbase = FB ? 0 : column_lds(S, r, x, js);
+                for (int j = js; j < je; ++j) {
+                    if (j == i) continue;
+                    P4 pj = strip_record<FB>(a.pos, j, lbase + (j - js));
+                    double dx = spp::min_image(me.x - pj.x, G.Lx2, G.invLx2);
+                    double dy = spp::min_image(me.y - pj.y, G.Ly2, G.invLy2);
+                    double d2 = dx * dx + dy * dy;
+                    if (d2 < a.T_list) {
+                        if (c < a.cap) {
+                            if (FB)
+                                a.nbr32[(size_t)c * a.N + i] = j;
+                            else
+                                a.nbr16[(size_t)c * a.N + i] = (uint16_t)(lbase + (j - js));
+                        }
+                        ++c;
+                    }
+                }
+            }
+        }
+        a.ncount[i] = c < a.cap ? c : a.cap;
+        a.build[i] = make_float2((float)me.x, (float)me.y);
+        a.cellxy[i] = cx | (cy << 16);
+        mlen = max(mlen, c);
+        tnnz += c;
+    }
+}
+
+__global__ __launch_bounds__(kStripThreads) void k_nlist(NlistArgs a)
+{
+    __shared__ int red_len[kStripThreads / 64], red_nnz[kStripThreads / 64];
+    int s = strip_block();
+    if (s >= a.SG.nstrips) return;  // grid padding (uniform)
+    const Strip S = a.strips[s];
+    const bool fb = S.total > kLdsCap;
+    if (!fb) stage_strip(S, a.pos);
+    __syncthreads();
+    int mlen = 0, tnnz = 0;
+    if (fb)
+        nlist_strip<true>(a, S, mlen, tnnz);
+    else
+        nlist_strip<false>(a, S, mlen, tnnz);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mlen = max(mlen, __shfl_xor(mlen, o));
+    tnnz = wave_sum(tnnz);
+    if ((threadIdx.x & 63) == 0) {
+        red_len[threadIdx.x >> 6] = mlen;
+        red_nnz[threadIdx.x >> 6] = tnnz;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int mm = 0, tt = 0;
+        for (int k = 0; k < kStripThreads / 64; ++k) {
+            mm = max(mm, red_len[k]);
+            tt += red_nnz[k];
+        }
+        a.blklen[s] = mm;
+        a.blknnz[s] = tt;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Stage kernels: interact + Move fused, one particle per lane, one strip per workgroup.
+// ------------------------------------------------------------------------------------------------
+
+struct StageArgs {
+    int N;
+    Phys P;
+    StripGeom SG;
+    const Strip* strips;
+    const P4* gather;      // stage input: neighbours' (and own) position and heading
+    const double* th_in;   // own heading angle of the stage input
+    P4* out;               // stage output
+    double* th_out;
+    const double* th0;     // step-start heading (theta_old) for RK stages >= 2
+    const double2* U_in;   // step-start unwrapped position
+    double2* U_out;        // written by the final stage
+    double* dth;           // angular noise of this step
+    P4* kacc;              // RK4 accumulator
+    const uint16_t* nbr16;
+    const int* nbr32;
+    const int* ncount;
+    const int* id;
+    int* id_out;           // first step of an interval: the output set's copy of id
+    const double* inj;     // injected noise, slot order (null = stream)
+    const float2* build;
+    int64_t step;          // noise-stream step
+    int kstep;             // step index inside the current interval
+    double thresh2;        // displacement trigger, squared
+    int* blkbad;           // per-strip first bad step
+    double* blockmax;      // per-strip max displacement^2 (last step of an interval)
+    int record;
+};
+
+// Sum of particle i's pair terms from the strip's LDS copy (FB = false) or, for a fallback
+// strip, from global memory; four neighbours per round with their loads issued together.
+template <int MODEL, int REP, bool FB>
+__device__ __forceinline__ void gather_forces(const StageArgs& a, int i, const P4& me, double& fx,
+                                              double& fy, double& tq)
+{
+    const int cnt = a.ncount[i];
+    double s1 = 0, s2 = 0;
+    fx = me.c * a.P.v0;  // Reset: f = v (speed v0), particle.h:559-564
+    fy = me.s * a.P.v0;
+    for (int k = 0; k < cnt; k += 4) {
+        P4 q[4];
+        if (!FB) {
+            int j[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) j[u] = k + u < cnt ? a.nbr16[(size_t)(k + u) * a.N + i] : 0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) q[u] = s_lds[j[u]];
+        } else {
+            int j[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) j[u] = k + u < cnt ? a.nbr32[(size_t)(k + u) * a.N + i] : i;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) q[u] = a.gather[j[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (k + u < cnt) spp::pair_accumulate<MODEL, REP>(a.P, me, q[u], fx, fy, s1, s2);
+    }
+    tq = spp::drift_torque<MODEL>(a.P, s1, s2);
+}
+
+// Displacement trigger: marks the step (per strip, atomics stay inside one workgroup's slot) and,
+// on the last step of an interval, records the strip's largest squared displacement.
+__device__ __forceinline__ void check_disp(const StageArgs& a, int s, bool live, int i, double x,
+                                           double y, int mark, double& dmax)
+{
+    double d2 = 0;
+    if (live) {
+        float2 b = a.build[i];
+        double dx = spp::min_image(x - (double)b.x, a.P.Lx2, a.P.invLx2);
+        double dy = spp::min_image(y - (double)b.y, a.P.Ly2, a.P.invLy2);
+        d2 = dx * dx + dy * dy;
+    }
+    if (__any(d2 > a.thresh2) && (threadIdx.x & 63) == 0) atomicMin(&a.blkbad[s], mark);
+    dmax = fmax(dmax, d2);
+}
+
+__device__ __forceinline__ void finish_disp(const StageArgs& a, int s, double dmax)
+{
+    __shared__ double red[kStripThreads / 64];
+    if (!a.record) return;
+    double m = wave_max(dmax);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = red[0];
+        for (int k = 1; k < kStripThreads / 64; ++k) r = fmax(r, red[k]);
+        a.blockmax[s] = r;
+    }
+}
+
+// INTEG/STAGE: Euler (1,0); RK2 (2,1..2); RK4 (4,1..4).  Reference updates: Move
+// (particle.h:571-587), Move_Runge_Kutta2_1/2 (:589-631), Move_Runge_Kutta4_1..4 (:634-725).
+template <int MODEL, int REP, int INTEG, int STAGE, bool FB>
+__device__ __forceinline__ void stage_body(const StageArgs& a, int s, const Strip& S)
+{
+    const Phys& P = a.P;
+    constexpr bool final_stage = (INTEG == SPP_EULER) || (INTEG == SPP_RK2 && STAGE == 2) || (STAGE == 4);
+    double dmax = 0;
+    for (int base = 0; base < S.np; base += kStripThreads) {
+        int t = base + threadIdx.x;
+        bool live = t < S.np;
+        int i = S.p0 + t;
+        double x = 0, y = 0;
+        if (live) {
+            P4 me = FB ? a.gather[i] : s_lds[S.main_off[1] + (i - S.main_start[1])];
+            double dn = 0, th_in = 0, thold = 0;
+            double2 u = make_double2(0, 0);
+            P4 kk{0, 0, 0, 0};
+            if (STAGE <= 1) {
+                int pid = a.id[i];
+                if (a.id_out) a.id_out[i] = pid;
+                th_in = a.th_in[i];
+                if (INTEG == SPP_EULER) u = a.U_in[i];
+                dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)pid, a.step);
+            } else {
+                thold = a.th0[i];
+                dn = a.dth[i];
+                u = a.U_in[i];
+                if (INTEG == SPP_RK4) kk = a.kacc[i];
+            }
+            double fx, fy, tq;
+            gather_forces<MODEL, REP, FB>(a, i, me, fx, fy, tq);
+            double th;
+            if (STAGE <= 1) {
+                if (INTEG == SPP_EULER) {
+                    th = th_in + P.dt * (tq);
+                    th += dn;
+                    u.x += fx * P.dt;
+                    u.y += fy * P.dt;
+                    a.U_out[i] = u;
+                    x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
+                    y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
+                } else {
+                    a.dth[i] = dn;
+                    th = th_in;
+                    th += P.half_dt * tq;
+                    th += dn / 2;
+                    th = spp::wrap_angle(th);
+                    x = spp::periodic_wrap(me.x + fx * P.half_dt, P.Lx, P.Lx2);
+                    y = spp::periodic_wrap(me.y + fy * P.half_dt, P.Ly, P.Ly2);
+                    if (INTEG == SPP_RK4) a.kacc[i] = P4{fx, fy, tq, 0.0};
+                }
+            } else {
+                double h = (INTEG == SPP_RK4 && STAGE == 2) ? P.half_dt : P.dt;
+                if (INTEG == SPP_RK4 && STAGE == 4) {
+                    th = thold + (kk.c + tq) * P.dt6;
+                    th += dn;
+                    u.x = u.x + (kk.x + fx) * P.dt6;
+                    u.y = u.y + (kk.y + fy) * P.dt6;
+                } else {
+                    th = thold + h * tq;
+                    th += (INTEG == SPP_RK4 && STAGE == 2) ? dn / 2 : dn;
+                    u.x = u.x + fx * h;
+                    u.y = u.y + fy * h;
+                    if (INTEG == SPP_RK4) a.kacc[i] = P4{kk.x + fx * 2, kk.y + fy * 2, kk.c + tq * 2, 0.0};
+                }
+                th = spp::wrap_angle(th);
+                if (INTEG == SPP_RK2 || STAGE == 4) a.U_out[i] = u;
+                x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
+                y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
+            }
+            double sn, cs;
+            sincos(th, &sn, &cs);
+            a.out[i] = P4{x, y, cs, sn};
+            a.th_out[i] = th;
+        }
+        check_disp(a, s, live, i, x, y, final_stage ? a.kstep + 1 : a.kstep, dmax);
+    }
+    finish_disp(a, s, dmax);
+}
+
+template <int MODEL, int REP, int INTEG, int STAGE>
+__global__ __launch_bounds__(kStripThreads) void k_stage(StageArgs a)
+{
+    int s = strip_block();
+    if (s >= a.SG.nstrips) return;  // grid padding (uniform)
+    const Strip S = a.strips[s];
+    const bool fb = S.total > kLdsCap;
+    if (!fb) stage_strip(S, a.gather);
+    __syncthreads();
+    if (fb)
+        stage_body<MODEL, REP, INTEG, STAGE, true>(a, s, S);
+    else
+        stage_body<MODEL, REP, INTEG, STAGE, false>(a, s, S);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Slot-parallel variant: one lane per particle over the whole slot range, neighbour records
+// gathered from L2 through a 32-bit slot-index list.  Loads of lanes past their list length are
+// masked off (they would cost texture-address cycles for nothing); with SPLIT the heading half of
+// a record is requested only for neighbours inside the interaction cutoff.
+// ------------------------------------------------------------------------------------------------
+
+struct NlistFlatArgs {
+    int N, cap;
+    Grid G;
+    double T_list;
+    const P4* pos;
+    const int* start;
+    int* nbr32;
+    int* ncount;
+    float2* build;
+    int* cellxy;
+    int* blklen;
+    int* blknnz;
+};
+
+// Cell::Neighbor_List (shared/cell.h:74-103) over the 3x3 periodic stencil; each stencil row is
+// scanned as one or two contiguous slot ranges, four candidates at a time.
+__global__ __launch_bounds__(kBlock) void k_nlist_flat(NlistFlatArgs a)
+{
+    __shared__ int red_len[kBlock / 64], red_nnz[kBlock / 64];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool live = i < a.N;
     int c = 0;
@@ -292,7 +708,6 @@ __global__ __launch_bounds__(kBlock) void k_nlist(NlistArgs a)
         P4 me = a.pos[i];
         int cx = cell_coord(me.x, G.Lx, G.Lx2, G.ncx, G.quirk);
         int cy = cell_coord(me.y, G.Ly, G.Ly2, G.ncy, G.quirk);
-        // x ranges of the three stencil columns: [x0, x1] plus an optional wrapped cell xw
         int x0 = cx - 1, x1 = cx + 1, xw = -1;
         if (x0 < 0) {
             xw = G.ncx - 1;
@@ -315,15 +730,23 @@ __global__ __launch_bounds__(kBlock) void k_nlist(NlistArgs a)
                     s = a.start[row + xw];
                     e = a.start[row + xw + 1];
                 }
-                for (int j = s; j < e; ++j) {
-                    if (j == i) continue;
-                    P4 pj = a.pos[j];
-                    double dx = spp::min_image(me.x - pj.x, G.Lx2, G.invLx2);
-                    double dy = spp::min_image(me.y - pj.y, G.Ly2, G.invLy2);
-                    double d2 = dx * dx + dy * dy;
-                    if (d2 < a.T_list) {
-                        if (c < a.cap) a.nbr[(size_t)c * a.N + i] = j;
-                        ++c;
+                for (int j = s; j < e; j += 4) {
+                    double2 q[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (j + u < e) q[u] = *reinterpret_cast<const double2*>(&a.pos[j + u].x);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        int jj = j + u;
+                        if (jj < e && jj != i) {
+                            double dx = spp::min_image(me.x - q[u].x, G.Lx2, G.invLx2);
+                            double dy = spp::min_image(me.y - q[u].y, G.Ly2, G.invLy2);
+                            double d2 = dx * dx + dy * dy;
+                            if (d2 < a.T_list) {
+                                if (c < a.cap) a.nbr32[(size_t)c * a.N + i] = jj;
+                                ++c;
+                            }
+                        }
                     }
                 }
             }
@@ -337,153 +760,163 @@ __global__ __launch_bounds__(kBlock) void k_nlist(NlistArgs a)
     for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
     int t = wave_sum(c);
     if ((threadIdx.x & 63) == 0) {
-        atomicMax(&a.flags->max_len, m);
-        atomicAdd(&a.flags->nnz, t);
+        red_len[threadIdx.x >> 6] = m;
+        red_nnz[threadIdx.x >> 6] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int mm = 0, tt = 0;
+        for (int k = 0; k < kBlock / 64; ++k) {
+            mm = max(mm, red_len[k]);
+            tt += red_nnz[k];
+        }
+        a.blklen[blockIdx.x] = mm;
+        a.blknnz[blockIdx.x] = tt;
     }
 }
 
-// ------------------------------------------------------------------------------------------------
-// Stage kernels: interact + Move fused, one particle per lane.
-// ------------------------------------------------------------------------------------------------
-
-struct StageArgs {
-    int N;
-    Phys P;
-    const P4* gather;      // stage input: neighbours' (and own) position and heading
-    const double* th_in;   // own heading angle of the stage input
-    P4* out;               // stage output
-    double* th_out;
-    const double* th0;     // step-start heading (theta_old) for RK stages >= 2
-    const double2* U_in;   // step-start unwrapped position
-    double2* U_out;        // written by the final stage
-    double* dth;           // angular noise of this step
-    P4* kacc;              // RK4 accumulator
-    const int* nbr;
-    const int* ncount;
-    const int* id;
-    int* id_out;           // first step of an interval: the output set's copy of id
-    const double* inj;     // injected noise, slot order (null = stream)
-    const float2* build;
-    int64_t step;          // noise-stream step
-    int kstep;             // step index inside the current interval
-    double thresh2;        // displacement trigger, squared
-    Flags* flags;
-    double* blockmax;      // per-block max displacement^2 (last step of an interval)
-    int record;
-};
-
-template <int MODEL, int REP>
-__device__ __forceinline__ void gather_forces(const StageArgs& a, int i, const P4& me, double& fx,
-                                              double& fy, double& tq)
+template <int MODEL, int REP, bool SPLIT>
+__device__ __forceinline__ void gather_forces_flat(const StageArgs& a, int i, const P4& me, double& fx,
+                                                   double& fy, double& tq)
 {
+    const int cnt = a.ncount[i];
+    const int* nb = a.nbr32 + i;
     double s1 = 0, s2 = 0;
     fx = me.c * a.P.v0;  // Reset: f = v (speed v0), particle.h:559-564
     fy = me.s * a.P.v0;
-    int cnt = a.ncount[i];
-    const int* nb = a.nbr + i;
-    for (int k = 0; k < cnt; ++k) {
-        int j = nb[(size_t)k * a.N];
-        P4 pj = a.gather[j];
-        spp::pair_accumulate<MODEL, REP>(a.P, me, pj, fx, fy, s1, s2);
+    for (int k = 0; k < cnt; k += 4) {
+        int j[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (k + u < cnt) j[u] = nb[(size_t)(k + u) * a.N];
+        if (!SPLIT) {
+            P4 q[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (k + u < cnt) q[u] = a.gather[j[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (k + u < cnt) spp::pair_accumulate<MODEL, REP>(a.P, me, q[u], fx, fy, s1, s2);
+        } else {
+            P4 q[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (k + u < cnt) {
+                    double2 xy = *reinterpret_cast<const double2*>(&a.gather[j[u]].x);
+                    q[u].x = xy.x;
+                    q[u].y = xy.y;
+                }
+            bool in[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                in[u] = false;
+                if (k + u < cnt) {
+                    double dx = spp::min_image(me.x - q[u].x, a.P.Lx2, a.P.invLx2);
+                    double dy = spp::min_image(me.y - q[u].y, a.P.Ly2, a.P.invLy2);
+                    double d2 = dx * dx + dy * dy;
+                    in[u] = MODEL == SPP_MODEL_CONTINUOUS ? d2 < a.P.Ra2 : d2 < a.P.T_max;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (in[u]) {
+                    double2 cs = *reinterpret_cast<const double2*>(&a.gather[j[u]].c);
+                    q[u].c = cs.x;
+                    q[u].s = cs.y;
+                }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (in[u]) spp::pair_accumulate<MODEL, REP>(a.P, me, q[u], fx, fy, s1, s2);
+        }
     }
     tq = spp::drift_torque<MODEL>(a.P, s1, s2);
 }
 
-__device__ __forceinline__ void check_disp(const StageArgs& a, bool live, int i, double x,
-                                           double y, int mark)
+template <int MODEL, int REP, int INTEG, int STAGE, bool SPLIT>
+__global__ __launch_bounds__(kBlock) void k_stage_flat(StageArgs a)
 {
-    __shared__ double red[kBlock / 64];
-    double d2 = 0;
-    if (live) {
-        float2 b = a.build[i];
-        double dx = spp::min_image(x - (double)b.x, a.P.Lx2, a.P.invLx2);
-        double dy = spp::min_image(y - (double)b.y, a.P.Ly2, a.P.invLy2);
-        d2 = dx * dx + dy * dy;
-    }
-    if (__any(d2 > a.thresh2) && (threadIdx.x & 63) == 0) atomicMin(&a.flags->bad_step, mark);
-    if (a.record) {
-        double m = wave_max(d2);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double r = red[0];
-            for (int k = 1; k < kBlock / 64; ++k) r = fmax(r, red[k]);
-            a.blockmax[blockIdx.x] = r;
-        }
-    }
-}
-
-// INTEG/STAGE: Euler (1,0); RK2 (2,1..2); RK4 (4,1..4).  Reference updates: Move
-// (particle.h:571-587), Move_Runge_Kutta2_1/2 (:589-631), Move_Runge_Kutta4_1..4 (:634-725).
-template <int MODEL, int REP, int INTEG, int STAGE>
-__global__ __launch_bounds__(kBlock) void k_stage(StageArgs a)
-{
-    int i = xcd_block() * kBlock + threadIdx.x;
+    int b = xcd_block();
+    int i = b * kBlock + threadIdx.x;
     bool live = i < a.N;
-    double x = 0, y = 0;
+    const Phys& P = a.P;
+    constexpr bool final_stage = (INTEG == SPP_EULER) || (INTEG == SPP_RK2 && STAGE == 2) || (STAGE == 4);
+    double x = 0, y = 0, dmax = 0;
     if (live) {
-        const Phys& P = a.P;
         P4 me = a.gather[i];
-        double fx, fy, tq;
-        gather_forces<MODEL, REP>(a, i, me, fx, fy, tq);
-        double th;
+        double dn = 0, th_in = 0, thold = 0;
+        double2 u = make_double2(0, 0);
+        P4 kk{0, 0, 0, 0};
         if (STAGE <= 1) {
             int pid = a.id[i];
             if (a.id_out) a.id_out[i] = pid;
-            double dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)pid, a.step);
-            double th_in = a.th_in[i];
+            th_in = a.th_in[i];
+            if (INTEG == SPP_EULER) u = a.U_in[i];
+            dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)pid, a.step);
+        } else {
+            thold = a.th0[i];
+            dn = a.dth[i];
+            u = a.U_in[i];
+            if (INTEG == SPP_RK4) kk = a.kacc[i];
+        }
+        double fx, fy, tq;
+        gather_forces_flat<MODEL, REP, SPLIT>(a, i, me, fx, fy, tq);
+        double th;
+        if (STAGE <= 1) {
             if (INTEG == SPP_EULER) {
                 th = th_in + P.dt * (tq);
                 th += dn;
-                double2 u = a.U_in[i];
                 u.x += fx * P.dt;
                 u.y += fy * P.dt;
                 a.U_out[i] = u;
-                x = spp::periodic_wrap(u.x, P.Lx2);
-                y = spp::periodic_wrap(u.y, P.Ly2);
+                x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
+                y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
             } else {
                 a.dth[i] = dn;
                 th = th_in;
                 th += P.half_dt * tq;
                 th += dn / 2;
                 th = spp::wrap_angle(th);
-                x = spp::periodic_wrap(me.x + fx * P.half_dt, P.Lx2);
-                y = spp::periodic_wrap(me.y + fy * P.half_dt, P.Ly2);
+                x = spp::periodic_wrap(me.x + fx * P.half_dt, P.Lx, P.Lx2);
+                y = spp::periodic_wrap(me.y + fy * P.half_dt, P.Ly, P.Ly2);
                 if (INTEG == SPP_RK4) a.kacc[i] = P4{fx, fy, tq, 0.0};
             }
         } else {
-            double thold = a.th0[i];
-            double dn = a.dth[i];
-            double2 u = a.U_in[i];
             double h = (INTEG == SPP_RK4 && STAGE == 2) ? P.half_dt : P.dt;
             if (INTEG == SPP_RK4 && STAGE == 4) {
-                P4 k = a.kacc[i];
-                th = thold + (k.c + tq) * P.dt6;
+                th = thold + (kk.c + tq) * P.dt6;
                 th += dn;
-                u.x = u.x + (k.x + fx) * P.dt6;
-                u.y = u.y + (k.y + fy) * P.dt6;
+                u.x = u.x + (kk.x + fx) * P.dt6;
+                u.y = u.y + (kk.y + fy) * P.dt6;
             } else {
                 th = thold + h * tq;
                 th += (INTEG == SPP_RK4 && STAGE == 2) ? dn / 2 : dn;
                 u.x = u.x + fx * h;
                 u.y = u.y + fy * h;
-                if (INTEG == SPP_RK4) {
-                    P4 k = a.kacc[i];
-                    a.kacc[i] = P4{k.x + fx * 2, k.y + fy * 2, k.c + tq * 2, 0.0};
-                }
+                if (INTEG == SPP_RK4) a.kacc[i] = P4{kk.x + fx * 2, kk.y + fy * 2, kk.c + tq * 2, 0.0};
             }
             th = spp::wrap_angle(th);
             if (INTEG == SPP_RK2 || STAGE == 4) a.U_out[i] = u;
-            x = spp::periodic_wrap(u.x, P.Lx2);
-            y = spp::periodic_wrap(u.y, P.Ly2);
+            x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
+            y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
         }
         double sn, cs;
         sincos(th, &sn, &cs);
         a.out[i] = P4{x, y, cs, sn};
         a.th_out[i] = th;
     }
-    constexpr bool final_stage = (INTEG == SPP_EULER) || (INTEG == SPP_RK2 && STAGE == 2) || (STAGE == 4);
-    check_disp(a, live, i, x, y, final_stage ? a.kstep + 1 : a.kstep);
+    check_disp(a, b, live, i, x, y, final_stage ? a.kstep + 1 : a.kstep, dmax);
+    finish_disp(a, b, dmax);
+}
+
+template <int MODEL, int REP>
+__global__ __launch_bounds__(kBlock) void k_interact_flat(StageArgs a, P4* force)
+{
+    int i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= a.N) return;
+    P4 me = a.gather[i];
+    double fx, fy, tq;
+    gather_forces_flat<MODEL, REP, false>(a, i, me, fx, fy, tq);
+    force[i] = P4{fx, fy, tq, 0.0};
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -492,14 +925,25 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageArgs a)
 // ------------------------------------------------------------------------------------------------
 
 template <int MODEL, int REP>
-__global__ __launch_bounds__(kBlock) void k_interact(StageArgs a, P4* force)
+__global__ __launch_bounds__(kStripThreads) void k_interact(StageArgs a, P4* force)
 {
-    int i = xcd_block() * kBlock + threadIdx.x;
-    if (i >= a.N) return;
-    P4 me = a.gather[i];
-    double fx, fy, tq;
-    gather_forces<MODEL, REP>(a, i, me, fx, fy, tq);
-    force[i] = P4{fx, fy, tq, 0.0};
+    int s = strip_block();
+    if (s >= a.SG.nstrips) return;
+    const Strip S = a.strips[s];
+    const bool fb = S.total > kLdsCap;
+    if (!fb) stage_strip(S, a.gather);
+    __syncthreads();
+    for (int t = threadIdx.x; t < S.np; t += kStripThreads) {
+        int i = S.p0 + t;
+        double fx, fy, tq;
+        if (fb) {
+            gather_forces<MODEL, REP, true>(a, i, a.gather[i], fx, fy, tq);
+        } else {
+            P4 me = s_lds[S.main_off[1] + (i - S.main_start[1])];
+            gather_forces<MODEL, REP, false>(a, i, me, fx, fy, tq);
+        }
+        force[i] = P4{fx, fy, tq, 0.0};
+    }
 }
 
 struct MoveArgs {
@@ -534,8 +978,8 @@ __global__ __launch_bounds__(kBlock) void k_move(MoveArgs a)
             u.x += f.x * P.dt;
             u.y += f.y * P.dt;
             a.cur.U[i] = u;
-            x = spp::periodic_wrap(u.x, P.Lx2);
-            y = spp::periodic_wrap(u.y, P.Ly2);
+            x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
+            y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
         } else {
             a.old[i] = P4{u.x, u.y, th_in, 0.0};
             th = th_in;
@@ -543,8 +987,8 @@ __global__ __launch_bounds__(kBlock) void k_move(MoveArgs a)
             th += dn / 2;
             th = spp::wrap_angle(th);
             if (a.integ == SPP_RK4) a.kacc[i] = f;
-            x = spp::periodic_wrap(me.x + f.x * P.half_dt, P.Lx2);
-            y = spp::periodic_wrap(me.y + f.y * P.half_dt, P.Ly2);
+            x = spp::periodic_wrap(me.x + f.x * P.half_dt, P.Lx, P.Lx2);
+            y = spp::periodic_wrap(me.y + f.y * P.half_dt, P.Ly, P.Ly2);
         }
     } else {
         P4 o = a.old[i];
@@ -568,8 +1012,8 @@ __global__ __launch_bounds__(kBlock) void k_move(MoveArgs a)
         }
         th = spp::wrap_angle(th);
         a.cur.U[i] = u;  // the reference writes r_original in every stage >= 2
-        x = spp::periodic_wrap(u.x, P.Lx2);
-        y = spp::periodic_wrap(u.y, P.Ly2);
+        x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
+        y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
     }
     double sn, cs;
     sincos(th, &sn, &cs);
@@ -638,6 +1082,58 @@ __global__ __launch_bounds__(kBlock) void k_polar(int N, const P4* __restrict__ 
     }
 }
 
+// Reduces one interval's per-block partials into the host-mapped report (one block).
+__global__ __launch_bounds__(kBlock) void k_report(int nblk, SlotBufs b, int with_list, Report* out)
+{
+    __shared__ int r_bad[kBlock / 64], r_len[kBlock / 64];
+    __shared__ long long r_nnz[kBlock / 64];
+    __shared__ double r_max[kBlock / 64];
+    int bad = INT_MAX, len = 0;
+    long long nnz = 0;
+    double mx = 0;
+    for (int k = threadIdx.x; k < nblk; k += kBlock) {
+        bad = min(bad, b.blkbad[k]);
+        mx = fmax(mx, b.blkmax[k]);
+        if (with_list) {
+            len = max(len, b.blklen[k]);
+            nnz += b.blknnz[k];
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        bad = min(bad, __shfl_xor(bad, o));
+        len = max(len, __shfl_xor(len, o));
+        nnz += __shfl_xor(nnz, o);
+        mx = fmax(mx, __shfl_xor(mx, o));
+    }
+    int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        r_bad[w] = bad;
+        r_len[w] = len;
+        r_nnz[w] = nnz;
+        r_max[w] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlock / 64; ++k) {
+            bad = min(bad, r_bad[k]);
+            len = max(len, r_len[k]);
+            nnz += r_nnz[k];
+            mx = fmax(mx, r_max[k]);
+        }
+        bad = min(bad, r_bad[0]);
+        len = max(len, r_len[0]);
+        mx = fmax(mx, r_max[0]);
+        nnz = 0;
+        for (int k = 0; k < kBlock / 64; ++k) nnz += r_nnz[k];
+        out->bad_step = bad;
+        out->max_len = len;
+        out->nnz = nnz;
+        out->maxdisp2 = mx;
+        __threadfence_system();
+    }
+}
+
 // ------------------------------------------------------------------------------------------------
 // Kernel dispatch tables
 // ------------------------------------------------------------------------------------------------
@@ -645,42 +1141,58 @@ __global__ __launch_bounds__(kBlock) void k_polar(int N, const P4* __restrict__ 
 using StageFn = void (*)(StageArgs);
 using InteractFn = void (*)(StageArgs, P4*);
 
+template <int MODEL, int REP, int I, int ST>
+StageFn variant_fn(int variant)
+{
+    if (variant == SPP_KERNEL_STRIP_LDS) return k_stage<MODEL, REP, I, ST>;
+    if (variant == SPP_KERNEL_SLOT_SPLIT) return k_stage_flat<MODEL, REP, I, ST, true>;
+    return k_stage_flat<MODEL, REP, I, ST, false>;
+}
+
 template <int MODEL, int REP>
-StageFn stage_fn(int integ, int stage)
+StageFn stage_fn(int integ, int stage, int variant)
 {
-    if (integ == SPP_EULER) return k_stage<MODEL, REP, SPP_EULER, 0>;
-    if (integ == SPP_RK2) return stage == 1 ? k_stage<MODEL, REP, SPP_RK2, 1> : k_stage<MODEL, REP, SPP_RK2, 2>;
+    if (integ == SPP_EULER) return variant_fn<MODEL, REP, SPP_EULER, 0>(variant);
+    if (integ == SPP_RK2)
+        return stage == 1 ? variant_fn<MODEL, REP, SPP_RK2, 1>(variant) : variant_fn<MODEL, REP, SPP_RK2, 2>(variant);
     switch (stage) {
-    case 1: return k_stage<MODEL, REP, SPP_RK4, 1>;
-    case 2: return k_stage<MODEL, REP, SPP_RK4, 2>;
-    case 3: return k_stage<MODEL, REP, SPP_RK4, 3>;
-    default: return k_stage<MODEL, REP, SPP_RK4, 4>;
+    case 1: return variant_fn<MODEL, REP, SPP_RK4, 1>(variant);
+    case 2: return variant_fn<MODEL, REP, SPP_RK4, 2>(variant);
+    case 3: return variant_fn<MODEL, REP, SPP_RK4, 3>(variant);
+    default: return variant_fn<MODEL, REP, SPP_RK4, 4>(variant);
     }
 }
 
-StageFn pick_stage(int model, int rep, int integ, int stage)
+StageFn pick_stage(int model, int rep, int integ, int stage, int variant)
 {
-    if (model == SPP_MODEL_CONTINUOUS) return stage_fn<SPP_MODEL_CONTINUOUS, SPP_REP_NONE>(integ, stage);
+    if (model == SPP_MODEL_CONTINUOUS) return stage_fn<SPP_MODEL_CONTINUOUS, SPP_REP_NONE>(integ, stage, variant);
     switch (rep) {
-    case SPP_REP_NONE: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_NONE>(integ, stage);
-    case SPP_REP_R12_TRUNCATED: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_R12_TRUNCATED>(integ, stage);
-    case SPP_REP_HARMONIC: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_HARMONIC>(integ, stage);
-    case SPP_REP_R12: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_R12>(integ, stage);
-    case SPP_REP_YUKAWA_TRUNCATED: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_YUKAWA_TRUNCATED>(integ, stage);
-    default: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_YUKAWA>(integ, stage);
+    case SPP_REP_NONE: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_NONE>(integ, stage, variant);
+    case SPP_REP_R12_TRUNCATED: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_R12_TRUNCATED>(integ, stage, variant);
+    case SPP_REP_HARMONIC: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_HARMONIC>(integ, stage, variant);
+    case SPP_REP_R12: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_R12>(integ, stage, variant);
+    case SPP_REP_YUKAWA_TRUNCATED: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_YUKAWA_TRUNCATED>(integ, stage, variant);
+    default: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_YUKAWA>(integ, stage, variant);
     }
 }
 
-InteractFn pick_interact(int model, int rep)
+template <int MODEL, int REP>
+InteractFn interact_fn(int variant)
 {
-    if (model == SPP_MODEL_CONTINUOUS) return k_interact<SPP_MODEL_CONTINUOUS, SPP_REP_NONE>;
+    if (variant == SPP_KERNEL_STRIP_LDS) return k_interact<MODEL, REP>;
+    return k_interact_flat<MODEL, REP>;
+}
+
+InteractFn pick_interact(int model, int rep, int variant)
+{
+    if (model == SPP_MODEL_CONTINUOUS) return interact_fn<SPP_MODEL_CONTINUOUS, SPP_REP_NONE>(variant);
     switch (rep) {
-    case SPP_REP_NONE: return k_interact<SPP_MODEL_REPULSIVE, SPP_REP_NONE>;
-    case SPP_REP_R12_TRUNCATED: return k_interact<SPP_MODEL_REPULSIVE, SPP_REP_R12_TRUNCATED>;
-    case SPP_REP_HARMONIC: return k_interact<SPP_MODEL_REPULSIVE, SPP_REP_HARMONIC>;
-    case SPP_REP_R12: return k_interact<SPP_MODEL_REPULSIVE, SPP_REP_R12>;
-    case SPP_REP_YUKAWA_TRUNCATED: return k_interact<SPP_MODEL_REPULSIVE, SPP_REP_YUKAWA_TRUNCATED>;
-    default: return k_interact<SPP_MODEL_REPULSIVE, SPP_REP_YUKAWA>;
+    case SPP_REP_NONE: return interact_fn<SPP_MODEL_REPULSIVE, SPP_REP_NONE>(variant);
+    case SPP_REP_R12_TRUNCATED: return interact_fn<SPP_MODEL_REPULSIVE, SPP_REP_R12_TRUNCATED>(variant);
+    case SPP_REP_HARMONIC: return interact_fn<SPP_MODEL_REPULSIVE, SPP_REP_HARMONIC>(variant);
+    case SPP_REP_R12: return interact_fn<SPP_MODEL_REPULSIVE, SPP_REP_R12>(variant);
+    case SPP_REP_YUKAWA_TRUNCATED: return interact_fn<SPP_MODEL_REPULSIVE, SPP_REP_YUKAWA_TRUNCATED>(variant);
+    default: return interact_fn<SPP_MODEL_REPULSIVE, SPP_REP_YUKAWA>(variant);
     }
 }
 
@@ -725,6 +1237,7 @@ struct Snapshot {
     SetPtrs s{nullptr, nullptr, nullptr, nullptr};
     int64_t step = 0;
     double t = 0;
+    double T_cross = 0, T_hist[3] = {0, 0, 0};  // scheduler state: a replay rebuilds at the same steps
     bool valid = false;
 };
 
@@ -763,14 +1276,20 @@ struct spp_engine {
     double *dth = nullptr, *dth_alt = nullptr, *inj_host_order = nullptr, *inj = nullptr;
     P4 *kacc = nullptr, *force = nullptr, *old = nullptr;
     P4* aux_alt[3] = {nullptr, nullptr, nullptr};
-    int *nbr = nullptr, *ncount = nullptr, *cellxy = nullptr;
+    uint16_t* nbr16 = nullptr;   // list of LDS offsets (strips that fit in LDS)
+    int* nbr32 = nullptr;        // list of slot indices (fallback strips)
+    int *ncount = nullptr, *cellxy = nullptr;
+    StripGeom SG{};
+    int strip_grid = 0;          // nstrips rounded up to a multiple of 8 (XCD remap)
+    int variant = SPP_KERNEL_SLOT;
+    int nstat = 0;               // entries of the per-workgroup statistics arrays
+    Strip* strips = nullptr;
     float2* build = nullptr;
     int *key = nullptr, *perm = nullptr, *cell_count = nullptr, *cell_start = nullptr,
         *block_sums = nullptr;
-    Flags* dflags = nullptr;     // 2 slots
-    double* dblockmax = nullptr; // 2 slots x nblk
-    Flags* hflags = nullptr;     // pinned mirrors
-    double* hblockmax = nullptr;
+    SlotBufs slotb[2] = {};      // per-block partials of the (at most two) intervals in flight
+    Report* hrep = nullptr;      // host-mapped reports, 2 slots
+    Report* drep = nullptr;      // device view of hrep
     double2* dpolar = nullptr;
     hipEvent_t slot_event[2] = {nullptr, nullptr};
     std::map<int, Snapshot> slots;
@@ -858,6 +1377,25 @@ int launch(spp_engine* e, const char* name, F kernel, int grid, Args... args)
     return SPP_OK;
 }
 
+// Strip kernels: one workgroup of kStripThreads per strip.
+template <typename F, typename... Args>
+int launch_strips(spp_engine* e, const char* name, F kernel, Args... args)
+{
+    ProfRec rec{name, nullptr, nullptr};
+    if (e->profiling) {
+        rec.a = take_event(e);
+        rec.b = take_event(e);
+        HIPC(e, hipEventRecord(rec.a, e->stream));
+    }
+    hipLaunchKernelGGL(kernel, dim3(e->strip_grid), dim3(kStripThreads), 0, e->stream, args...);
+    HIPC(e, hipGetLastError());
+    if (e->profiling) {
+        HIPC(e, hipEventRecord(rec.b, e->stream));
+        e->prof.push_back(rec);
+    }
+    return SPP_OK;
+}
+
 template <typename T>
 int dalloc(spp_engine* e, T** p, size_t n)
 {
@@ -879,9 +1417,12 @@ double max_cutoff(const spp_params& p)
 
 int alloc_nbr(spp_engine* e)
 {
-    dfree(e->nbr);
-    e->nbr = nullptr;
-    return dalloc(e, &e->nbr, (size_t)e->cap * e->N);
+    dfree(e->nbr16);
+    dfree(e->nbr32);
+    e->nbr16 = nullptr;
+    e->nbr32 = nullptr;
+    TRY(dalloc(e, &e->nbr16, (size_t)e->cap * e->N));
+    return dalloc(e, &e->nbr32, (size_t)e->cap * e->N);
 }
 
 int alloc_set(spp_engine* e, SetPtrs& s)
@@ -907,20 +1448,34 @@ void free_set(SetPtrs& s)
 // ------------------------------------------------------------------------------------------------
 int launch_nlist(spp_engine* e, int slot)
 {
-    NlistArgs na{e->N, e->cap, e->G, e->T_list, e->set[e->cur].pos, e->cell_start, e->nbr,
-                 e->ncount, e->build, e->cellxy, e->dflags + slot};
-    return launch(e, "nlist", k_nlist, nblocks(e->N), na);
+    if (e->variant != SPP_KERNEL_STRIP_LDS) {
+        NlistFlatArgs fa{e->N, e->cap, e->G, e->T_list, e->set[e->cur].pos, e->cell_start, e->nbr32,
+                         e->ncount, e->build, e->cellxy, e->slotb[slot].blklen, e->slotb[slot].blknnz};
+        return launch(e, "nlist", k_nlist_flat, e->nblk, fa);
+    }
+    NlistArgs na{e->N, e->cap, e->G, e->SG, e->T_list, e->set[e->cur].pos, e->cell_start,
+                 e->strips, e->nbr16, e->nbr32, e->ncount, e->build, e->cellxy,
+                 e->slotb[slot].blklen, e->slotb[slot].blknnz};
+    return launch_strips(e, "nlist", k_nlist, na);
+}
+
+int launch_report(spp_engine* e, int slot, bool with_list)
+{
+    return launch(e, "report", k_report, 1, e->nstat, e->slotb[slot], with_list ? 1 : 0, e->drep + slot);
 }
 
 int launch_rebuild(spp_engine* e, int dst, int slot, bool aux)
 {
     int N = e->N, nk = e->nkeys, src = e->cur;
     TRY(launch(e, "bin", k_bin, nblocks(N), N, e->G, (const P4*)e->set[src].pos, e->key,
-               e->cell_count, e->dflags + slot));
+               e->cell_count, (int*)nullptr));
     TRY(launch(e, "scan_partial", k_scan_partial, e->nscan_blocks, (const int*)e->cell_count, nk,
                e->block_sums));
     TRY(launch(e, "scan_final", k_scan_final, e->nscan_blocks, (const int*)e->cell_count, nk,
                (const int*)e->block_sums, e->cell_start));
+    if (e->variant == SPP_KERNEL_STRIP_LDS)
+        TRY(launch(e, "strips", k_strips, nblocks(e->SG.nstrips), e->G, e->SG, (const int*)e->cell_start,
+                   e->strips));
     TRY(launch(e, "scatter", k_scatter, nblocks(N), N, (const int*)e->key,
                (const int*)e->cell_start, e->cell_count, e->perm));
     TRY(launch(e, "cell_sort", k_cell_sort, nblocks(nk), nk, (const int*)e->cell_start, e->perm,
@@ -966,16 +1521,16 @@ int rebuild_sync(spp_engine* e, bool aux)
     int dst = free_set_other_than(e->cur, -1);
     TRY(launch_rebuild(e, dst, 0, aux));
     for (;;) {
-        HIPC(e, hipMemcpyAsync(e->hflags, e->dflags, sizeof(Flags), hipMemcpyDeviceToHost, e->stream));
+        TRY(launch(e, "fill", k_fill<double>, nblocks(e->nstat), e->slotb[0].blkmax, 0.0, e->nstat));
+        TRY(launch_report(e, 0, true));
         HIPC(e, hipStreamSynchronize(e->stream));
         e->host_syncs++;
-        int need = e->hflags[0].max_len;
+        int need = e->hrep[0].max_len;
         e->max_nbrs = need;
-        e->mean_nbrs = (double)(unsigned)e->hflags[0].nnz / e->N;
+        e->mean_nbrs = (double)e->hrep[0].nnz / e->N;
         if (need <= e->cap) return SPP_OK;
         e->cap = ((need + 7) / 8) * 8 + 8;
         TRY(alloc_nbr(e));
-        TRY(launch(e, "fill", k_fill<int>, 1, (int*)e->dflags, 0, 4));
         TRY(launch_nlist(e, 0));
     }
 }
@@ -995,17 +1550,20 @@ int launch_step(spp_engine* e, int S, int W, int kstep, int slot, bool record)
     StageArgs a{};
     a.N = e->N;
     a.P = e->P;
+    a.SG = e->SG;
+    a.strips = e->strips;
     a.dth = e->dth;
     a.kacc = e->kacc;
-    a.nbr = e->nbr;
+    a.nbr16 = e->nbr16;
+    a.nbr32 = e->nbr32;
     a.ncount = e->ncount;
     a.id = e->set[S].id;
     a.build = e->build;
     a.step = e->step;
     a.kstep = kstep;
     a.thresh2 = e->thresh * e->thresh;
-    a.flags = e->dflags + slot;
-    a.blockmax = e->dblockmax + (size_t)slot * e->nblk;
+    a.blkbad = e->slotb[slot].blkbad;
+    a.blockmax = e->slotb[slot].blkmax;
     a.th0 = e->set[S].th;
     a.U_in = e->set[S].U;
     a.U_out = e->set[W].U;
@@ -1039,8 +1597,11 @@ int launch_step(spp_engine* e, int S, int W, int kstep, int slot, bool record)
             a.out = o[s];
             a.th_out = ot[s];
         }
-        StageFn fn = pick_stage(e->p.model, e->p.repulsion, integ, stage);
-        TRY(launch(e, stage_name(integ, stage), fn, e->nblk, a));
+        StageFn fn = pick_stage(e->p.model, e->p.repulsion, integ, stage, e->variant);
+        if (e->variant == SPP_KERNEL_STRIP_LDS)
+            TRY(launch_strips(e, stage_name(integ, stage), fn, a));
+        else
+            TRY(launch(e, stage_name(integ, stage), fn, e->nblk, a));
         a.inj = nullptr;
         a.id_out = nullptr;
     }
@@ -1074,13 +1635,10 @@ int enqueue_interval(spp_engine* e, int m, bool rebuild, int protect, Interval& 
     iv.slot = slot;
     iv.m = m;
     iv.rebuilt = rebuild;
+    TRY(launch(e, "fill", k_fill<int>, nblocks(e->nstat), e->slotb[slot].blkbad, INT_MAX, e->nstat));
     if (rebuild) {
         int dst = free_set_other_than(e->cur, protect);
         TRY(launch_rebuild(e, dst, slot, false));
-    } else {
-        // flags are reset by k_bin on a rebuild; otherwise reset them here
-        TRY(launch(e, "fill", k_fill<int>, 1, (int*)(e->dflags + slot), 0, 4));
-        TRY(launch(e, "fill", k_fill<int>, 1, (int*)(e->dflags + slot), INT_MAX, 1));
     }
     iv.start_set = e->cur;
     iv.step0 = e->step;
@@ -1091,9 +1649,7 @@ int enqueue_interval(spp_engine* e, int m, bool rebuild, int protect, Interval& 
         TRY(launch_step(e, e->cur, k == 0 ? W : e->cur, k, slot, k == m - 1));
         if (k == 0) e->inj_pending = false;
     }
-    HIPC(e, hipMemcpyAsync(e->hflags + slot, e->dflags + slot, sizeof(Flags), hipMemcpyDeviceToHost, e->stream));
-    HIPC(e, hipMemcpyAsync(e->hblockmax + (size_t)slot * e->nblk, e->dblockmax + (size_t)slot * e->nblk,
-                           e->nblk * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    TRY(launch_report(e, slot, rebuild));
     HIPC(e, hipEventRecord(e->slot_event[slot], e->stream));
     iv.done = e->slot_event[slot];
     return SPP_OK;
@@ -1136,21 +1692,18 @@ int run_intervals(spp_engine* e, int steps)
         Interval iv = q.front();
         HIPC(e, hipEventSynchronize(iv.done));
         e->host_syncs++;
-        const Flags f = e->hflags[iv.slot];
+        const Report f = e->hrep[iv.slot];
         bool overflow = iv.rebuilt && f.max_len > e->cap;
         int bad = period ? INT_MAX : f.bad_step;
         if (iv.rebuilt) {
             e->max_nbrs = f.max_len;
-            e->mean_nbrs = (double)(unsigned)f.nnz / e->N;
+            e->mean_nbrs = (double)f.nnz / e->N;
         }
         if (!overflow && bad >= iv.m) {
             q.pop_front();
             if (!period && iv.rebuilt) {
                 // linear extrapolation of the largest displacement to the trigger
-                const double* bm = e->hblockmax + (size_t)iv.slot * e->nblk;
-                double d2 = 0;
-                for (int b = 0; b < e->nblk; ++b) d2 = std::max(d2, bm[b]);
-                double disp = std::sqrt(d2);
+                double disp = std::sqrt(f.maxdisp2);
                 double est = disp > 0 ? e->thresh / disp * iv.m : 4096.0;
                 e->T_hist[2] = e->T_hist[1];
                 e->T_hist[1] = e->T_hist[0];
@@ -1300,6 +1853,8 @@ int spp_create(const spp_params* p, spp_engine** out)
         return fail(nullptr, SPP_ERR_ARG, "noise_mode %d", p->noise_mode);
     if (p->rebuild != SPP_REBUILD_DISPLACEMENT && p->rebuild != SPP_REBUILD_PERIOD)
         return fail(nullptr, SPP_ERR_ARG, "rebuild %d", p->rebuild);
+    if (p->kernel_variant < SPP_KERNEL_SLOT || p->kernel_variant > SPP_KERNEL_STRIP_LDS)
+        return fail(nullptr, SPP_ERR_ARG, "kernel_variant %d", p->kernel_variant);
     if (p->rebuild == SPP_REBUILD_PERIOD && p->rebuild_period <= 0)
         return fail(nullptr, SPP_ERR_ARG, "rebuild_period must be positive");
     double rcut = max_cutoff(*p);
@@ -1333,6 +1888,19 @@ int spp_create(const spp_params* p, spp_engine** out)
     e->nkeys = ncx * ncy;
     e->nscan_blocks = (e->nkeys + kScanTile - 1) / kScanTile;
     e->nblk = nblocks(e->N);
+    {
+        // strips of ~220 particles (most fit one pass of a 256-thread workgroup); their 3-row
+        // neighbourhood (~3 x 220 records) then fits the LDS budget with room for clustering
+        double occ = (double)e->N / ((double)ncx * ncy);
+        int W = (int)std::lround(220.0 / std::max(occ, 1e-9));
+        W = std::max(1, std::min(ncx, W));
+        e->SG.W = W;
+        e->SG.nsx = (ncx + W - 1) / W;
+        e->SG.nstrips = e->SG.nsx * ncy;
+        e->strip_grid = (e->SG.nstrips + 7) / 8 * 8;
+    }
+    e->variant = p->kernel_variant;
+    e->nstat = std::max(e->strip_grid, e->nblk);
     e->cap = p->ell_capacity > 0 ? p->ell_capacity : 32;
     e->P = make_phys(*p);
     e->T_list = sqrt_threshold(p->r_list);
@@ -1373,22 +1941,34 @@ int spp_create(const spp_params* p, spp_engine** out)
     A(dalloc(e, &e->cell_count, (size_t)e->nkeys));
     A(dalloc(e, &e->cell_start, (size_t)e->nkeys + 1));
     A(dalloc(e, &e->block_sums, (size_t)e->nscan_blocks));
-    A(dalloc(e, &e->dflags, 2));
-    A(dalloc(e, &e->dblockmax, 2 * (size_t)e->nblk));
+    for (int sl = 0; sl < 2; ++sl) {
+        A(dalloc(e, &e->slotb[sl].blkbad, (size_t)e->nstat));
+        A(dalloc(e, &e->slotb[sl].blkmax, (size_t)e->nstat));
+        A(dalloc(e, &e->slotb[sl].blklen, (size_t)e->nstat));
+        A(dalloc(e, &e->slotb[sl].blknnz, (size_t)e->nstat));
+    }
+    A(dalloc(e, &e->strips, (size_t)e->SG.nstrips));
     A(dalloc(e, &e->dpolar, (size_t)e->nblk));
     A(alloc_nbr(e));
-    if (rc == SPP_OK && hipHostMalloc((void**)&e->hflags, 2 * sizeof(Flags), hipHostMallocDefault) != hipSuccess)
+    if (rc == SPP_OK && hipHostMalloc((void**)&e->hrep, 2 * sizeof(Report), hipHostMallocMapped) != hipSuccess)
         rc = SPP_ERR_HIP;
-    if (rc == SPP_OK &&
-        hipHostMalloc((void**)&e->hblockmax, 2 * (size_t)e->nblk * sizeof(double), hipHostMallocDefault) != hipSuccess)
-        rc = SPP_ERR_HIP;
+    if (rc == SPP_OK && hipHostGetDevicePointer((void**)&e->drep, e->hrep, 0) != hipSuccess) rc = SPP_ERR_HIP;
     for (int s = 0; s < 2 && rc == SPP_OK; ++s)
         if (hipEventCreateWithFlags(&e->slot_event[s], hipEventDisableTiming) != hipSuccess) rc = SPP_ERR_HIP;
     if (rc == SPP_OK) {
         // cell counts start at zero and every rebuild's scatter returns them to zero
-        if (launch(e, "fill", k_fill<int>, nblocks(e->nkeys), e->cell_count, 0, e->nkeys) != SPP_OK ||
-            hipStreamSynchronize(e->stream) != hipSuccess)
+        if (launch(e, "fill", k_fill<int>, nblocks(e->nkeys), e->cell_count, 0, e->nkeys) != SPP_OK)
             rc = SPP_ERR_HIP;
+        for (int sl = 0; sl < 2 && rc == SPP_OK; ++sl) {
+            const SlotBufs& b = e->slotb[sl];
+            int G8 = e->nstat;
+            if (launch(e, "fill", k_fill<int>, nblocks(G8), b.blkbad, INT_MAX, G8) != SPP_OK ||
+                launch(e, "fill", k_fill<double>, nblocks(G8), b.blkmax, 0.0, G8) != SPP_OK ||
+                launch(e, "fill", k_fill<int>, nblocks(G8), b.blklen, 0, G8) != SPP_OK ||
+                launch(e, "fill", k_fill<int>, nblocks(G8), b.blknnz, 0, G8) != SPP_OK)
+                rc = SPP_ERR_HIP;
+        }
+        if (rc == SPP_OK && hipStreamSynchronize(e->stream) != hipSuccess) rc = SPP_ERR_HIP;
     }
     if (rc != SPP_OK) {
         std::string msg = e->err.empty() ? std::string("device allocation failed") : e->err;
@@ -1406,9 +1986,15 @@ void spp_destroy(spp_engine* e)
     for (int k = 0; k < 3; ++k) free_set(e->set[k]);
     void* bufs[] = {e->pos1, e->pos2, e->th1, e->th2, e->dth, e->dth_alt, e->inj_host_order,
                     e->inj, e->kacc, e->force, e->old, e->aux_alt[0], e->aux_alt[1], e->aux_alt[2],
-                    e->nbr, e->ncount, e->cellxy, e->build, e->key, e->perm, e->cell_count,
-                    e->cell_start, e->block_sums, e->dflags, e->dblockmax, e->dpolar};
+                    e->nbr16, e->nbr32, e->strips, e->ncount, e->cellxy, e->build, e->key, e->perm, e->cell_count,
+                    e->cell_start, e->block_sums, e->dpolar};
     for (void* b : bufs) dfree(b);
+    for (auto& b : e->slotb) {
+        dfree(b.blkbad);
+        dfree(b.blkmax);
+        dfree(b.blklen);
+        dfree(b.blknnz);
+    }
     for (auto& kv : e->slots) free_set(kv.second.s);
     for (auto& r : e->prof) {
         (void)hipEventDestroy(r.a);
@@ -1417,8 +2003,7 @@ void spp_destroy(spp_engine* e)
     for (auto ev : e->event_pool) (void)hipEventDestroy(ev);
     for (auto ev : e->slot_event)
         if (ev) (void)hipEventDestroy(ev);
-    if (e->hflags) (void)hipHostFree(e->hflags);
-    if (e->hblockmax) (void)hipHostFree(e->hblockmax);
+    if (e->hrep) (void)hipHostFree(e->hrep);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
@@ -1505,11 +2090,15 @@ int spp_get_neighbors(spp_engine* e, int32_t* offsets, int32_t* idx, int64_t cap
 {
     TRY(check_ready(e));
     if (!e->list_valid) TRY(spp_build_neighbors(e));
-    int N = e->N, C = e->cap;
-    std::vector<int> nb((size_t)C * N), cnt(N), id(N);
-    HIPC(e, hipMemcpyAsync(nb.data(), e->nbr, nb.size() * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    int N = e->N, C = e->cap, NS = e->SG.nstrips;
+    std::vector<uint16_t> n16((size_t)C * N);
+    std::vector<int> n32((size_t)C * N), cnt(N), id(N);
+    std::vector<Strip> st(NS);
+    HIPC(e, hipMemcpyAsync(n16.data(), e->nbr16, n16.size() * sizeof(uint16_t), hipMemcpyDeviceToHost, e->stream));
+    HIPC(e, hipMemcpyAsync(n32.data(), e->nbr32, n32.size() * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipMemcpyAsync(cnt.data(), e->ncount, N * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipMemcpyAsync(id.data(), e->set[e->cur].id, N * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIPC(e, hipMemcpyAsync(st.data(), e->strips, NS * sizeof(Strip), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipStreamSynchronize(e->stream));
     std::vector<int64_t> rowlen(N + 1, 0);
     for (int k = 0; k < N; ++k) rowlen[id[k] + 1] = cnt[k];
@@ -1519,9 +2108,29 @@ int spp_get_neighbors(spp_engine* e, int32_t* offsets, int32_t* idx, int64_t cap
     if (!offsets || !idx || cap < total)
         return fail(e, SPP_ERR_CAPACITY, "neighbour list has %lld entries", (long long)total);
     for (int i = 0; i <= N; ++i) offsets[i] = (int32_t)rowlen[i];
-    for (int k = 0; k < N; ++k) {
-        int64_t o = rowlen[id[k]];
-        for (int c = 0; c < cnt[k]; ++c) idx[o + c] = id[nb[(size_t)c * N + k]];
+    if (e->variant != SPP_KERNEL_STRIP_LDS) {
+        for (int k = 0; k < N; ++k) {
+            int64_t o = rowlen[id[k]];
+            for (int c = 0; c < cnt[k]; ++c) idx[o + c] = id[n32[(size_t)c * N + k]];
+        }
+        return SPP_OK;
+    }
+    for (const Strip& S : st) {
+        bool fb = S.total > kLdsCap;
+        for (int k = S.p0; k < S.p0 + S.np; ++k) {
+            int64_t o = rowlen[id[k]];
+            for (int c = 0; c < cnt[k]; ++c) {
+                int slot;
+                if (fb) {
+                    slot = n32[(size_t)c * N + k];
+                } else {
+                    int l = n16[(size_t)c * N + k], q = 0;
+                    while (q < 5 && !(l >= S.poff[q] && l < S.poff[q] + S.plen[q])) ++q;
+                    slot = S.pstart[q] + (l - S.poff[q]);
+                }
+                idx[o + c] = id[slot];
+            }
+        }
     }
     return SPP_OK;
 }
@@ -1547,10 +2156,14 @@ int spp_interact(spp_engine* e)
     StageArgs a{};
     a.N = e->N;
     a.P = e->P;
+    a.SG = e->SG;
+    a.strips = e->strips;
     a.gather = e->set[e->cur].pos;
-    a.nbr = e->nbr;
+    a.nbr16 = e->nbr16;
+    a.nbr32 = e->nbr32;
     a.ncount = e->ncount;
-    InteractFn fn = pick_interact(e->p.model, e->p.repulsion);
+    InteractFn fn = pick_interact(e->p.model, e->p.repulsion, e->variant);
+    if (e->variant == SPP_KERNEL_STRIP_LDS) return launch_strips(e, "interact", fn, a, e->force);
     return launch(e, "interact", fn, e->nblk, a, e->force);
 }
 
@@ -1669,6 +2282,8 @@ int spp_snapshot(spp_engine* e, int32_t slot)
     TRY(copy_set(e, e->set[e->cur], s.s));
     s.step = e->step;
     s.t = e->t;
+    s.T_cross = e->T_cross;
+    for (int k = 0; k < 3; ++k) s.T_hist[k] = e->T_hist[k];
     s.valid = true;
     return SPP_OK;
 }
@@ -1681,6 +2296,9 @@ int spp_restore(spp_engine* e, int32_t slot)
     TRY(copy_set(e, it->second.s, e->set[e->cur]));
     e->step = it->second.step;
     e->t = it->second.t;
+    e->T_cross = it->second.T_cross;
+    for (int k = 0; k < 3; ++k) e->T_hist[k] = it->second.T_hist[k];
+    e->force_m = 0;
     e->unfused_stage = 0;
     // Box::Load ends in a cell update (parallel/box.h:241)
     return rebuild_sync(e, false);
@@ -1737,6 +2355,64 @@ int spp_profile_read(spp_engine* e, char* names, int32_t cap, double* ms_avg, in
         std::strncpy(names, joined.c_str(), cap - 1);
         names[cap - 1] = 0;
     }
+    return SPP_OK;
+}
+
+int spp_time_stage(spp_engine* e, int32_t stage, int32_t reps, double* avg_us)
+{
+    TRY(check_ready(e));
+    int integ = e->p.integrator;
+    int nst = integ == SPP_EULER ? 1 : integ;
+    if (reps <= 0 || !avg_us || (integ == SPP_EULER ? stage != 0 : (stage < 1 || stage > nst)))
+        return fail(e, SPP_ERR_ARG, "stage %d / reps %d", stage, reps);
+    if (e->unfused_stage != 0) return fail(e, SPP_ERR_STATE, "a split step is in progress");
+    const SetPtrs& S = e->set[e->cur];
+    StageArgs a{};
+    a.N = e->N;
+    a.P = e->P;
+    a.SG = e->SG;
+    a.strips = e->strips;
+    a.nbr16 = e->nbr16;
+    a.nbr32 = e->nbr32;
+    a.ncount = e->ncount;
+    a.id = S.id;
+    a.build = e->build;
+    a.step = e->step;
+    a.thresh2 = e->thresh * e->thresh;
+    a.blkbad = e->slotb[0].blkbad;
+    a.blockmax = e->slotb[0].blkmax;
+    a.gather = stage <= 1 ? S.pos : e->pos1;
+    a.th_in = stage <= 1 ? S.th : e->th1;
+    a.th0 = S.th;
+    a.U_in = S.U;
+    // every output goes to scratch: the engine state is left as it was
+    a.out = e->pos2;
+    a.th_out = e->th2;
+    a.U_out = reinterpret_cast<double2*>(e->aux_alt[0]);
+    a.dth = stage <= 1 ? e->dth_alt : e->dth;
+    a.kacc = e->aux_alt[1];
+    StageFn fn = pick_stage(e->p.model, e->p.repulsion, integ, stage, e->variant);
+    hipEvent_t t0, t1;
+    HIPC(e, hipEventCreate(&t0));
+    HIPC(e, hipEventCreate(&t1));
+    bool prof = e->profiling;
+    e->profiling = false;
+    auto go = [&]() -> int {
+        if (e->variant == SPP_KERNEL_STRIP_LDS) return launch_strips(e, "time", fn, a);
+        return launch(e, "time", fn, e->nblk, a);
+    };
+    int rc = go();  // warm
+    if (rc == SPP_OK) rc = hipEventRecord(t0, e->stream) == hipSuccess ? SPP_OK : SPP_ERR_HIP;
+    for (int r = 0; r < reps && rc == SPP_OK; ++r) rc = go();
+    if (rc == SPP_OK) rc = hipEventRecord(t1, e->stream) == hipSuccess ? SPP_OK : SPP_ERR_HIP;
+    float ms = 0;
+    if (rc == SPP_OK) rc = hipEventSynchronize(t1) == hipSuccess ? SPP_OK : SPP_ERR_HIP;
+    if (rc == SPP_OK) rc = hipEventElapsedTime(&ms, t0, t1) == hipSuccess ? SPP_OK : SPP_ERR_HIP;
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    e->profiling = prof;
+    if (rc != SPP_OK) return rc == SPP_ERR_HIP ? fail(e, rc, "event timing failed") : rc;
+    *avg_us = 1e3 * ms / reps;
     return SPP_OK;
 }
 

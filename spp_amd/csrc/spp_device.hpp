@@ -44,10 +44,18 @@ struct Phys {
 
 // ------------------------------------------------------------------------------------------------
 // Periodic_Transform, shared/c2dvector.h:67-73, one component: x -= L2*(int)floor(x/L2 + 0.5).
-// Used for particle positions (once per particle per stage), where it must match bit for bit.
-__device__ __forceinline__ double periodic_wrap(double x, double L2)
+// Positions are wrapped ones plus one stage's displacement, so k = floor(x/L2 + 0.5) is -1, 0 or
+// 1 and is decided by comparing x with +-L; only within 1e-9 L of the edges (where rounding of
+// x/L2 + 0.5 could decide) or far outside is the reference's division evaluated.  Either way the
+// result is bit-identical to the reference formula.
+__device__ __forceinline__ double periodic_wrap(double x, double L, double L2)
 {
-    return x - L2 * (double)(int)floor(x / L2 + 0.5);
+    double ax = fabs(x), k;
+    if (fabs(ax - L) < 1e-9 * L || ax >= 2.9 * L)
+        k = (double)(int)floor(x / L2 + 0.5);
+    else
+        k = ax < L ? 0.0 : (x > 0 ? 1.0 : -1.0);
+    return x - L2 * k;
 }
 
 // Minimum image of a difference of two wrapped coordinates (|dx| <= 2L).  k = rint(dx / 2L) is
@@ -60,10 +68,16 @@ __device__ __forceinline__ double min_image(double dx, double L2, double invL2)
     return fma(-k, L2, dx);
 }
 
-// Theta wrap of the RK stages, shared/particle.h:599.
+// Theta wrap of the RK stages, shared/particle.h:599: th - floor(th/(2 pi) + 0.5)*2*pi, with the
+// same comparison shortcut as periodic_wrap (bit-identical result).
 __device__ __forceinline__ double wrap_angle(double th)
 {
-    return th - floor(th / kTwoPi + 0.5) * 2 * kPi;
+    double a = fabs(th), k;
+    if (fabs(a - kPi) < 1e-9 || a >= 2.9 * kPi)
+        k = floor(th / kTwoPi + 0.5);
+    else
+        k = a < kPi ? 0.0 : (th > 0 ? 1.0 : -1.0);
+    return th - k * 2 * kPi;
 }
 
 // ------------------------------------------------------------------------------------------------

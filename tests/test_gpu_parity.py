@@ -299,3 +299,41 @@ def test_polarization_matches_oracle(oracle_mod):
     gp, gx, gy = e.polarization()
     op, ox, oy = o.polarization()
     assert abs(gp - op) < 1e-13 and abs(gx - ox) < 1e-9 and abs(gy - oy) < 1e-9
+
+
+# ---------------------------------------------------------------------------------------------
+# Interaction-kernel variants: same physics, different gather strategies
+# ---------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("variant", [S.KERNEL_SLOT, S.KERNEL_SLOT_SPLIT, S.KERNEL_STRIP_LDS])
+def test_kernel_variants_match_oracle(oracle_mod, variant):
+    p = c2_params(n=16384, L=64.0, kernel_variant=variant)
+    x, y, th = oracle_mod.random_state(21, p.n, p.Lx, p.Ly)
+    e, o = _pair(p, x, y, th, oracle_mod)
+    off, idx = e.get_neighbors()
+    assert csr_pair_set(off, idx) == pair_set(*o.half_list())
+    e.interact()
+    o.interact()
+    gf, of = e.get_forces(), o.get_forces()
+    assert vec_rel_err(gf[0], gf[1], of[0], of[1]) < FORCE_TOL
+    assert rel_err(gf[2], of[2]) < FORCE_TOL
+    e.multi_step(100)
+    o.multi_step(100)
+    gs, os_ = e.get_state(), o.get_state()
+    assert np.abs(gs["xu"] - os_["xu"]).max() < TRAJ_TOL
+    assert angle_diff(gs["theta"], os_["theta"]).max() < TRAJ_TOL
+
+
+@pytest.mark.parametrize("variant", [S.KERNEL_SLOT, S.KERNEL_SLOT_SPLIT, S.KERNEL_STRIP_LDS])
+def test_kernel_variants_dense_clusters(oracle_mod, variant):
+    """A strip whose neighbourhood overflows the LDS budget takes the global-gather fallback."""
+    n, L = 6000, 16.0
+    p = c2_params(n=n, L=L, kernel_variant=variant)
+    x, y, th = oracle_mod.random_state(14, n, L, L)
+    x = x * 0.3
+    e, o = _pair(p, x, y, th, oracle_mod)
+    e.interact()
+    o.interact()
+    gf, of = e.get_forces(), o.get_forces()
+    assert vec_rel_err(gf[0], gf[1], of[0], of[1]) < FORCE_TOL
+    assert rel_err(gf[2], of[2]) < FORCE_TOL
