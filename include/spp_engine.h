@@ -136,6 +136,18 @@ typedef struct spp_params {
 
 typedef struct spp_engine spp_engine;
 
+/* One independent periodic box of a batched engine (spp_create_batch).  Everything not listed
+   here (model, integrator, laws, radii, dt, r_list, rebuild policy) comes from spp_params.        */
+typedef struct spp_system {
+    int32_t n;              /* particles of this box                                                */
+    int32_t ncell_x;        /* 0 = floor(2 Lx / r_list)                                             */
+    int32_t ncell_y;
+    int32_t reserved;
+    double Lx, Ly;          /* half widths                                                          */
+    double v0, g, D_rot;    /* self-propulsion, alignment coupling, rotational diffusion            */
+    uint64_t seed;          /* noise stream key; systems with equal seeds draw identical noise      */
+} spp_system;
+
 typedef struct spp_info {
     int64_t steps;          /* completed steps (also the noise-stream step counter)                 */
     int64_t rebuilds;       /* neighbour-list rebuilds so far                                       */
@@ -156,6 +168,11 @@ int spp_params_default(spp_params* p, int32_t model);
 
 /* Box::Box + Box::Init (parallel/box.h:69-151): validates parameters, allocates device state.     */
 int spp_create(const spp_params* p, spp_engine** out);
+/* Many independent boxes in one engine: every kernel launch advances all of them (the reference
+   runs independent realizations as separate MPI processes, serial/mpi-main.cpp:157-160).  Caller
+   arrays are the systems' particles concatenated in order.  p->n, p->Lx, p->Ly, p->v0, p->g,
+   p->D_rot and p->seed are ignored in favour of the per-system values.                           */
+int spp_create_batch(const spp_params* p, const spp_system* systems, int32_t nsys, spp_engine** out);
 void spp_destroy(spp_engine* e);
 /* Text of the last error on this engine (e may be NULL for errors raised by spp_create).          */
 const char* spp_last_error(const spp_engine* e);
@@ -205,6 +222,36 @@ int spp_multi_step(spp_engine* e, int32_t steps);
 
 /* Box::Compute_Polarization (serial/box.h:531-538): P = |sum vhat| / N, plus the summed vector.   */
 int spp_polarization(spp_engine* e, double* p, double* sum_x, double* sum_y);
+/* Polarization of each system of a batched engine: p[s] = |sum vhat| / n_s.                     */
+int spp_batch_polarization(spp_engine* e, double* p, int32_t nsys);
+
+/* ---- Lyapunov exponents by finite-difference clones ----------------------------------------------
+   The reference evolves perturbed copies of the state with identical noise and re-orthonormalises
+   their deviations by Gram-Schmidt (parallel/lyapunovbox.h:128-194, shared/vector-set.h:107-127,
+   parallel/separate-lyapunovbox.h:224-305).  Here a batched engine holds `replicas` groups of
+   (1 + vectors) systems, [base, clone_1 .. clone_K], all with identical parameters and seeds.
+   Deviations live in the 3n space (x, y, theta) with the reference's inner product
+   (state-hyper-vector.h:157-167) and minimum-image / wrapped-angle differences (:96-106, :223).   */
+/* Sets every clone to base + epsilon * (orthonormalised random deviation).  rescale = 1: every
+   renormalisation restarts the clones at distance epsilon (Benettin).  rescale = 0: the reference's
+   choice -- the projected deviation keeps its grown length (lyapunovbox.h:177-183).  Both
+   accumulate the same log growth in the linear regime.  Needs an engine from spp_create_batch
+   with replicas * (1 + vectors) systems, all with the same n; a replica's systems need equal
+   boxes, parameters and seeds.  Clone states are overwritten; the bases keep theirs.             */
+int spp_lyapunov_init(spp_engine* e, int32_t replicas, int32_t vectors, double epsilon, int32_t rescale,
+                      uint64_t seed);
+/* Gram-Schmidt (QR) of the current deviations, adds log(R_kk / start length) to the accumulators
+   and resets the clones to base + length * Q.  Optional output (synchronises): the accumulated
+   sums, replicas x vectors.                                                                     */
+int spp_lyapunov_renormalize(spp_engine* e, double* log_growth);
+/* `steps` steps, renormalising every `interval` steps (steps must be a multiple of interval).    */
+int spp_lyapunov_run(spp_engine* e, int32_t steps, int32_t interval);
+/* Accumulated log growth per (replica, vector) and the time it covers: lambda = sum / time.      */
+int spp_lyapunov_get(spp_engine* e, double* sum_log, double* time);
+/* The orthonormal directions of one replica after the last renormalisation, laid out
+   [vector][x|y|theta][particle] (len = vectors * 3 * n).                                          */
+int spp_lyapunov_vectors(spp_engine* e, int32_t replica, double* out, int64_t len);
+
 /* Box::Save / Box::Load (parallel/box.h:222-267) into device-resident slots, including the noise
    step counter (the counter-based stream makes replay exact).                                    */
 int spp_snapshot(spp_engine* e, int32_t slot);

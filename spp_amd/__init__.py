@@ -105,6 +105,23 @@ class Info(C.Structure):
     ]
 
 
+class System(C.Structure):
+    """Mirror of ``spp_system`` (one box of a batched engine)."""
+
+    _fields_ = [
+        ("n", C.c_int32),
+        ("ncell_x", C.c_int32),
+        ("ncell_y", C.c_int32),
+        ("reserved", C.c_int32),
+        ("Lx", C.c_double),
+        ("Ly", C.c_double),
+        ("v0", C.c_double),
+        ("g", C.c_double),
+        ("D_rot", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+
 # Exported symbols and their signatures (every declaration of include/spp_engine.h).
 _D = C.POINTER(C.c_double)
 _I32 = C.POINTER(C.c_int32)
@@ -112,6 +129,7 @@ _I64 = C.POINTER(C.c_int64)
 SIGNATURES = {
     "spp_params_default": (C.c_int, [C.POINTER(Params), C.c_int32]),
     "spp_create": (C.c_int, [C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    "spp_create_batch": (C.c_int, [C.POINTER(Params), C.POINTER(System), C.c_int32, C.POINTER(C.c_void_p)]),
     "spp_destroy": (None, [C.c_void_p]),
     "spp_last_error": (C.c_char_p, [C.c_void_p]),
     "spp_get_info": (C.c_int, [C.c_void_p, C.POINTER(Info)]),
@@ -127,6 +145,12 @@ SIGNATURES = {
     "spp_one_step": (C.c_int, [C.c_void_p]),
     "spp_multi_step": (C.c_int, [C.c_void_p, C.c_int32]),
     "spp_polarization": (C.c_int, [C.c_void_p, _D, _D, _D]),
+    "spp_batch_polarization": (C.c_int, [C.c_void_p, _D, C.c_int32]),
+    "spp_lyapunov_init": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_uint64]),
+    "spp_lyapunov_renormalize": (C.c_int, [C.c_void_p, _D]),
+    "spp_lyapunov_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "spp_lyapunov_get": (C.c_int, [C.c_void_p, _D, _D]),
+    "spp_lyapunov_vectors": (C.c_int, [C.c_void_p, C.c_int32, _D, C.c_int64]),
     "spp_snapshot": (C.c_int, [C.c_void_p, C.c_int32]),
     "spp_restore": (C.c_int, [C.c_void_p, C.c_int32]),
     "spp_synchronize": (C.c_int, [C.c_void_p]),
@@ -189,18 +213,35 @@ def noise_normal(seed: int, pid: int, step: int) -> float:
     return lib().spp_noise_normal(seed, pid, step)
 
 
-class Engine:
-    """One engine (device state + stream) for one box of particles."""
+def system(n: int, Lx: float, Ly: float | None = None, v0: float = 1.0, g: float = 1.0,
+           D_rot: float = 0.0, seed: int = 1, ncell_x: int = 0, ncell_y: int = 0) -> System:
+    s = System()
+    s.n, s.Lx, s.Ly = n, Lx, Lx if Ly is None else Ly
+    s.v0, s.g, s.D_rot, s.seed = v0, g, D_rot, seed
+    s.ncell_x, s.ncell_y = ncell_x, ncell_y
+    return s
 
-    def __init__(self, params: Params):
+
+class Engine:
+    """One engine (device state + stream) for one box of particles, or for a batch of independent
+    boxes when ``systems`` is given (caller arrays are then the systems' particles concatenated)."""
+
+    def __init__(self, params: Params, systems=None):
         self._lib = lib()
         self.params = params.copy()
         h = C.c_void_p()
-        rc = self._lib.spp_create(C.byref(self.params), C.byref(h))
+        if systems is None:
+            rc = self._lib.spp_create(C.byref(self.params), C.byref(h))
+            self.sizes = [params.n]
+        else:
+            arr = (System * len(systems))(*systems)
+            rc = self._lib.spp_create_batch(C.byref(self.params), arr, len(systems), C.byref(h))
+            self.sizes = [s.n for s in systems]
         if rc:
             raise SppError(rc, self._lib.spp_last_error(None).decode())
         self._h = h
-        self.n = params.n
+        self.n = int(sum(self.sizes))
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
 
     # -- plumbing ------------------------------------------------------------------------------
     def _check(self, rc: int):
@@ -286,6 +327,44 @@ class Engine:
         p, sx, sy = C.c_double(), C.c_double(), C.c_double()
         self._check(self._lib.spp_polarization(self._h, C.byref(p), C.byref(sx), C.byref(sy)))
         return p.value, sx.value, sy.value
+
+    def batch_polarization(self):
+        p = np.empty(len(self.sizes))
+        self._check(self._lib.spp_batch_polarization(self._h, _ptr(p), len(self.sizes)))
+        return p
+
+    # -- Lyapunov clones (batched engine: replicas x (base + vectors) systems) -------------------
+    def lyapunov_init(self, replicas: int, vectors: int, epsilon: float, rescale: bool = True, seed: int = 1):
+        self._lyap = (replicas, vectors)
+        self._check(self._lib.spp_lyapunov_init(self._h, replicas, vectors, epsilon, 1 if rescale else 0, seed))
+
+    def lyapunov_renormalize(self, want: bool = False):
+        if not want:
+            self._check(self._lib.spp_lyapunov_renormalize(self._h, None))
+            return None
+        R, K = self._lyap
+        out = np.empty(R * K)
+        self._check(self._lib.spp_lyapunov_renormalize(self._h, _ptr(out)))
+        return out.reshape(R, K)
+
+    def lyapunov_run(self, steps: int, interval: int):
+        self._check(self._lib.spp_lyapunov_run(self._h, steps, interval))
+
+    def lyapunov_get(self):
+        """(accumulated log growth [replica, vector], time covered)."""
+        R, K = self._lyap
+        out = np.empty(R * K)
+        t = C.c_double()
+        self._check(self._lib.spp_lyapunov_get(self._h, _ptr(out), C.byref(t)))
+        return out.reshape(R, K), t.value
+
+    def lyapunov_vectors(self, replica: int):
+        """Orthonormal directions of one replica, shape [vector, 3 (x, y, theta), n]."""
+        R, K = self._lyap
+        n = self.sizes[0]
+        out = np.empty(K * 3 * n)
+        self._check(self._lib.spp_lyapunov_vectors(self._h, replica, _ptr(out), out.size))
+        return out.reshape(K, 3, n)
 
     def snapshot(self, slot: int = 0):
         self._check(self._lib.spp_snapshot(self._h, slot))

@@ -87,6 +87,32 @@ struct Grid {
     int ncx, ncy, quirk;
 };
 
+// Per-system constants of a batched engine (one independent periodic box per system).  Systems
+// are concatenated; cell keys are system-major, so after every sort system s occupies the slots
+// [pid_off, pid_off + n) and its cells the keys [key_off, key_off + ncx*ncy).
+struct SysInfo {
+    Grid G;
+    int key_off, pid_off, n, pad;
+    double v0, c_align, g, noise_amp;
+    uint64_t seed;
+};
+
+// Overrides the per-system fields of the launch constants for a batched engine.
+__device__ __forceinline__ void apply_sys(Phys& P, const SysInfo& S)
+{
+    P.Lx = S.G.Lx;
+    P.Ly = S.G.Ly;
+    P.Lx2 = S.G.Lx2;
+    P.Ly2 = S.G.Ly2;
+    P.invLx2 = S.G.invLx2;
+    P.invLy2 = S.G.invLy2;
+    P.v0 = S.v0;
+    P.c_align = S.c_align;
+    P.g = S.g;
+    P.noise_amp = S.noise_amp;
+    P.seed = S.seed;
+}
+
 template <typename T>
 __global__ void k_fill(T* a, T v, int n)
 {
@@ -113,17 +139,24 @@ __device__ __forceinline__ double wave_max(double v)
 // (Box::Update_Cells + Update_Neighbor_List, serial/box.h:107-179)
 // ------------------------------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kBlock) void k_bin(int N, Grid G, const P4* __restrict__ pos,
+__global__ __launch_bounds__(kBlock) void k_bin(int N, Grid G0, const P4* __restrict__ pos,
                                                 int* __restrict__ key, int* __restrict__ count,
-                                                int* __restrict__ blkbad)
+                                                const SysInfo* __restrict__ sys_tab,
+                                                const int* __restrict__ sys_of)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (threadIdx.x == 0 && blkbad) blkbad[blockIdx.x] = INT_MAX;  // grid == stage grid
     if (i >= N) return;
+    Grid G = G0;
+    int koff = 0;
+    if (sys_of) {
+        const SysInfo& S = sys_tab[sys_of[i]];
+        G = S.G;
+        koff = S.key_off;
+    }
     P4 p = pos[i];
     int cx = cell_coord(p.x, G.Lx, G.Lx2, G.ncx, G.quirk);
     int cy = cell_coord(p.y, G.Ly, G.Ly2, G.ncy, G.quirk);
-    int k = cy * G.ncx + cx;
+    int k = koff + cy * G.ncx + cx;
     key[i] = k;
     atomicAdd(&count[k], 1);
 }
@@ -490,6 +523,8 @@ __global__ __launch_bounds__(kStripThreads) void k_nlist(NlistArgs a)
 struct StageArgs {
     int N;
     Phys P;
+    const SysInfo* sys_tab;  // batched engine (null: single system, constants in P)
+    const int* sys_of;
     StripGeom SG;
     const Strip* strips;
     const P4* gather;      // stage input: neighbours' (and own) position and heading
@@ -550,14 +585,14 @@ __device__ __forceinline__ void gather_forces(const StageArgs& a, int i, const P
 
 // Displacement trigger: marks the step (per strip, atomics stay inside one workgroup's slot) and,
 // on the last step of an interval, records the strip's largest squared displacement.
-__device__ __forceinline__ void check_disp(const StageArgs& a, int s, bool live, int i, double x,
-                                           double y, int mark, double& dmax)
+__device__ __forceinline__ void check_disp(const StageArgs& a, const Phys& P, int s, bool live, int i,
+                                           double x, double y, int mark, double& dmax)
 {
     double d2 = 0;
     if (live) {
         float2 b = a.build[i];
-        double dx = spp::min_image(x - (double)b.x, a.P.Lx2, a.P.invLx2);
-        double dy = spp::min_image(y - (double)b.y, a.P.Ly2, a.P.invLy2);
+        double dx = spp::min_image(x - (double)b.x, P.Lx2, P.invLx2);
+        double dy = spp::min_image(y - (double)b.y, P.Ly2, P.invLy2);
         d2 = dx * dx + dy * dy;
     }
     if (__any(d2 > a.thresh2) && (threadIdx.x & 63) == 0) atomicMin(&a.blkbad[s], mark);
@@ -654,7 +689,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int s, const Stri
             a.out[i] = P4{x, y, cs, sn};
             a.th_out[i] = th;
         }
-        check_disp(a, s, live, i, x, y, final_stage ? a.kstep + 1 : a.kstep, dmax);
+        check_disp(a, P, s, live, i, x, y, final_stage ? a.kstep + 1 : a.kstep, dmax);
     }
     finish_disp(a, s, dmax);
 }
@@ -684,6 +719,8 @@ __global__ __launch_bounds__(kStripThreads) void k_stage(StageArgs a)
 struct NlistFlatArgs {
     int N, cap;
     Grid G;
+    const SysInfo* sys_tab;  // batched engine (null: single system)
+    const int* sys_of;
     double T_list;
     const P4* pos;
     const int* start;
@@ -704,7 +741,13 @@ __global__ __launch_bounds__(kBlock) void k_nlist_flat(NlistFlatArgs a)
     bool live = i < a.N;
     int c = 0;
     if (live) {
-        const Grid& G = a.G;
+        Grid G = a.G;
+        int koff = 0;
+        if (a.sys_of) {
+            const SysInfo& S = a.sys_tab[a.sys_of[i]];
+            G = S.G;
+            koff = S.key_off;
+        }
         P4 me = a.pos[i];
         int cx = cell_coord(me.x, G.Lx, G.Lx2, G.ncx, G.quirk);
         int cy = cell_coord(me.y, G.Ly, G.Ly2, G.ncy, G.quirk);
@@ -719,7 +762,7 @@ __global__ __launch_bounds__(kBlock) void k_nlist_flat(NlistFlatArgs a)
         for (int oy = -1; oy <= 1; ++oy) {
             int y = cy + oy;
             y = y < 0 ? y + G.ncy : (y >= G.ncy ? y - G.ncy : y);
-            int row = y * G.ncx;
+            int row = koff + y * G.ncx;
             for (int seg = 0; seg < 2; ++seg) {
                 int s, e;
                 if (seg == 0) {
@@ -776,14 +819,14 @@ __global__ __launch_bounds__(kBlock) void k_nlist_flat(NlistFlatArgs a)
 }
 
 template <int MODEL, int REP, bool SPLIT>
-__device__ __forceinline__ void gather_forces_flat(const StageArgs& a, int i, const P4& me, double& fx,
-                                                   double& fy, double& tq)
+__device__ __forceinline__ void gather_forces_flat(const StageArgs& a, const Phys& P, int i, const P4& me,
+                                                   double& fx, double& fy, double& tq)
 {
     const int cnt = a.ncount[i];
     const int* nb = a.nbr32 + i;
     double s1 = 0, s2 = 0;
-    fx = me.c * a.P.v0;  // Reset: f = v (speed v0), particle.h:559-564
-    fy = me.s * a.P.v0;
+    fx = me.c * P.v0;  // Reset: f = v (speed v0), particle.h:559-564
+    fy = me.s * P.v0;
     for (int k = 0; k < cnt; k += 4) {
         int j[4];
 #pragma unroll
@@ -796,7 +839,7 @@ __device__ __forceinline__ void gather_forces_flat(const StageArgs& a, int i, co
                 if (k + u < cnt) q[u] = a.gather[j[u]];
 #pragma unroll
             for (int u = 0; u < 4; ++u)
-                if (k + u < cnt) spp::pair_accumulate<MODEL, REP>(a.P, me, q[u], fx, fy, s1, s2);
+                if (k + u < cnt) spp::pair_accumulate<MODEL, REP>(P, me, q[u], fx, fy, s1, s2);
         } else {
             P4 q[4];
 #pragma unroll
@@ -811,10 +854,10 @@ __device__ __forceinline__ void gather_forces_flat(const StageArgs& a, int i, co
             for (int u = 0; u < 4; ++u) {
                 in[u] = false;
                 if (k + u < cnt) {
-                    double dx = spp::min_image(me.x - q[u].x, a.P.Lx2, a.P.invLx2);
-                    double dy = spp::min_image(me.y - q[u].y, a.P.Ly2, a.P.invLy2);
+                    double dx = spp::min_image(me.x - q[u].x, P.Lx2, P.invLx2);
+                    double dy = spp::min_image(me.y - q[u].y, P.Ly2, P.invLy2);
                     double d2 = dx * dx + dy * dy;
-                    in[u] = MODEL == SPP_MODEL_CONTINUOUS ? d2 < a.P.Ra2 : d2 < a.P.T_max;
+                    in[u] = MODEL == SPP_MODEL_CONTINUOUS ? d2 < P.Ra2 : d2 < P.T_max;
                 }
             }
 #pragma unroll
@@ -826,19 +869,25 @@ __device__ __forceinline__ void gather_forces_flat(const StageArgs& a, int i, co
                 }
 #pragma unroll
             for (int u = 0; u < 4; ++u)
-                if (in[u]) spp::pair_accumulate<MODEL, REP>(a.P, me, q[u], fx, fy, s1, s2);
+                if (in[u]) spp::pair_accumulate<MODEL, REP>(P, me, q[u], fx, fy, s1, s2);
         }
     }
-    tq = spp::drift_torque<MODEL>(a.P, s1, s2);
+    tq = spp::drift_torque<MODEL>(P, s1, s2);
 }
 
-template <int MODEL, int REP, int INTEG, int STAGE, bool SPLIT>
+template <int MODEL, int REP, int INTEG, int STAGE, bool SPLIT, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_stage_flat(StageArgs a)
 {
     int b = xcd_block();
     int i = b * kBlock + threadIdx.x;
     bool live = i < a.N;
-    const Phys& P = a.P;
+    Phys P = a.P;
+    int pid_off = 0;
+    if (BATCH && live) {
+        const SysInfo& S = a.sys_tab[a.sys_of[i]];
+        apply_sys(P, S);
+        pid_off = S.pid_off;
+    }
     constexpr bool final_stage = (INTEG == SPP_EULER) || (INTEG == SPP_RK2 && STAGE == 2) || (STAGE == 4);
     double x = 0, y = 0, dmax = 0;
     if (live) {
@@ -851,7 +900,9 @@ __global__ __launch_bounds__(kBlock) void k_stage_flat(StageArgs a)
             if (a.id_out) a.id_out[i] = pid;
             th_in = a.th_in[i];
             if (INTEG == SPP_EULER) u = a.U_in[i];
-            dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)pid, a.step);
+            // the stream is addressed by the particle's index within its system, so systems with
+            // equal seeds (Lyapunov clones) draw identical noise
+            dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)(pid - pid_off), a.step);
         } else {
             thold = a.th0[i];
             dn = a.dth[i];
@@ -859,7 +910,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_flat(StageArgs a)
             if (INTEG == SPP_RK4) kk = a.kacc[i];
         }
         double fx, fy, tq;
-        gather_forces_flat<MODEL, REP, SPLIT>(a, i, me, fx, fy, tq);
+        gather_forces_flat<MODEL, REP, SPLIT>(a, P, i, me, fx, fy, tq);
         double th;
         if (STAGE <= 1) {
             if (INTEG == SPP_EULER) {
@@ -904,18 +955,20 @@ __global__ __launch_bounds__(kBlock) void k_stage_flat(StageArgs a)
         a.out[i] = P4{x, y, cs, sn};
         a.th_out[i] = th;
     }
-    check_disp(a, b, live, i, x, y, final_stage ? a.kstep + 1 : a.kstep, dmax);
+    check_disp(a, P, b, live, i, x, y, final_stage ? a.kstep + 1 : a.kstep, dmax);
     finish_disp(a, b, dmax);
 }
 
-template <int MODEL, int REP>
+template <int MODEL, int REP, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_interact_flat(StageArgs a, P4* force)
 {
     int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.N) return;
+    Phys P = a.P;
+    if (BATCH) apply_sys(P, a.sys_tab[a.sys_of[i]]);
     P4 me = a.gather[i];
     double fx, fy, tq;
-    gather_forces_flat<MODEL, REP, false>(a, i, me, fx, fy, tq);
+    gather_forces_flat<MODEL, REP, false>(a, P, i, me, fx, fy, tq);
     force[i] = P4{fx, fy, tq, 0.0};
 }
 
@@ -949,6 +1002,8 @@ __global__ __launch_bounds__(kStripThreads) void k_interact(StageArgs a, P4* for
 struct MoveArgs {
     int N;
     Phys P;
+    const SysInfo* sys_tab;
+    const int* sys_of;
     int integ, stage;
     SetPtrs cur;
     P4* old;     // r_old.x, r_old.y, theta_old
@@ -963,14 +1018,20 @@ __global__ __launch_bounds__(kBlock) void k_move(MoveArgs a)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.N) return;
-    const Phys& P = a.P;
+    Phys P = a.P;
+    int pid_off = 0;
+    if (a.sys_of) {
+        const SysInfo& S = a.sys_tab[a.sys_of[i]];
+        apply_sys(P, S);
+        pid_off = S.pid_off;
+    }
     P4 me = a.cur.pos[i];
     double th_in = a.cur.th[i];
     P4 f = a.force[i];
     double2 u = a.cur.U[i];
     double th, x, y;
     if (a.stage <= 1) {
-        double dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)a.cur.id[i], a.step);
+        double dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)(a.cur.id[i] - pid_off), a.step);
         a.dth[i] = dn;
         if (a.integ == SPP_EULER) {
             th = th_in + P.dt * (f.c);
@@ -1082,6 +1143,37 @@ __global__ __launch_bounds__(kBlock) void k_polar(int N, const P4* __restrict__ 
     }
 }
 
+// Per-system heading sums of a batched engine: one workgroup per system over its slot range.
+__global__ __launch_bounds__(kBlock) void k_polar_sys(const SysInfo* __restrict__ sys, const P4* __restrict__ pos,
+                                                      double2* __restrict__ out)
+{
+    __shared__ double sx[kBlock / 64], sy[kBlock / 64];
+    const SysInfo& S = sys[blockIdx.x];
+    double vx = 0, vy = 0;
+    for (int i = S.pid_off + threadIdx.x; i < S.pid_off + S.n; i += kBlock) {
+        vx += pos[i].c;
+        vy += pos[i].s;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        vx += __shfl_xor(vx, o);
+        vy += __shfl_xor(vy, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sx[threadIdx.x >> 6] = vx;
+        sy[threadIdx.x >> 6] = vy;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0, b = 0;
+        for (int k = 0; k < kBlock / 64; ++k) {
+            a += sx[k];
+            b += sy[k];
+        }
+        out[blockIdx.x] = make_double2(a, b);
+    }
+}
+
 // Reduces one interval's per-block partials into the host-mapped report (one block).
 __global__ __launch_bounds__(kBlock) void k_report(int nblk, SlotBufs b, int with_list, Report* out)
 {
@@ -1139,14 +1231,16 @@ __global__ __launch_bounds__(kBlock) void k_report(int nblk, SlotBufs b, int wit
 // ------------------------------------------------------------------------------------------------
 
 using StageFn = void (*)(StageArgs);
+constexpr int kVariantBatch = -1;  // internal: slot-parallel kernels with per-system constants
 using InteractFn = void (*)(StageArgs, P4*);
 
 template <int MODEL, int REP, int I, int ST>
 StageFn variant_fn(int variant)
 {
     if (variant == SPP_KERNEL_STRIP_LDS) return k_stage<MODEL, REP, I, ST>;
-    if (variant == SPP_KERNEL_SLOT_SPLIT) return k_stage_flat<MODEL, REP, I, ST, true>;
-    return k_stage_flat<MODEL, REP, I, ST, false>;
+    if (variant == SPP_KERNEL_SLOT_SPLIT) return k_stage_flat<MODEL, REP, I, ST, true, false>;
+    if (variant == kVariantBatch) return k_stage_flat<MODEL, REP, I, ST, false, true>;
+    return k_stage_flat<MODEL, REP, I, ST, false, false>;
 }
 
 template <int MODEL, int REP>
@@ -1180,7 +1274,8 @@ template <int MODEL, int REP>
 InteractFn interact_fn(int variant)
 {
     if (variant == SPP_KERNEL_STRIP_LDS) return k_interact<MODEL, REP>;
-    return k_interact_flat<MODEL, REP>;
+    if (variant == kVariantBatch) return k_interact_flat<MODEL, REP, true>;
+    return k_interact_flat<MODEL, REP, false>;
 }
 
 InteractFn pick_interact(int model, int rep, int variant)
@@ -1229,6 +1324,8 @@ double sqrt_threshold(double R)
 
 }  // namespace
 
+#include "lyapunov_kernels.hpp"
+
 // ==================================================================================================
 // Engine
 // ==================================================================================================
@@ -1256,6 +1353,18 @@ struct Interval {
     double t0;
     bool rebuilt;
     hipEvent_t done;
+};
+
+// Tangent-vector state of a Lyapunov engine (device arrays described in lyapunov_kernels.hpp).
+struct LyapState {
+    bool on = false;
+    int R = 0, K = 0, n = 0, nchunk = 0, rescale = 1;
+    double eps = 0, t_start = 0, t_last = 0;
+    int64_t renorms = 0;
+    int* inv = nullptr;
+    int* status = nullptr;
+    double *dev = nullptr, *part = nullptr, *coef = nullptr, *rdiag = nullptr, *logsum = nullptr,
+           *norm0 = nullptr;
 };
 
 struct spp_engine {
@@ -1291,8 +1400,15 @@ struct spp_engine {
     Report* hrep = nullptr;      // host-mapped reports, 2 slots
     Report* drep = nullptr;      // device view of hrep
     double2* dpolar = nullptr;
+    // batched engine: per-system constants and the (fixed) slot -> system map; null for one box
+    int nsys = 1;
+    std::vector<SysInfo> hsys;
+    SysInfo* dsys = nullptr;
+    int* sys_of = nullptr;
+    double2* dpolar_sys = nullptr;
     hipEvent_t slot_event[2] = {nullptr, nullptr};
     std::map<int, Snapshot> slots;
+    LyapState ly;
 
     int64_t step = 0;
     double t = 0;
@@ -1449,7 +1565,7 @@ void free_set(SetPtrs& s)
 int launch_nlist(spp_engine* e, int slot)
 {
     if (e->variant != SPP_KERNEL_STRIP_LDS) {
-        NlistFlatArgs fa{e->N, e->cap, e->G, e->T_list, e->set[e->cur].pos, e->cell_start, e->nbr32,
+        NlistFlatArgs fa{e->N, e->cap, e->G, e->dsys, e->sys_of, e->T_list, e->set[e->cur].pos, e->cell_start, e->nbr32,
                          e->ncount, e->build, e->cellxy, e->slotb[slot].blklen, e->slotb[slot].blknnz};
         return launch(e, "nlist", k_nlist_flat, e->nblk, fa);
     }
@@ -1468,7 +1584,7 @@ int launch_rebuild(spp_engine* e, int dst, int slot, bool aux)
 {
     int N = e->N, nk = e->nkeys, src = e->cur;
     TRY(launch(e, "bin", k_bin, nblocks(N), N, e->G, (const P4*)e->set[src].pos, e->key,
-               e->cell_count, (int*)nullptr));
+               e->cell_count, (const SysInfo*)e->dsys, (const int*)e->sys_of));
     TRY(launch(e, "scan_partial", k_scan_partial, e->nscan_blocks, (const int*)e->cell_count, nk,
                e->block_sums));
     TRY(launch(e, "scan_final", k_scan_final, e->nscan_blocks, (const int*)e->cell_count, nk,
@@ -1550,6 +1666,8 @@ int launch_step(spp_engine* e, int S, int W, int kstep, int slot, bool record)
     StageArgs a{};
     a.N = e->N;
     a.P = e->P;
+    a.sys_tab = e->dsys;
+    a.sys_of = e->sys_of;
     a.SG = e->SG;
     a.strips = e->strips;
     a.dth = e->dth;
@@ -1748,6 +1866,79 @@ int run_intervals(spp_engine* e, int steps)
     return SPP_OK;
 }
 
+void lyap_free(LyapState& L)
+{
+    void* bufs[] = {L.inv, L.status, L.dev, L.part, L.coef, L.rdiag, L.logsum, L.norm0};
+    for (void* b : bufs) dfree(b);
+    L = LyapState{};
+}
+
+lyap::Args lyap_args(spp_engine* e)
+{
+    const LyapState& L = e->ly;
+    SetPtrs& s = e->set[e->cur];
+    lyap::Args a{};
+    a.R = L.R;
+    a.K = L.K;
+    a.n = L.n;
+    a.nchunk = L.nchunk;
+    a.sys = e->dsys;
+    a.inv = L.inv;
+    a.pos = s.pos;
+    a.th = s.th;
+    a.U = s.U;
+    a.dev = L.dev;
+    a.part = L.part;
+    a.coef = L.coef;
+    a.rdiag = L.rdiag;
+    a.logsum = L.logsum;
+    a.norm0 = L.norm0;
+    a.status = L.status;
+    return a;
+}
+
+template <int K>
+int lyap_qr_k(spp_engine* e, const lyap::Args& a, bool init)
+{
+    const LyapState& L = e->ly;
+    for (int pass = 0; pass < 2; ++pass) {
+        TRY(launch(e, "lyap_gram", lyap::k_gram<K>, L.R * L.nchunk, a));
+        TRY(launch(e, "lyap_chol", lyap::k_chol, nblocks(L.R), a, pass, init ? 1 : 0, L.rescale, L.eps));
+        TRY(launch(e, "lyap_apply", lyap::k_apply<K>, nblocks(L.R * L.n), a));
+    }
+    return SPP_OK;
+}
+
+// Orthonormalise the deviations in e->ly.dev (Cholesky QR, twice), accumulate the growth and put
+// every clone at base + scale * direction.
+int lyap_qr(spp_engine* e, bool init)
+{
+    lyap::Args a = lyap_args(e);
+    switch (e->ly.K) {
+    case 1: return lyap_qr_k<1>(e, a, init);
+    case 2: return lyap_qr_k<2>(e, a, init);
+    case 3: return lyap_qr_k<3>(e, a, init);
+    case 4: return lyap_qr_k<4>(e, a, init);
+    case 5: return lyap_qr_k<5>(e, a, init);
+    case 6: return lyap_qr_k<6>(e, a, init);
+    case 7: return lyap_qr_k<7>(e, a, init);
+    case 8: return lyap_qr_k<8>(e, a, init);
+    }
+    return fail(e, SPP_ERR_ARG, "vectors must be 1..%d", lyap::kMaxK);
+}
+
+int lyap_check_status(spp_engine* e)
+{
+    std::vector<int> st(e->ly.R);
+    HIPC(e, hipMemcpyAsync(st.data(), e->ly.status, st.size() * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIPC(e, hipStreamSynchronize(e->stream));
+    for (int r = 0; r < e->ly.R; ++r)
+        if (st[r])
+            return fail(e, SPP_ERR_STATE, "replica %d: tangent vectors became linearly dependent "
+                        "(Gram matrix not positive definite)", r);
+    return SPP_OK;
+}
+
 Phys make_phys(const spp_params& p)
 {
     Phys P{};
@@ -1835,7 +2026,18 @@ int spp_params_default(spp_params* p, int32_t model)
     return SPP_OK;
 }
 
-int spp_create(const spp_params* p, spp_engine** out)
+}  // extern "C"
+
+namespace {
+
+// One box of an engine before validation: spp_create describes one, spp_create_batch many.
+struct SysDesc {
+    int n, ncx, ncy;
+    double Lx, Ly, v0, g, D;
+    uint64_t seed;
+};
+
+int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_engine** out)
 {
     if (!p || !out) return fail(nullptr, SPP_ERR_ARG, "null argument");
     *out = nullptr;
@@ -1847,50 +2049,79 @@ int spp_create(const spp_params* p, spp_engine** out)
         return fail(nullptr, SPP_ERR_ARG, "integrator %d", p->integrator);
     if (p->repulsion < SPP_REP_NONE || p->repulsion > SPP_REP_YUKAWA)
         return fail(nullptr, SPP_ERR_ARG, "repulsion %d", p->repulsion);
-    if (p->n <= 0 || !(p->Lx > 0) || !(p->Ly > 0) || !(p->dt > 0) || !(p->r_list > 0))
-        return fail(nullptr, SPP_ERR_ARG, "n, Lx, Ly, dt and r_list must be positive");
+    if (!(p->dt > 0) || !(p->r_list > 0)) return fail(nullptr, SPP_ERR_ARG, "dt and r_list must be positive");
     if (p->noise_mode != SPP_NOISE_STREAM && p->noise_mode != SPP_NOISE_INJECTED)
         return fail(nullptr, SPP_ERR_ARG, "noise_mode %d", p->noise_mode);
     if (p->rebuild != SPP_REBUILD_DISPLACEMENT && p->rebuild != SPP_REBUILD_PERIOD)
         return fail(nullptr, SPP_ERR_ARG, "rebuild %d", p->rebuild);
     if (p->kernel_variant < SPP_KERNEL_SLOT || p->kernel_variant > SPP_KERNEL_STRIP_LDS)
         return fail(nullptr, SPP_ERR_ARG, "kernel_variant %d", p->kernel_variant);
+    if (systems && p->kernel_variant != SPP_KERNEL_SLOT)
+        return fail(nullptr, SPP_ERR_UNSUPPORTED, "batched engines run the slot kernels (kernel_variant 0)");
     if (p->rebuild == SPP_REBUILD_PERIOD && p->rebuild_period <= 0)
         return fail(nullptr, SPP_ERR_ARG, "rebuild_period must be positive");
     double rcut = max_cutoff(*p);
     if (p->r_list < rcut) return fail(nullptr, SPP_ERR_ARG, "r_list %g below cutoff %g", p->r_list, rcut);
-    int ncx = p->ncell_x > 0 ? p->ncell_x : (int)std::floor(2 * p->Lx / p->r_list);
-    int ncy = p->ncell_y > 0 ? p->ncell_y : (int)std::floor(2 * p->Ly / p->r_list);
-    // Cell::Cell (shared/cell.h:45-49): cells no narrower than rv.  Three or more per side keep
-    // the nine stencil cells distinct.
-    if (ncx < 3 || ncy < 3 || 2 * p->Lx / ncx < p->r_list || 2 * p->Ly / ncy < p->r_list)
-        return fail(nullptr, SPP_ERR_ARG, "cell grid %dx%d invalid for r_list %g in box %gx%g", ncx,
-                    ncy, p->r_list, 2 * p->Lx, 2 * p->Ly);
-    if ((int64_t)ncx * ncy >= (1 << 30) || ncx >= 32768 || ncy >= 32768)
-        return fail(nullptr, SPP_ERR_ARG, "cell grid too large");
+
+    std::vector<SysDesc> sd(nsys);
+    for (int s = 0; s < nsys; ++s) {
+        if (systems) {
+            const spp_system& q = systems[s];
+            sd[s] = SysDesc{q.n, q.ncell_x, q.ncell_y, q.Lx, q.Ly, q.v0, q.g, q.D_rot, q.seed};
+        } else {
+            sd[s] = SysDesc{p->n, p->ncell_x, p->ncell_y, p->Lx, p->Ly, p->v0, p->g, p->D_rot, p->seed};
+        }
+    }
+    int64_t ntot = 0, keys = 0;
+    double Lmax = 0;
+    for (int s = 0; s < nsys; ++s) {
+        SysDesc& d = sd[s];
+        if (d.n <= 0 || !(d.Lx > 0) || !(d.Ly > 0) || !(d.D >= 0))
+            return fail(nullptr, SPP_ERR_ARG, "system %d: n, Lx and Ly must be positive and D_rot >= 0", s);
+        if (d.ncx <= 0) d.ncx = (int)std::min(1e9, std::floor(2 * d.Lx / p->r_list));
+        if (d.ncy <= 0) d.ncy = (int)std::min(1e9, std::floor(2 * d.Ly / p->r_list));
+        // Cell::Cell (shared/cell.h:45-49): cells no narrower than rv.  Three or more per side keep
+        // the nine stencil cells distinct.
+        if (d.ncx < 3 || d.ncy < 3 || 2 * d.Lx / d.ncx < p->r_list || 2 * d.Ly / d.ncy < p->r_list)
+            return fail(nullptr, SPP_ERR_ARG, "system %d: cell grid %dx%d invalid for r_list %g in box %gx%g",
+                        s, d.ncx, d.ncy, p->r_list, 2 * d.Lx, 2 * d.Ly);
+        if (d.ncx >= 32768 || d.ncy >= 32768) return fail(nullptr, SPP_ERR_ARG, "cell grid too large");
+        ntot += d.n;
+        keys += (int64_t)d.ncx * d.ncy;
+        Lmax = std::max(Lmax, std::max(d.Lx, d.Ly));
+    }
+    if (ntot >= (int64_t)1 << 30 || keys >= (int64_t)1 << 30)
+        return fail(nullptr, SPP_ERR_ARG, "%lld particles / %lld cells: too many for one engine",
+                    (long long)ntot, (long long)keys);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return fail(nullptr, SPP_ERR_NO_DEVICE, "no HIP device visible");
 
     spp_engine* e = new spp_engine();
     e->p = *p;
-    e->N = p->n;
+    e->N = (int)ntot;
     (void)hipGetDevice(&e->device);
-    e->G.Lx = p->Lx;
-    e->G.Ly = p->Ly;
-    e->G.Lx2 = 2 * p->Lx;
-    e->G.Ly2 = 2 * p->Ly;
-    e->G.invLx2 = 1.0 / e->G.Lx2;
-    e->G.invLy2 = 1.0 / e->G.Ly2;
-    e->G.ncx = ncx;
-    e->G.ncy = ncy;
-    e->G.quirk = p->binning == SPP_BIN_REF_QUIRK;
-    e->nkeys = ncx * ncy;
+    auto grid_of = [&](const SysDesc& d) {
+        Grid G{};
+        G.Lx = d.Lx;
+        G.Ly = d.Ly;
+        G.Lx2 = 2 * d.Lx;
+        G.Ly2 = 2 * d.Ly;
+        G.invLx2 = 1.0 / G.Lx2;
+        G.invLy2 = 1.0 / G.Ly2;
+        G.ncx = d.ncx;
+        G.ncy = d.ncy;
+        G.quirk = p->binning == SPP_BIN_REF_QUIRK;
+        return G;
+    };
+    e->G = grid_of(sd[0]);
+    e->nkeys = (int)keys;
     e->nscan_blocks = (e->nkeys + kScanTile - 1) / kScanTile;
     e->nblk = nblocks(e->N);
     {
         // strips of ~220 particles (most fit one pass of a 256-thread workgroup); their 3-row
         // neighbourhood (~3 x 220 records) then fits the LDS budget with room for clustering
+        int ncx = e->G.ncx, ncy = e->G.ncy;
         double occ = (double)e->N / ((double)ncx * ncy);
         int W = (int)std::lround(220.0 / std::max(occ, 1e-9));
         W = std::max(1, std::min(ncx, W));
@@ -1899,15 +2130,52 @@ int spp_create(const spp_params* p, spp_engine** out)
         e->SG.nstrips = e->SG.nsx * ncy;
         e->strip_grid = (e->SG.nstrips + 7) / 8 * 8;
     }
-    e->variant = p->kernel_variant;
+    e->variant = systems ? kVariantBatch : p->kernel_variant;
     e->nstat = std::max(e->strip_grid, e->nblk);
     e->cap = p->ell_capacity > 0 ? p->ell_capacity : 32;
-    e->P = make_phys(*p);
+    {
+        spp_params q = *p;
+        q.Lx = sd[0].Lx;
+        q.Ly = sd[0].Ly;
+        q.v0 = sd[0].v0;
+        q.g = sd[0].g;
+        q.D_rot = sd[0].D;
+        q.seed = sd[0].seed;
+        q.n = sd[0].n;
+        e->P = make_phys(q);
+        if (systems) {
+            e->nsys = nsys;
+            e->hsys.resize(nsys);
+            int koff = 0, poff = 0;
+            for (int s = 0; s < nsys; ++s) {
+                q.Lx = sd[s].Lx;
+                q.Ly = sd[s].Ly;
+                q.v0 = sd[s].v0;
+                q.g = sd[s].g;
+                q.D_rot = sd[s].D;
+                q.seed = sd[s].seed;
+                Phys Ps = make_phys(q);
+                SysInfo& S = e->hsys[s];
+                S.G = grid_of(sd[s]);
+                S.key_off = koff;
+                S.pid_off = poff;
+                S.n = sd[s].n;
+                S.pad = 0;
+                S.v0 = Ps.v0;
+                S.c_align = Ps.c_align;
+                S.g = Ps.g;
+                S.noise_amp = Ps.noise_amp;
+                S.seed = Ps.seed;
+                koff += sd[s].ncx * sd[s].ncy;
+                poff += sd[s].n;
+            }
+        }
+    }
     e->T_list = sqrt_threshold(p->r_list);
     e->r_cut = rcut;
     e->half_skin = 0.5 * (p->r_list - rcut);
     // float32 build positions: allow for their rounding (|x| <= L) when triggering
-    double margin = 4.0 * std::max(p->Lx, p->Ly) * 5.960464477539063e-08;
+    double margin = 4.0 * Lmax * 5.960464477539063e-08;
     e->thresh = std::max(0.0, e->half_skin - margin);
     if (p->rebuild == SPP_REBUILD_DISPLACEMENT && e->thresh <= 0) {
         delete e;
@@ -1950,11 +2218,25 @@ int spp_create(const spp_params* p, spp_engine** out)
     A(dalloc(e, &e->strips, (size_t)e->SG.nstrips));
     A(dalloc(e, &e->dpolar, (size_t)e->nblk));
     A(alloc_nbr(e));
+    if (systems) {
+        A(dalloc(e, &e->dsys, (size_t)nsys));
+        A(dalloc(e, &e->sys_of, N));
+        A(dalloc(e, &e->dpolar_sys, (size_t)nsys));
+    }
     if (rc == SPP_OK && hipHostMalloc((void**)&e->hrep, 2 * sizeof(Report), hipHostMallocMapped) != hipSuccess)
         rc = SPP_ERR_HIP;
     if (rc == SPP_OK && hipHostGetDevicePointer((void**)&e->drep, e->hrep, 0) != hipSuccess) rc = SPP_ERR_HIP;
     for (int s = 0; s < 2 && rc == SPP_OK; ++s)
         if (hipEventCreateWithFlags(&e->slot_event[s], hipEventDisableTiming) != hipSuccess) rc = SPP_ERR_HIP;
+    if (rc == SPP_OK && systems) {
+        // systems occupy fixed slot ranges: cell keys are system-major, so every sort keeps them
+        std::vector<int> so(N);
+        for (int s = 0; s < nsys; ++s)
+            for (int k = 0; k < e->hsys[s].n; ++k) so[e->hsys[s].pid_off + k] = s;
+        if (hipMemcpy(e->dsys, e->hsys.data(), nsys * sizeof(SysInfo), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(e->sys_of, so.data(), N * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+            rc = SPP_ERR_HIP;
+    }
     if (rc == SPP_OK) {
         // cell counts start at zero and every rebuild's scatter returns them to zero
         if (launch(e, "fill", k_fill<int>, nblocks(e->nkeys), e->cell_count, 0, e->nkeys) != SPP_OK)
@@ -1979,6 +2261,21 @@ int spp_create(const spp_params* p, spp_engine** out)
     return SPP_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int spp_create(const spp_params* p, spp_engine** out) { return create_engine(p, nullptr, 1, out); }
+
+int spp_create_batch(const spp_params* p, const spp_system* systems, int32_t nsys, spp_engine** out)
+{
+    if (!systems || nsys <= 0) {
+        if (out) *out = nullptr;
+        return fail(nullptr, SPP_ERR_ARG, "spp_create_batch needs at least one system");
+    }
+    return create_engine(p, systems, nsys, out);
+}
+
 void spp_destroy(spp_engine* e)
 {
     if (!e) return;
@@ -1987,7 +2284,7 @@ void spp_destroy(spp_engine* e)
     void* bufs[] = {e->pos1, e->pos2, e->th1, e->th2, e->dth, e->dth_alt, e->inj_host_order,
                     e->inj, e->kacc, e->force, e->old, e->aux_alt[0], e->aux_alt[1], e->aux_alt[2],
                     e->nbr16, e->nbr32, e->strips, e->ncount, e->cellxy, e->build, e->key, e->perm, e->cell_count,
-                    e->cell_start, e->block_sums, e->dpolar};
+                    e->cell_start, e->block_sums, e->dpolar, e->dsys, e->sys_of, e->dpolar_sys};
     for (void* b : bufs) dfree(b);
     for (auto& b : e->slotb) {
         dfree(b.blkbad);
@@ -1996,6 +2293,7 @@ void spp_destroy(spp_engine* e)
         dfree(b.blknnz);
     }
     for (auto& kv : e->slots) free_set(kv.second.s);
+    lyap_free(e->ly);
     for (auto& r : e->prof) {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
@@ -2156,6 +2454,8 @@ int spp_interact(spp_engine* e)
     StageArgs a{};
     a.N = e->N;
     a.P = e->P;
+    a.sys_tab = e->dsys;
+    a.sys_of = e->sys_of;
     a.SG = e->SG;
     a.strips = e->strips;
     a.gather = e->set[e->cur].pos;
@@ -2210,6 +2510,8 @@ int spp_move_stage(spp_engine* e, int32_t stage)
     MoveArgs m{};
     m.N = e->N;
     m.P = e->P;
+    m.sys_tab = e->dsys;
+    m.sys_of = e->sys_of;
     m.integ = integ;
     m.stage = stage;
     m.cur = e->set[e->cur];
@@ -2266,6 +2568,149 @@ int spp_polarization(spp_engine* e, double* p, double* sum_x, double* sum_y)
     if (sum_y) *sum_y = sy;
     double ax = sx / e->N, ay = sy / e->N;
     if (p) *p = std::sqrt(ax * ax + ay * ay);
+    return SPP_OK;
+}
+
+int spp_batch_polarization(spp_engine* e, double* p, int32_t nsys)
+{
+    TRY(check_ready(e));
+    if (!p) return fail(e, SPP_ERR_ARG, "null array");
+    if (!e->dsys) {
+        if (nsys != 1) return fail(e, SPP_ERR_ARG, "single-system engine: nsys must be 1");
+        return spp_polarization(e, p, nullptr, nullptr);
+    }
+    if (nsys != e->nsys) return fail(e, SPP_ERR_ARG, "nsys = %d, engine has %d systems", nsys, e->nsys);
+    TRY(launch(e, "polar_sys", k_polar_sys, nsys, (const SysInfo*)e->dsys, (const P4*)e->set[e->cur].pos,
+               e->dpolar_sys));
+    std::vector<double2> v(nsys);
+    HIPC(e, hipMemcpyAsync(v.data(), e->dpolar_sys, nsys * sizeof(double2), hipMemcpyDeviceToHost, e->stream));
+    HIPC(e, hipStreamSynchronize(e->stream));
+    for (int s = 0; s < nsys; ++s) {
+        double ax = v[s].x / e->hsys[s].n, ay = v[s].y / e->hsys[s].n;
+        p[s] = std::sqrt(ax * ax + ay * ay);
+    }
+    return SPP_OK;
+}
+
+int spp_lyapunov_init(spp_engine* e, int32_t replicas, int32_t vectors, double epsilon, int32_t rescale,
+                      uint64_t seed)
+{
+    TRY(check_ready(e));
+    if (!e->dsys) return fail(e, SPP_ERR_STATE, "Lyapunov clones need a batched engine (spp_create_batch)");
+    if (e->unfused_stage != 0) return fail(e, SPP_ERR_STATE, "a split step is in progress");
+    if (replicas <= 0 || vectors < 1 || vectors > lyap::kMaxK)
+        return fail(e, SPP_ERR_ARG, "replicas >= 1 and 1 <= vectors <= %d required", lyap::kMaxK);
+    if (!(epsilon > 0)) return fail(e, SPP_ERR_ARG, "epsilon must be positive");
+    if ((int64_t)replicas * (vectors + 1) != e->nsys)
+        return fail(e, SPP_ERR_ARG, "%d replicas x (1 + %d vectors) != %d systems", replicas, vectors, e->nsys);
+    const int n = e->hsys[0].n;
+    for (int r = 0; r < replicas; ++r) {
+        const SysInfo& B = e->hsys[r * (vectors + 1)];
+        for (int c = 0; c <= vectors; ++c) {
+            const SysInfo& S = e->hsys[r * (vectors + 1) + c];
+            bool same = S.n == n && S.G.Lx == B.G.Lx && S.G.Ly == B.G.Ly && S.G.ncx == B.G.ncx &&
+                        S.G.ncy == B.G.ncy && S.v0 == B.v0 && S.g == B.g && S.noise_amp == B.noise_amp &&
+                        S.seed == B.seed;
+            if (!same)
+                return fail(e, SPP_ERR_ARG, "system %d differs from its replica's base system %d (all systems "
+                            "need n = %d; a replica's systems need equal boxes, parameters and seeds)",
+                            r * (vectors + 1) + c, r * (vectors + 1), n);
+        }
+    }
+    lyap_free(e->ly);
+    LyapState& L = e->ly;
+    L.R = replicas;
+    L.K = vectors;
+    L.n = n;
+    L.eps = epsilon;
+    L.rescale = rescale ? 1 : 0;
+    // enough Gram workgroups to fill the chip, each with at least one pass of 256 components
+    L.nchunk = std::max(1, std::min((3 * n + kBlock - 1) / kBlock, (2048 + replicas - 1) / replicas));
+    const int NP = vectors * (vectors + 1) / 2;
+    int rc = SPP_OK;
+    auto A = [&](int r) {
+        if (rc == SPP_OK && r != SPP_OK) rc = r;
+    };
+    A(dalloc(e, &L.inv, (size_t)e->N));
+    A(dalloc(e, &L.status, (size_t)replicas));
+    A(dalloc(e, &L.dev, (size_t)replicas * vectors * 3 * n));
+    A(dalloc(e, &L.part, (size_t)replicas * L.nchunk * NP));
+    A(dalloc(e, &L.coef, (size_t)replicas * (vectors * vectors + vectors)));
+    A(dalloc(e, &L.rdiag, (size_t)replicas * vectors));
+    A(dalloc(e, &L.logsum, (size_t)replicas * vectors));
+    A(dalloc(e, &L.norm0, (size_t)replicas * vectors));
+    if (rc != SPP_OK) {
+        lyap_free(e->ly);
+        return rc;
+    }
+    L.on = true;
+    L.t_start = L.t_last = e->t;
+    TRY(launch(e, "fill", k_fill<int>, nblocks(replicas), L.status, 0, replicas));
+    TRY(launch(e, "fill", k_fill<double>, nblocks(replicas * vectors), L.logsum, 0.0, replicas * vectors));
+    TRY(launch(e, "fill", k_fill<double>, nblocks(replicas * vectors), L.norm0, epsilon, replicas * vectors));
+    lyap::Args a = lyap_args(e);
+    TRY(launch(e, "lyap_inverse", lyap::k_inverse, nblocks(e->N), e->N, (const int*)e->set[e->cur].id, L.inv));
+    TRY(launch(e, "lyap_random", lyap::k_random_dev, nblocks(replicas * n), a, seed));
+    TRY(lyap_qr(e, true));
+    e->list_valid = false;  // clones moved: the next step rebuilds
+    return lyap_check_status(e);
+}
+
+int spp_lyapunov_renormalize(spp_engine* e, double* log_growth)
+{
+    TRY(check_ready(e));
+    LyapState& L = e->ly;
+    if (!L.on) return fail(e, SPP_ERR_STATE, "call spp_lyapunov_init first");
+    if (e->unfused_stage != 0) return fail(e, SPP_ERR_STATE, "a split step is in progress");
+    lyap::Args a = lyap_args(e);
+    TRY(launch(e, "lyap_inverse", lyap::k_inverse, nblocks(e->N), e->N, (const int*)e->set[e->cur].id, L.inv));
+    TRY(launch(e, "lyap_deviation", lyap::k_deviation, nblocks(L.R * L.n), a));
+    TRY(lyap_qr(e, false));
+    e->list_valid = false;
+    L.renorms++;
+    L.t_last = e->t;
+    if (log_growth) {
+        TRY(lyap_check_status(e));
+        HIPC(e, hipMemcpy(log_growth, L.logsum, (size_t)L.R * L.K * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return SPP_OK;
+}
+
+int spp_lyapunov_run(spp_engine* e, int32_t steps, int32_t interval)
+{
+    TRY(check_ready(e));
+    if (!e->ly.on) return fail(e, SPP_ERR_STATE, "call spp_lyapunov_init first");
+    if (interval <= 0 || steps < 0 || steps % interval)
+        return fail(e, SPP_ERR_ARG, "steps (%d) must be a non-negative multiple of interval (%d)", steps, interval);
+    for (int s = 0; s < steps; s += interval) {
+        TRY(spp_multi_step(e, interval));
+        TRY(spp_lyapunov_renormalize(e, nullptr));
+    }
+    return SPP_OK;
+}
+
+int spp_lyapunov_get(spp_engine* e, double* sum_log, double* time)
+{
+    TRY(check_ready(e));
+    const LyapState& L = e->ly;
+    if (!L.on) return fail(e, SPP_ERR_STATE, "call spp_lyapunov_init first");
+    TRY(lyap_check_status(e));
+    if (sum_log)
+        HIPC(e, hipMemcpy(sum_log, L.logsum, (size_t)L.R * L.K * sizeof(double), hipMemcpyDeviceToHost));
+    if (time) *time = L.t_last - L.t_start;
+    return SPP_OK;
+}
+
+int spp_lyapunov_vectors(spp_engine* e, int32_t replica, double* out, int64_t len)
+{
+    TRY(check_ready(e));
+    const LyapState& L = e->ly;
+    if (!L.on) return fail(e, SPP_ERR_STATE, "call spp_lyapunov_init first");
+    int64_t need = (int64_t)L.K * 3 * L.n;
+    if (replica < 0 || replica >= L.R || !out || len != need)
+        return fail(e, SPP_ERR_ARG, "replica in [0, %d) and an array of %lld doubles required", L.R, (long long)need);
+    HIPC(e, hipStreamSynchronize(e->stream));
+    HIPC(e, hipMemcpy(out, L.dev + (size_t)replica * need, need * sizeof(double), hipMemcpyDeviceToHost));
     return SPP_OK;
 }
 
@@ -2370,6 +2815,8 @@ int spp_time_stage(spp_engine* e, int32_t stage, int32_t reps, double* avg_us)
     StageArgs a{};
     a.N = e->N;
     a.P = e->P;
+    a.sys_tab = e->dsys;
+    a.sys_of = e->sys_of;
     a.SG = e->SG;
     a.strips = e->strips;
     a.nbr16 = e->nbr16;
