@@ -117,21 +117,20 @@ typedef struct spp_params {
     double rc_rep;          /* repulsion cutoff radius                                              */
     double D_rot;           /* rotational diffusion: d_theta ~ N(0, sqrt(2 D_rot dt))               */
     double r_list;          /* Verlet list radius (reference rv), >= every interaction cutoff       */
-    /* SPP_MODEL_ABP only ---------------------------------------------------------------------- */
+    /* SPP_MODEL_ABP only (v0 is the swimmers' self-propulsion force F0, speed mob_par * F0; every
+       pair within rc_rep feels the repulsion law, as the reference's single-bead branch) --------- */
     double mob_par;         /* mobility along the heading (reference m_parallel, 0.5 for nb = 1)   */
     double mob_perp;        /* mobility across the heading (reference m_perpendicular)             */
     double mob_rot;         /* rotational mobility (reference k_perpendicular, 1 for nb = 1)       */
     double torque0;         /* intrinsic torque (reference torque0)                                 */
-    double k_bond;          /* ring bond stiffness                                                  */
+    double k_bond;          /* ring bond stiffness (reference membrane_elasticity)                  */
     double l_bond;          /* ring bond rest length                                                */
-    double kappa_bend;      /* ring bending stiffness (0 = none)                                    */
-    double A_mm;            /* membrane-membrane R12 amplitude                                      */
-    double sigma_mm;        /* membrane-membrane R12 sigma (reference sigma_m)                      */
-    double rc_mm;           /* membrane-membrane R12 cutoff (reference repulsion_radius_m)          */
+    double kappa_bend;      /* ring bending: E = kappa sum_i (1 - cos phi_i), phi_i the turn angle at
+                               bead i (no reference counterpart; 0 = none)                          */
     int32_t ell_capacity;   /* neighbour slots per particle on the device (0 = default 32)          */
     int32_t kernel_variant; /* SPP_KERNEL_*: how the interaction kernels gather neighbours          */
     int32_t reserved_i[6];
-    double reserved_d[8];
+    double reserved_d[11];
 } spp_params;
 
 typedef struct spp_engine spp_engine;

@@ -55,6 +55,8 @@ def lib() -> C.CDLL:
         L.orc_half_list.restype = C.c_int64
         L.orc_half_list.argtypes = [C.c_void_p, _I32, _I32, C.c_int64]
         L.orc_interact.argtypes = [C.c_void_p]
+        L.orc_ring_energy.restype = C.c_double
+        L.orc_ring_energy.argtypes = [C.c_void_p]
         L.orc_get_forces.argtypes = [C.c_void_p, _D, _D, _D, C.c_int32]
         L.orc_set_noise.argtypes = [C.c_void_p, _D, C.c_int32]
         L.orc_move_stage.argtypes = [C.c_void_p, C.c_int32]
@@ -204,6 +206,9 @@ class Oracle:
         sx, sy = C.c_double(), C.c_double()
         p = self._lib.orc_polarization(self._h, C.byref(sx), C.byref(sy))
         return p, sx.value, sy.value
+
+    def ring_energy(self):
+        return self._lib.orc_ring_energy(self._h)
 
     def step_count(self):
         return self._lib.orc_step_count(self._h)

@@ -17,7 +17,11 @@
  *   - SPP_MODEL_CONTINUOUS integrates the ContinuousParticle torque law with the RepulsiveParticle
  *     RK2/RK4/Euler updates (the reference has Euler only for that law);
  *   - cell indices are clamped to the grid (the reference indexes out of range when a wrapped
- *     coordinate rounds onto +Lx).
+ *     coordinate rounds onto +Lx);
+ *   - SPP_MODEL_ABP adds a ring bending energy kappa * sum_i (1 - cos phi_i), which the
+ *     reference's bead boxes do not have;
+ *   - ring beads move their unwrapped position with the wrapped one (the reference's
+ *     Positioning_Particles copies r_original into r, serial/beadbox.h:255-259).
  */
 #include "spp_oracle.h"
 
@@ -267,6 +271,7 @@ static void push(int32_t** a, int32_t* n, int32_t* cap, int32_t v)
 
 static double max_cutoff(const spp_params* p)
 {
+    if (p->model == SPP_MODEL_ABP) return p->repulsion != SPP_REP_NONE ? p->rc_rep : 0.0;
     double c = p->R_align;
     if (p->repulsion != SPP_REP_NONE && p->rc_rep > c) c = p->rc_rep;
     return c;
@@ -278,8 +283,14 @@ orc_sys* orc_create(const spp_params* p)
         set_err("orc_create: bad parameters");
         return NULL;
     }
-    if (p->model != SPP_MODEL_REPULSIVE && p->model != SPP_MODEL_CONTINUOUS) {
+    if (p->model != SPP_MODEL_REPULSIVE && p->model != SPP_MODEL_CONTINUOUS && p->model != SPP_MODEL_ABP) {
         set_err("orc_create: model %d not in the oracle", p->model);
+        return NULL;
+    }
+    if (p->model == SPP_MODEL_ABP &&
+        (p->integrator == SPP_RK4 || p->n_membrane < 0 || p->n_membrane > p->n ||
+         (p->n_membrane > 0 && p->n_membrane < 3))) {
+        set_err("orc_create: active Brownian model needs Euler/RK2 and n_membrane 0 or in [3, n]");
         return NULL;
     }
     orc_sys* s = (orc_sys*)calloc(1, sizeof(orc_sys));
@@ -327,9 +338,19 @@ void orc_destroy(orc_sys* s)
     free(s);
 }
 
-/* Reset: RepulsiveParticle::Reset (particle.h:559-564), f = v (times v0), torque = 0.            */
-static void reset(orc_sys* s, particle* q)
+/* Reset: RepulsiveParticle::Reset (particle.h:559-564), f = v (times v0), torque = 0;
+ * ActiveBrownianChain::Reset (:878-883), torque = torque0, f = F0 v with F0 = 0 for ring beads
+ * (membrane.cpp:144-149).                                                                         */
+static void reset(orc_sys* s, int32_t i)
 {
+    particle* q = &s->P[i];
+    if (s->p.model == SPP_MODEL_ABP) {
+        double F0 = i < s->p.n_membrane ? 0.0 : s->p.v0;
+        q->torque = s->p.torque0;
+        q->f.x = q->v.x * F0;
+        q->f.y = q->v.y * F0;
+        return;
+    }
     q->torque = 0;
     q->f.x = q->v.x * s->p.v0;
     q->f.y = q->v.y * s->p.v0;
@@ -352,7 +373,7 @@ int orc_set_state(orc_sys* s, const double* xw, const double* yw, const double* 
         q->theta = theta[i];
         q->v.x = cos(q->theta);
         q->v.y = sin(q->theta);
-        reset(s, q);
+        reset(s, i);
     }
     s->has_state = 1;
     return orc_update_cells(s);
@@ -477,6 +498,19 @@ static void pair(orc_sys* s, particle* a, particle* b)
     dr.x = orc_periodic(dr.x, s->Lx2);
     dr.y = orc_periodic(dr.y, s->Ly2);
     double d2 = dr.x * dr.x + dr.y * dr.y;
+    if (p->model == SPP_MODEL_ABP) {
+        /* ActiveBrownianChain::Interact, nb == 1 on both sides (particle.h:968-981) */
+        double d = sqrt(d2);
+        if (p->repulsion != SPP_REP_NONE && d < p->rc_rep) {
+            double fx, fy;
+            orc_force_law(p->repulsion, dr.x, dr.y, d, p->rc_rep, p->sigma_rep, p->A_rep, &fx, &fy);
+            a->f.x += fx;
+            a->f.y += fy;
+            b->f.x -= fx;
+            b->f.y -= fy;
+        }
+        return;
+    }
     if (p->model == SPP_MODEL_REPULSIVE) {
         double d = sqrt(d2);
         if (p->repulsion != SPP_REP_NONE && d < p->rc_rep) {
@@ -506,6 +540,70 @@ static void pair(orc_sys* s, particle* a, particle* b)
     }
 }
 
+/* Minimum-image bond vector r_b - r_a. */
+static vec2 bond(const orc_sys* s, int32_t a, int32_t b)
+{
+    vec2 t = {s->P[b].r.x - s->P[a].r.x, s->P[b].r.y - s->P[a].r.y};
+    t.x = orc_periodic(t.x, s->Lx2);
+    t.y = orc_periodic(t.y, s->Ly2);
+    return t;
+}
+
+/* Ring forces.  Bonds: Box::Interact_Membrane_Beads (serial/beadbox.h:286-297), Spring(dr, d, l, k)
+ * = k (l - d) dr / d with dr = r_i - r_{i+1} (force-fields.h:87-97), added to bead i and
+ * subtracted from bead i+1.  Bending: E = kappa * sum_i (1 - cos phi_i) with cos phi_i = a.b/(|a||b|),
+ * a = r_i - r_{i-1}, b = r_{i+1} - r_i; force = -grad E = kappa * grad cos phi, so angle i adds
+ * kappa * (-dc/da) to bead i-1, kappa * (dc/da - dc/db) to bead i and kappa * dc/db to bead i+1.   */
+static void ring_forces(orc_sys* s)
+{
+    const int32_t M = s->p.n_membrane;
+    const double K = s->p.k_bond, l0 = s->p.l_bond, kappa = s->p.kappa_bend;
+    for (int32_t i = 0; i < M; ++i) {
+        int32_t j = (i + 1) % M;
+        vec2 t = bond(s, i, j);
+        vec2 dr = {-t.x, -t.y};
+        double d = sqrt(dr.x * dr.x + dr.y * dr.y);
+        double fx = K * (l0 - d) * dr.x / d, fy = K * (l0 - d) * dr.y / d;
+        s->P[i].f.x += fx;
+        s->P[i].f.y += fy;
+        s->P[j].f.x -= fx;
+        s->P[j].f.y -= fy;
+    }
+    if (kappa == 0) return;
+    for (int32_t i = 0; i < M; ++i) {
+        int32_t im = (i + M - 1) % M, ip = (i + 1) % M;
+        vec2 a = bond(s, im, i), b = bond(s, i, ip);
+        double aa = a.x * a.x + a.y * a.y, bb = b.x * b.x + b.y * b.y, ab = a.x * b.x + a.y * b.y;
+        double inv = 1.0 / sqrt(aa * bb), c = ab * inv;
+        vec2 ga = {b.x * inv - c * a.x / aa, b.y * inv - c * a.y / aa};
+        vec2 gb = {a.x * inv - c * b.x / bb, a.y * inv - c * b.y / bb};
+        s->P[im].f.x -= kappa * ga.x;
+        s->P[im].f.y -= kappa * ga.y;
+        s->P[i].f.x += kappa * (ga.x - gb.x);
+        s->P[i].f.y += kappa * (ga.y - gb.y);
+        s->P[ip].f.x += kappa * gb.x;
+        s->P[ip].f.y += kappa * gb.y;
+    }
+}
+
+/* Ring energy: sum over bonds of k/2 (d - l)^2 plus the bending energy above. */
+double orc_ring_energy(orc_sys* s)
+{
+    const int32_t M = s->p.n_membrane;
+    double E = 0;
+    for (int32_t i = 0; i < M; ++i) {
+        vec2 t = bond(s, i, (i + 1) % M);
+        double d = sqrt(t.x * t.x + t.y * t.y);
+        E += 0.5 * s->p.k_bond * (d - s->p.l_bond) * (d - s->p.l_bond);
+    }
+    for (int32_t i = 0; i < M && s->p.kappa_bend != 0; ++i) {
+        vec2 a = bond(s, (i + M - 1) % M, i), b = bond(s, i, (i + 1) % M);
+        double c = (a.x * b.x + a.y * b.y) / sqrt((a.x * a.x + a.y * a.y) * (b.x * b.x + b.y * b.y));
+        E += s->p.kappa_bend * (1 - c);
+    }
+    return E;
+}
+
 /* Box::Interact with the Verlet list, serial/box.h:324-327 -> Cell::Interact, cell.h:105-110.   */
 int orc_interact(orc_sys* s)
 {
@@ -513,7 +611,8 @@ int orc_interact(orc_sys* s)
         set_err("orc_interact: no state");
         return SPP_ERR_STATE;
     }
-    for (int32_t i = 0; i < s->N; ++i) reset(s, &s->P[i]);
+    for (int32_t i = 0; i < s->N; ++i) reset(s, i);
+    if (s->p.model == SPP_MODEL_ABP && s->p.n_membrane > 0) ring_forces(s);
     int32_t X = s->ncx, Y = s->ncy;
     for (int32_t x = 0; x < X; ++x)
         for (int32_t y = 0; y < Y; ++y) {
@@ -580,6 +679,58 @@ static void set_v(particle* q)
     q->v.y = sin(q->theta);
 }
 
+/* ActiveBrownianChain::Move (particle.h:898-914, heading updated first, projection on the new
+ * heading), Move_Runge_Kutta2_1 (:917-940, projection on the old heading) and _2 (:942-962,
+ * projection on the midpoint heading).  k_perpendicular = mob_rot, m_parallel = mob_par,
+ * m_perpendicular = mob_perp.                                                                       */
+static void abp_move(orc_sys* s, int32_t stage)
+{
+    const double dt = s->p.dt, half_dt = s->half_dt;
+    const double kp = s->p.mob_rot, mpar = s->p.mob_par, mperp = s->p.mob_perp;
+    for (int32_t i = 0; i < s->N; ++i) {
+        particle* q = &s->P[i];
+        double tq = q->torque;
+        if (stage == 0) {
+            q->theta += kp * dt * (tq);
+            q->theta += q->dtheta;
+            set_v(q);
+            q->r_original.x += q->f.x * (dt * mperp);
+            q->r_original.y += q->f.y * (dt * mperp);
+            double fv = q->f.x * q->v.x + q->f.y * q->v.y;
+            q->r_original.x += q->v.x * (fv * (dt * (mpar - mperp)));
+            q->r_original.y += q->v.y * (fv * (dt * (mpar - mperp)));
+            q->r = q->r_original;
+            wrap_r(s, &q->r);
+        } else if (stage == 1) {
+            q->theta_old = q->theta;
+            q->r_old = q->r_original;
+            q->theta += kp * half_dt * (tq);
+            q->theta += q->dtheta / 2;
+            q->theta = wrap_angle(q->theta);
+            q->r.x += q->f.x * (half_dt * mperp);
+            q->r.y += q->f.y * (half_dt * mperp);
+            double fv = q->f.x * q->v.x + q->f.y * q->v.y;
+            q->r.x += q->v.x * (fv * (half_dt * (mpar - mperp)));
+            q->r.y += q->v.y * (fv * (half_dt * (mpar - mperp)));
+            set_v(q);
+            wrap_r(s, &q->r);
+        } else {
+            q->theta = q->theta_old + kp * dt * (tq);
+            q->theta += q->dtheta;
+            q->theta = wrap_angle(q->theta);
+            q->r_original.x = q->r_old.x + q->f.x * (dt * mperp);
+            q->r_original.y = q->r_old.y + q->f.y * (dt * mperp);
+            double fv = q->f.x * q->v.x + q->f.y * q->v.y;
+            q->r_original.x += q->v.x * (fv * (dt * (mpar - mperp)));
+            q->r_original.y += q->v.y * (fv * (dt * (mpar - mperp)));
+            q->r = q->r_original;
+            set_v(q);
+            wrap_r(s, &q->r);
+        }
+        reset(s, i);
+    }
+}
+
 /* Move stages.  stage 0: RepulsiveParticle::Move (particle.h:571-587); RK2 stages 1-2:
  * Move_Runge_Kutta2_1/2 (:589-631); RK4 stages 1-4: Move_Runge_Kutta4_1..4 (:634-725).           */
 int orc_move_stage(orc_sys* s, int32_t stage)
@@ -602,6 +753,15 @@ int orc_move_stage(orc_sys* s, int32_t stage)
         if (rc) return rc;
     }
     double dt = s->p.dt, half_dt = s->half_dt, dt6 = s->dt_over_6;
+    if (s->p.model == SPP_MODEL_ABP) {
+        abp_move(s, stage);
+        if (last) {
+            s->step++;
+            s->t += dt;
+            s->steps_since_build++;
+        }
+        return SPP_OK;
+    }
     for (int32_t i = 0; i < s->N; ++i) {
         particle* q = &s->P[i];
         double tq = drift_torque(s, q);
@@ -668,7 +828,7 @@ int orc_move_stage(orc_sys* s, int32_t stage)
             q->r = q->r_original;
             wrap_r(s, &q->r);
         }
-        reset(s, q);
+        reset(s, i);
     }
     if (last) {
         s->step++;

@@ -56,6 +56,7 @@ int orc_multi_step(orc_sys* s, int32_t steps);
 double orc_polarization(orc_sys* s, double* sx, double* sy);
 int64_t orc_step_count(orc_sys* s);
 int64_t orc_rebuild_count(orc_sys* s);
+double orc_ring_energy(orc_sys* s);
 
 #ifdef __cplusplus
 }

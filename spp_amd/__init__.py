@@ -66,13 +66,10 @@ class Params(C.Structure):
         ("k_bond", C.c_double),
         ("l_bond", C.c_double),
         ("kappa_bend", C.c_double),
-        ("A_mm", C.c_double),
-        ("sigma_mm", C.c_double),
-        ("rc_mm", C.c_double),
         ("ell_capacity", C.c_int32),
         ("kernel_variant", C.c_int32),
         ("reserved_i", C.c_int32 * 6),
-        ("reserved_d", C.c_double * 8),
+        ("reserved_d", C.c_double * 11),
     ]
 
     def update(self, **kw) -> "Params":

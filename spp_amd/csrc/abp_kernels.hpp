@@ -1,0 +1,236 @@
+// abp_kernels.hpp -- single-bead active Brownian particles and a closed ring membrane (gfx950).
+//
+// Included by engine.hip after the Vicsek stage kernels.  The model restates ActiveBrownianChain
+// with nb = 1 (shared/particle.h:792-1057) and the bead box's ring (parallel/beadbox.h:185-196,
+// serial/beadbox.h:286-297):
+//   * particles 0 .. n_membrane-1 (caller order) are ring beads with F0 = 0, the rest swimmers
+//     with F0 = v0 (Set_F0; the swimming speed is mob_par * F0);
+//   * Reset: torque = torque0, f = F0 * vhat (particle.h:878-883);
+//   * every pair within rc feels one repulsion law (the nb = 1 branch applies a single law to
+//     every pair, particle.h:968-981);
+//   * ring bonds i <-> i+1 (mod n_membrane): Spring(dr, d, l_bond, k_bond) = k (l - d) dr / d
+//     (force-fields.h:87-97);
+//   * ring bending (no reference counterpart): E = kappa * sum_i (1 - cos phi_i), phi_i the angle
+//     between the bonds that meet at bead i;
+//   * Move: theta += mob_rot h torque + noise; r += mob_perp h f + (mob_par - mob_perp) h (f.v) v
+//     (Move / Move_Runge_Kutta2_1/2, particle.h:898-962; stage 1 projects on the heading before
+//     the update, stage 2 on the midpoint heading, Euler on the new heading).
+#pragma once
+
+// d(cos phi)/da and d(cos phi)/db for the angle between bond vectors a (into the bead) and b
+// (out of it): cos phi = a.b / (|a||b|).
+__device__ __forceinline__ void bend_grad(double ax, double ay, double bx, double by, double& gax,
+                                          double& gay, double& gbx, double& gby)
+{
+    double aa = ax * ax + ay * ay, bb = bx * bx + by * by, ab = ax * bx + ay * by;
+    double inv = 1.0 / sqrt(aa * bb);
+    double c = ab * inv;
+    gax = bx * inv - c * ax / aa;
+    gay = by * inv - c * ay / aa;
+    gbx = ax * inv - c * bx / bb;
+    gby = ay * inv - c * by / bb;
+}
+
+// Bond and bending forces on ring bead `pid`, read from the stage input through the inverse
+// permutation (ring neighbours are caller indices pid +- 1, pid +- 2).
+__device__ __forceinline__ void ring_forces(const StageArgs& a, const Phys& P, int pid, const P4& me,
+                                            double& fx, double& fy)
+{
+    const AbpConst& A = a.abp;
+    const int Nm = A.n_membrane;
+    const P4 qm1 = a.gather[a.inv[(pid + Nm - 1) % Nm]];
+    const P4 qp1 = a.gather[a.inv[(pid + 1) % Nm]];
+    // bond vectors along the ring: t1 = r_k - r_{k-1}, t2 = r_{k+1} - r_k
+    double t1x = spp::min_image(me.x - qm1.x, P.Lx2, P.invLx2);
+    double t1y = spp::min_image(me.y - qm1.y, P.Ly2, P.invLy2);
+    double t2x = spp::min_image(qp1.x - me.x, P.Lx2, P.invLx2);
+    double t2y = spp::min_image(qp1.y - me.y, P.Ly2, P.invLy2);
+    // Spring on bond (k, k+1) with dr = r_k - r_{k+1} = -t2, and on bond (k-1, k) with
+    // dr = r_{k-1} - r_k = -t1, which acts on bead k with the opposite sign
+    double d2 = sqrt(t2x * t2x + t2y * t2y), d1 = sqrt(t1x * t1x + t1y * t1y);
+    fx += A.k_bond * (A.l_bond - d2) * (-t2x) / d2;
+    fy += A.k_bond * (A.l_bond - d2) * (-t2y) / d2;
+    fx -= A.k_bond * (A.l_bond - d1) * (-t1x) / d1;
+    fy -= A.k_bond * (A.l_bond - d1) * (-t1y) / d1;
+    if (A.kappa != 0.0) {
+        const P4 qm2 = a.gather[a.inv[(pid + Nm - 2) % Nm]];
+        const P4 qp2 = a.gather[a.inv[(pid + 2) % Nm]];
+        double t0x = spp::min_image(qm1.x - qm2.x, P.Lx2, P.invLx2);
+        double t0y = spp::min_image(qm1.y - qm2.y, P.Ly2, P.invLy2);
+        double t3x = spp::min_image(qp2.x - qp1.x, P.Lx2, P.invLx2);
+        double t3y = spp::min_image(qp2.y - qp1.y, P.Ly2, P.invLy2);
+        double ax, ay, bx, by;
+        // F_k = kappa [dc_k/da - dc_k/db - dc_{k+1}/da + dc_{k-1}/db]
+        double gx = 0, gy = 0;
+        bend_grad(t0x, t0y, t1x, t1y, ax, ay, bx, by);  // angle at k-1
+        gx += bx;
+        gy += by;
+        bend_grad(t1x, t1y, t2x, t2y, ax, ay, bx, by);  // angle at k
+        gx += ax - bx;
+        gy += ay - by;
+        bend_grad(t2x, t2y, t3x, t3y, ax, ay, bx, by);  // angle at k+1
+        gx -= ax;
+        gy -= ay;
+        fx += A.kappa * gx;
+        fy += A.kappa * gy;
+    }
+}
+
+// Forces and torque on one particle: Reset + pair repulsion + ring forces.
+template <int REP>
+__device__ __forceinline__ void abp_forces(const StageArgs& a, const Phys& P0, int i, int pid, const P4& me,
+                                           double& fx, double& fy)
+{
+    Phys P = P0;
+    P.v0 = pid < a.abp.n_membrane ? 0.0 : a.abp.F0;
+    double tq;
+    gather_forces_flat<SPP_MODEL_REPULSIVE, REP, false>(a, P, i, me, fx, fy, tq);
+    if (pid < a.abp.n_membrane) ring_forces(a, P, pid, me, fx, fy);
+}
+
+template <int REP, int INTEG, int STAGE>
+__global__ __launch_bounds__(kBlock) void k_stage_abp(StageArgs a)
+{
+    int b = xcd_block();
+    int i = b * kBlock + threadIdx.x;
+    bool live = i < a.N;
+    const Phys& P = a.P;
+    const AbpConst& A = a.abp;
+    constexpr bool final_stage = INTEG == SPP_EULER || STAGE == 2;
+    double x = 0, y = 0, dmax = 0;
+    if (live) {
+        P4 me = a.gather[i];
+        int pid = a.id[i];
+        double dn, th_in = 0, thold = 0;
+        double2 u = make_double2(0, 0);
+        if (STAGE <= 1) {
+            if (a.id_out) a.id_out[i] = pid;
+            th_in = a.th_in[i];
+            if (INTEG == SPP_EULER) u = a.U_in[i];
+            dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)pid, a.step);
+        } else {
+            thold = a.th0[i];
+            dn = a.dth[i];
+            u = a.U_in[i];
+        }
+        double fx, fy;
+        abp_forces<REP>(a, P, i, pid, me, fx, fy);
+        const double tq = A.torque0, dm = A.mob_par - A.mob_perp;
+        double th, sn, cs;
+        if (INTEG == SPP_EULER) {
+            th = th_in + A.mob_rot * P.dt * (tq);
+            th += dn;
+            sincos(th, &sn, &cs);
+            u.x += fx * (P.dt * A.mob_perp);
+            u.y += fy * (P.dt * A.mob_perp);
+            double fv = fx * cs + fy * sn;
+            u.x += cs * (fv * (P.dt * dm));
+            u.y += sn * (fv * (P.dt * dm));
+            a.U_out[i] = u;
+            x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
+            y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
+        } else if (STAGE == 1) {
+            a.dth[i] = dn;
+            th = th_in + A.mob_rot * P.half_dt * (tq);
+            th += dn / 2;
+            th = spp::wrap_angle(th);
+            double xx = me.x + fx * (P.half_dt * A.mob_perp);
+            double yy = me.y + fy * (P.half_dt * A.mob_perp);
+            double fv = fx * me.c + fy * me.s;
+            xx += me.c * (fv * (P.half_dt * dm));
+            yy += me.s * (fv * (P.half_dt * dm));
+            x = spp::periodic_wrap(xx, P.Lx, P.Lx2);
+            y = spp::periodic_wrap(yy, P.Ly, P.Ly2);
+            sincos(th, &sn, &cs);
+        } else {
+            th = thold + A.mob_rot * P.dt * (tq);
+            th += dn;
+            th = spp::wrap_angle(th);
+            u.x = u.x + fx * (P.dt * A.mob_perp);
+            u.y = u.y + fy * (P.dt * A.mob_perp);
+            double fv = fx * me.c + fy * me.s;
+            u.x += me.c * (fv * (P.dt * dm));
+            u.y += me.s * (fv * (P.dt * dm));
+            a.U_out[i] = u;
+            x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
+            y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
+            sincos(th, &sn, &cs);
+        }
+        a.out[i] = P4{x, y, cs, sn};
+        a.th_out[i] = th;
+    }
+    check_disp(a, P, b, live, i, x, y, final_stage ? a.kstep + 1 : a.kstep, dmax);
+    finish_disp(a, b, dmax);
+}
+
+template <int REP>
+__global__ __launch_bounds__(kBlock) void k_interact_abp(StageArgs a, P4* force)
+{
+    int i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= a.N) return;
+    P4 me = a.gather[i];
+    double fx, fy;
+    abp_forces<REP>(a, a.P, i, a.id[i], me, fx, fy);
+    force[i] = P4{fx, fy, a.abp.torque0, 0.0};
+}
+
+// Split-call move for the single-bead model (forces from k_interact_abp).
+__global__ __launch_bounds__(kBlock) void k_move_abp(MoveArgs a)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.N) return;
+    const Phys& P = a.P;
+    const AbpConst& A = a.abp;
+    P4 me = a.cur.pos[i];
+    double th_in = a.cur.th[i];
+    P4 f = a.force[i];
+    double2 u = a.cur.U[i];
+    const double dm = A.mob_par - A.mob_perp;
+    double th, x, y, sn, cs;
+    if (a.stage <= 1) {
+        double dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)a.cur.id[i], a.step);
+        a.dth[i] = dn;
+        if (a.integ == SPP_EULER) {
+            th = th_in + A.mob_rot * P.dt * (f.c);
+            th += dn;
+            sincos(th, &sn, &cs);
+            u.x += f.x * (P.dt * A.mob_perp);
+            u.y += f.y * (P.dt * A.mob_perp);
+            double fv = f.x * cs + f.y * sn;
+            u.x += cs * (fv * (P.dt * dm));
+            u.y += sn * (fv * (P.dt * dm));
+            a.cur.U[i] = u;
+            x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
+            y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
+        } else {
+            a.old[i] = P4{u.x, u.y, th_in, 0.0};
+            th = th_in + A.mob_rot * P.half_dt * (f.c);
+            th += dn / 2;
+            th = spp::wrap_angle(th);
+            double xx = me.x + f.x * (P.half_dt * A.mob_perp);
+            double yy = me.y + f.y * (P.half_dt * A.mob_perp);
+            double fv = f.x * me.c + f.y * me.s;
+            xx += me.c * (fv * (P.half_dt * dm));
+            yy += me.s * (fv * (P.half_dt * dm));
+            x = spp::periodic_wrap(xx, P.Lx, P.Lx2);
+            y = spp::periodic_wrap(yy, P.Ly, P.Ly2);
+            sincos(th, &sn, &cs);
+        }
+    } else {
+        P4 o = a.old[i];
+        th = o.c + A.mob_rot * P.dt * (f.c);
+        th += a.dth[i];
+        th = spp::wrap_angle(th);
+        u.x = o.x + f.x * (P.dt * A.mob_perp);
+        u.y = o.y + f.y * (P.dt * A.mob_perp);
+        double fv = f.x * me.c + f.y * me.s;
+        u.x += me.c * (fv * (P.dt * dm));
+        u.y += me.s * (fv * (P.dt * dm));
+        a.cur.U[i] = u;
+        x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
+        y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
+        sincos(th, &sn, &cs);
+    }
+    a.cur.pos[i] = P4{x, y, cs, sn};
+    a.cur.th[i] = th;
+}

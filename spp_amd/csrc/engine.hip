@@ -281,6 +281,7 @@ struct PermuteArgs {
     int N;
     const int* perm;
     SetPtrs in, out;
+    int* inv;  // caller index -> slot after the permutation
     // per-stage buffers of the split-call path (null when permuting between steps)
     const P4* a_in[3];
     P4* a_out[3];
@@ -296,7 +297,9 @@ __global__ __launch_bounds__(kBlock) void k_permute(PermuteArgs a)
     a.out.pos[k] = a.in.pos[o];
     a.out.th[k] = a.in.th[o];
     a.out.U[k] = a.in.U[o];
-    a.out.id[k] = a.in.id[o];
+    int pid = a.in.id[o];
+    a.out.id[k] = pid;
+    a.inv[pid] = k;
     if (a.d_in) {
         a.d_out[k] = a.d_in[o];
 #pragma unroll
@@ -520,9 +523,20 @@ __global__ __launch_bounds__(kStripThreads) void k_nlist(NlistArgs a)
 // Stage kernels: interact + Move fused, one particle per lane, one strip per workgroup.
 // ------------------------------------------------------------------------------------------------
 
+// Constants of the single-bead active Brownian model (zero for the Vicsek models).
+struct AbpConst {
+    int n_membrane;     // ring beads: caller indices [0, n_membrane)
+    int pad;
+    double F0;          // swimmers' self-propulsion force
+    double mob_par, mob_perp, mob_rot, torque0;
+    double k_bond, l_bond, kappa;
+};
+
 struct StageArgs {
     int N;
     Phys P;
+    AbpConst abp;
+    const int* inv;        // caller index -> slot (ring neighbours of the active Brownian model)
     const SysInfo* sys_tab;  // batched engine (null: single system, constants in P)
     const int* sys_of;
     StripGeom SG;
@@ -1002,6 +1016,7 @@ __global__ __launch_bounds__(kStripThreads) void k_interact(StageArgs a, P4* for
 struct MoveArgs {
     int N;
     Phys P;
+    AbpConst abp;
     const SysInfo* sys_tab;
     const int* sys_of;
     int integ, stage;
@@ -1081,6 +1096,8 @@ __global__ __launch_bounds__(kBlock) void k_move(MoveArgs a)
     a.cur.pos[i] = P4{x, y, cs, sn};
     a.cur.th[i] = th;
 }
+
+#include "abp_kernels.hpp"
 
 // Headings -> (cos, sin) after a host upload, computed by the same device sincos as the stages.
 __global__ __launch_bounds__(kBlock) void k_init_heading(int N, P4* __restrict__ pos,
@@ -1257,8 +1274,25 @@ StageFn stage_fn(int integ, int stage, int variant)
     }
 }
 
+template <int REP>
+StageFn abp_stage_fn(int integ, int stage)
+{
+    if (integ == SPP_EULER) return k_stage_abp<REP, SPP_EULER, 0>;
+    return stage <= 1 ? k_stage_abp<REP, SPP_RK2, 1> : k_stage_abp<REP, SPP_RK2, 2>;
+}
+
 StageFn pick_stage(int model, int rep, int integ, int stage, int variant)
 {
+    if (model == SPP_MODEL_ABP) {
+        switch (rep) {
+        case SPP_REP_NONE: return abp_stage_fn<SPP_REP_NONE>(integ, stage);
+        case SPP_REP_R12_TRUNCATED: return abp_stage_fn<SPP_REP_R12_TRUNCATED>(integ, stage);
+        case SPP_REP_HARMONIC: return abp_stage_fn<SPP_REP_HARMONIC>(integ, stage);
+        case SPP_REP_R12: return abp_stage_fn<SPP_REP_R12>(integ, stage);
+        case SPP_REP_YUKAWA_TRUNCATED: return abp_stage_fn<SPP_REP_YUKAWA_TRUNCATED>(integ, stage);
+        default: return abp_stage_fn<SPP_REP_YUKAWA>(integ, stage);
+        }
+    }
     if (model == SPP_MODEL_CONTINUOUS) return stage_fn<SPP_MODEL_CONTINUOUS, SPP_REP_NONE>(integ, stage, variant);
     switch (rep) {
     case SPP_REP_NONE: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_NONE>(integ, stage, variant);
@@ -1280,6 +1314,16 @@ InteractFn interact_fn(int variant)
 
 InteractFn pick_interact(int model, int rep, int variant)
 {
+    if (model == SPP_MODEL_ABP) {
+        switch (rep) {
+        case SPP_REP_NONE: return k_interact_abp<SPP_REP_NONE>;
+        case SPP_REP_R12_TRUNCATED: return k_interact_abp<SPP_REP_R12_TRUNCATED>;
+        case SPP_REP_HARMONIC: return k_interact_abp<SPP_REP_HARMONIC>;
+        case SPP_REP_R12: return k_interact_abp<SPP_REP_R12>;
+        case SPP_REP_YUKAWA_TRUNCATED: return k_interact_abp<SPP_REP_YUKAWA_TRUNCATED>;
+        default: return k_interact_abp<SPP_REP_YUKAWA>;
+        }
+    }
     if (model == SPP_MODEL_CONTINUOUS) return interact_fn<SPP_MODEL_CONTINUOUS, SPP_REP_NONE>(variant);
     switch (rep) {
     case SPP_REP_NONE: return interact_fn<SPP_MODEL_REPULSIVE, SPP_REP_NONE>(variant);
@@ -1396,6 +1440,8 @@ struct spp_engine {
     float2* build = nullptr;
     int *key = nullptr, *perm = nullptr, *cell_count = nullptr, *cell_start = nullptr,
         *block_sums = nullptr;
+    int* inv = nullptr;          // caller index -> slot of the current set
+    AbpConst abp{};
     SlotBufs slotb[2] = {};      // per-block partials of the (at most two) intervals in flight
     Report* hrep = nullptr;      // host-mapped reports, 2 slots
     Report* drep = nullptr;      // device view of hrep
@@ -1526,6 +1572,7 @@ void dfree(void* p)
 
 double max_cutoff(const spp_params& p)
 {
+    if (p.model == SPP_MODEL_ABP) return p.repulsion != SPP_REP_NONE ? p.rc_rep : 0.0;
     double c = p.R_align;
     if (p.model == SPP_MODEL_REPULSIVE && p.repulsion != SPP_REP_NONE && p.rc_rep > c) c = p.rc_rep;
     return c;
@@ -1601,6 +1648,7 @@ int launch_rebuild(spp_engine* e, int dst, int slot, bool aux)
     pa.perm = e->perm;
     pa.in = e->set[src];
     pa.out = e->set[dst];
+    pa.inv = e->inv;
     if (aux) {
         pa.a_in[0] = e->kacc;
         pa.a_in[1] = e->force;
@@ -1668,6 +1716,8 @@ int launch_step(spp_engine* e, int S, int W, int kstep, int slot, bool record)
     a.P = e->P;
     a.sys_tab = e->dsys;
     a.sys_of = e->sys_of;
+    a.abp = e->abp;
+    a.inv = e->inv;
     a.SG = e->SG;
     a.strips = e->strips;
     a.dth = e->dth;
@@ -1957,11 +2007,14 @@ Phys make_phys(const spp_params& p)
     if (p.model == SPP_MODEL_CONTINUOUS) {
         P.c_align = (1 - p.alpha) / spp::kPi;
         P.c_self = p.alpha / spp::kPi;
+    } else if (p.model == SPP_MODEL_ABP) {
+        P.c_align = 0;  // no alignment: the pair kernel sees T_align = 0
+        P.c_self = 0;
     } else {
         P.c_align = p.g / (spp::kPi * p.R_align * p.R_align);
         P.c_self = 0;
     }
-    P.T_align = sqrt_threshold(p.R_align);
+    P.T_align = p.model == SPP_MODEL_ABP ? 0.0 : sqrt_threshold(p.R_align);
     P.T_rep = p.repulsion != SPP_REP_NONE ? sqrt_threshold(p.rc_rep) : 0.0;
     P.T_max = std::max(P.T_align, P.T_rep);
     P.rc = p.rc_rep;
@@ -2013,11 +2066,20 @@ int spp_params_default(spp_params* p, int32_t model)
     p->mob_par = 0.5;        // ActiveBrownianChain::Set_nb(1), particle.h:861-863
     p->mob_perp = 0.5;
     p->mob_rot = 1;
-    p->sigma_mm = 0.5;       // particle.h:1052-1053
-    p->rc_mm = 0.55;
-    p->A_mm = 1;
     p->k_bond = 100;
     p->l_bond = 1;
+    if (model == SPP_MODEL_ABP) {
+        // single-bead ActiveBrownianChain: every pair uses A_p with the membrane-bead sigma_m and
+        // repulsion_radius_m (particle.h:968-981, 1048-1053); F0 = 1 (membrane.cpp:150); ring rest
+        // length repulsion_radius_m (serial/beadbox.h:292)
+        p->repulsion = SPP_REP_R12_TRUNCATED;
+        p->A_rep = 1;
+        p->sigma_rep = 0.5;
+        p->rc_rep = 0.55;
+        p->l_bond = 0.55;
+        p->v0 = 1;
+        p->r_list = 0.8;
+    }
     if (model == SPP_MODEL_CONTINUOUS) {
         p->repulsion = SPP_REP_NONE;
         p->R_align = 1;      // d2 < 1, particle.h:305
@@ -2043,10 +2105,21 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
     *out = nullptr;
     if (p->abi_version != SPP_ABI_VERSION)
         return fail(nullptr, SPP_ERR_ARG, "abi_version %d, library is %d", p->abi_version, SPP_ABI_VERSION);
-    if (p->model != SPP_MODEL_REPULSIVE && p->model != SPP_MODEL_CONTINUOUS)
+    if (p->model != SPP_MODEL_REPULSIVE && p->model != SPP_MODEL_CONTINUOUS && p->model != SPP_MODEL_ABP)
         return fail(nullptr, SPP_ERR_UNSUPPORTED, "model %d not available", p->model);
     if (p->integrator != SPP_EULER && p->integrator != SPP_RK2 && p->integrator != SPP_RK4)
         return fail(nullptr, SPP_ERR_ARG, "integrator %d", p->integrator);
+    if (p->model == SPP_MODEL_ABP) {
+        // ActiveBrownianChain has Move and Move_Runge_Kutta2_1/2 only (particle.h:898-962)
+        if (p->integrator == SPP_RK4)
+            return fail(nullptr, SPP_ERR_UNSUPPORTED, "the active Brownian model integrates with Euler or RK2");
+        if (systems || p->kernel_variant != SPP_KERNEL_SLOT)
+            return fail(nullptr, SPP_ERR_UNSUPPORTED, "the active Brownian model runs one box on the slot kernels");
+        if (p->n_membrane < 0 || p->n_membrane > p->n || (p->n_membrane > 0 && p->n_membrane < 3))
+            return fail(nullptr, SPP_ERR_ARG, "n_membrane must be 0 or in [3, n]");
+        if (p->n_membrane > 0 && !(p->l_bond > 0))
+            return fail(nullptr, SPP_ERR_ARG, "l_bond must be positive");
+    }
     if (p->repulsion < SPP_REP_NONE || p->repulsion > SPP_REP_YUKAWA)
         return fail(nullptr, SPP_ERR_ARG, "repulsion %d", p->repulsion);
     if (!(p->dt > 0) || !(p->r_list > 0)) return fail(nullptr, SPP_ERR_ARG, "dt and r_list must be positive");
@@ -2171,6 +2244,17 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
             }
         }
     }
+    if (p->model == SPP_MODEL_ABP) {
+        e->abp.n_membrane = p->n_membrane;
+        e->abp.F0 = p->v0;
+        e->abp.mob_par = p->mob_par;
+        e->abp.mob_perp = p->mob_perp;
+        e->abp.mob_rot = p->mob_rot;
+        e->abp.torque0 = p->torque0;
+        e->abp.k_bond = p->k_bond;
+        e->abp.l_bond = p->l_bond;
+        e->abp.kappa = p->kappa_bend;
+    }
     e->T_list = sqrt_threshold(p->r_list);
     e->r_cut = rcut;
     e->half_skin = 0.5 * (p->r_list - rcut);
@@ -2206,6 +2290,7 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
     A(dalloc(e, &e->build, N));
     A(dalloc(e, &e->key, N));
     A(dalloc(e, &e->perm, N));
+    A(dalloc(e, &e->inv, N));
     A(dalloc(e, &e->cell_count, (size_t)e->nkeys));
     A(dalloc(e, &e->cell_start, (size_t)e->nkeys + 1));
     A(dalloc(e, &e->block_sums, (size_t)e->nscan_blocks));
@@ -2283,7 +2368,7 @@ void spp_destroy(spp_engine* e)
     for (int k = 0; k < 3; ++k) free_set(e->set[k]);
     void* bufs[] = {e->pos1, e->pos2, e->th1, e->th2, e->dth, e->dth_alt, e->inj_host_order,
                     e->inj, e->kacc, e->force, e->old, e->aux_alt[0], e->aux_alt[1], e->aux_alt[2],
-                    e->nbr16, e->nbr32, e->strips, e->ncount, e->cellxy, e->build, e->key, e->perm, e->cell_count,
+                    e->nbr16, e->nbr32, e->strips, e->ncount, e->cellxy, e->build, e->key, e->perm, e->inv, e->cell_count,
                     e->cell_start, e->block_sums, e->dpolar, e->dsys, e->sys_of, e->dpolar_sys};
     for (void* b : bufs) dfree(b);
     for (auto& b : e->slotb) {
@@ -2451,14 +2536,18 @@ int spp_get_cells(spp_engine* e, int32_t* cell_x, int32_t* cell_y, int32_t n)
 int spp_interact(spp_engine* e)
 {
     TRY(check_ready(e));
+    if (!e->list_valid) TRY(rebuild_sync(e, true));
     StageArgs a{};
     a.N = e->N;
     a.P = e->P;
     a.sys_tab = e->dsys;
     a.sys_of = e->sys_of;
+    a.abp = e->abp;
+    a.inv = e->inv;
     a.SG = e->SG;
     a.strips = e->strips;
     a.gather = e->set[e->cur].pos;
+    a.id = e->set[e->cur].id;
     a.nbr16 = e->nbr16;
     a.nbr32 = e->nbr32;
     a.ncount = e->ncount;
@@ -2512,6 +2601,7 @@ int spp_move_stage(spp_engine* e, int32_t stage)
     m.P = e->P;
     m.sys_tab = e->dsys;
     m.sys_of = e->sys_of;
+    m.abp = e->abp;
     m.integ = integ;
     m.stage = stage;
     m.cur = e->set[e->cur];
@@ -2526,7 +2616,10 @@ int spp_move_stage(spp_engine* e, int32_t stage)
         m.inj = e->inj;
         e->inj_pending = false;
     }
-    TRY(launch(e, "move", k_move, e->nblk, m));
+    if (e->p.model == SPP_MODEL_ABP)
+        TRY(launch(e, "move", k_move_abp, e->nblk, m));
+    else
+        TRY(launch(e, "move", k_move, e->nblk, m));
     int done = integ == SPP_EULER ? 1 : stage;
     if (done == nst) {
         e->unfused_stage = 0;
@@ -2817,6 +2910,8 @@ int spp_time_stage(spp_engine* e, int32_t stage, int32_t reps, double* avg_us)
     a.P = e->P;
     a.sys_tab = e->dsys;
     a.sys_of = e->sys_of;
+    a.abp = e->abp;
+    a.inv = e->inv;
     a.SG = e->SG;
     a.strips = e->strips;
     a.nbr16 = e->nbr16;
