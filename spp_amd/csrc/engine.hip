@@ -308,113 +308,141 @@ __global__ __launch_bounds__(kBlock) void k_permute(PermuteArgs a)
 }
 
 // ------------------------------------------------------------------------------------------------
-// Strips: the unit of work of the list build and of every interaction kernel.  A strip is a run of
-// W cells of one cell row; its particles are one contiguous slot range, and every neighbour they
-// can have lies in the three rows y-1, y, y+1 over columns [x0-1, x1+1] (periodic) -- at most six
-// contiguous slot ranges ("pieces").  A workgroup copies those pieces into LDS with coalesced loads
-// once per kernel and then serves every pair gather from LDS; the Verlet list stores 16-bit LDS
-// offsets.  A strip whose neighbourhood exceeds kLdsCap records (rare dense clusters) falls back
-// to global gathers through a 32-bit slot-index list.
+// Chunks: the unit of work of the LDS kernel variant.  Chunk b is the slot range
+// [256 b, 256 b + 256) -- exactly one workgroup of particles, so the grid has no tail.  Slots are
+// sorted by cell key (row-major, W cells per row), so a chunk covers the cell keys [k0, k1] and,
+// away from the periodic edges, every neighbour of its particles lies in a cell with key in
+// [k0 - 1 + dy W, k1 + 1 + dy W] for dy = -1, 0, 1.  Those three key ranges (clipped to the grid,
+// merged when they overlap) are contiguous slot ranges ("pieces"); a workgroup copies them into LDS
+// with coalesced loads once per kernel and serves pair gathers from LDS through 16-bit offsets.
+// Neighbours in cells outside the pieces (periodic wrap in x or y) carry the marker kGlobalNbr and
+// are read from global memory through the 32-bit slot list, which is always complete.  A chunk
+// whose pieces exceed the LDS budget gathers everything from global memory.
 // ------------------------------------------------------------------------------------------------
-constexpr int kStripThreads = 256;
-constexpr int kLdsCap = 1024;  // records (32 KB): four workgroups (16 wavefronts) per CU
+constexpr int kChunk = 256;
+constexpr int kLdsCap = 1216;          // records (38 KB): four workgroups per CU
+constexpr int kPieceMax = 2 * kChunk;  // records of one piece that the unrolled staging covers
+constexpr uint16_t kGlobalNbr = 0xFFFF;
 
-// The strip's LDS copy.  Declared at file scope so that every access is compiled as an LDS
+// The chunk's LDS copy.  Declared at file scope so that every access is compiled as an LDS
 // (address space 3) access -- through a generic pointer the compiler would emit flat loads.
 __shared__ P4 s_lds[kLdsCap];
 
-struct Strip {
-    int p0, np;               // the strip's particle slots
-    int total;                // staged records; > kLdsCap means global gathers
-    int yrow[3];              // wrapped rows y-1, y, y+1
-    int lo[3], hi[3];         // main piece column range of each row
-    int main_start[3];        // slot where each row's main piece starts
-    int main_off[3];          // its LDS offset
-    int wrap_col[3];          // extra wrapped column of each row (-1 = none)
-    int wrap_off[3];          // its LDS offset
-    int pstart[6], plen[6], poff[6];  // the pieces, in LDS order
+struct Chunk {
+    int p0, np;                    // the chunk's particle slots
+    int fb;                        // 1: nothing staged, every gather from global memory
+    int npieces;
+    int kst[3], kend[3];           // cell-key range [kst, kend) of each piece
+    int pst[3], plen[3], poff[3];  // slot range and LDS offset of each piece
 };
 
-struct StripGeom {
-    int W, nsx, nstrips;      // cells per strip, strips per row, total strips
-};
-
-// One thread per strip: pieces and LDS offsets from the fresh cell starts.
-__global__ __launch_bounds__(kBlock) void k_strips(Grid G, StripGeom SG, const int* __restrict__ start,
-                                                   Strip* __restrict__ strips)
+// One thread per chunk, after the sort (perm[slot] = the particle's index in the binning order, whose
+// cell key is key[]): pieces and LDS offsets from the fresh cell starts.
+__global__ __launch_bounds__(kBlock) void k_chunks(int N, int nchunks, int W, int nkeys,
+                                                   const int* __restrict__ start, const int* __restrict__ key,
+                                                   const int* __restrict__ perm, Chunk* __restrict__ chunks)
 {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= SG.nstrips) return;
-    int y = s / SG.nsx, sx = s - y * SG.nsx;
-    int x0 = sx * SG.W, x1 = min(G.ncx, x0 + SG.W) - 1;
-    Strip S;
-    S.p0 = start[y * G.ncx + x0];
-    S.np = start[y * G.ncx + x1 + 1] - S.p0;
-    int off = 0, np = 0;
-    for (int r = 0; r < 3; ++r) {
-        int yr = y + r - 1;
-        yr = yr < 0 ? yr + G.ncy : (yr >= G.ncy ? yr - G.ncy : yr);
-        int lo = max(x0 - 1, 0), hi = min(x1 + 1, G.ncx - 1);
-        int wc = -1;
-        if (x0 == 0 && hi < G.ncx - 1) wc = G.ncx - 1;
-        if (x1 == G.ncx - 1 && lo > 0) wc = 0;
-        S.yrow[r] = yr;
-        S.lo[r] = lo;
-        S.hi[r] = hi;
-        int ms = start[yr * G.ncx + lo], me = start[yr * G.ncx + hi + 1];
-        S.main_start[r] = ms;
-        S.main_off[r] = off;
-        S.pstart[np] = ms;
-        S.plen[np] = me - ms;
-        S.poff[np] = off;
-        off += me - ms;
-        ++np;
-        S.wrap_col[r] = wc;
-        if (wc >= 0) {
-            int ws = start[yr * G.ncx + wc], we = start[yr * G.ncx + wc + 1];
-            S.wrap_off[r] = off;
-            S.pstart[np] = ws;
-            S.plen[np] = we - ws;
-            S.poff[np] = off;
-            off += we - ws;
-        } else {
-            S.wrap_off[r] = off;
-            S.pstart[np] = 0;
-            S.plen[np] = 0;
-            S.poff[np] = off;
-        }
-        ++np;
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nchunks) return;
+    const int p0 = b * kChunk, np = min(kChunk, N - p0);
+    const int k0 = key[perm[p0]], k1 = key[perm[p0 + np - 1]];
+    // the ranges for dy = -1, 0, 1 ascend, so overlapping neighbours merge into the previous piece
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = max(0, k0 - 1 + (d - 1) * W);
+        hi[d] = min(nkeys, k1 + 2 + (d - 1) * W);
     }
-    S.total = off;
-    strips[s] = S;
-}
-
-// Copy a strip's pieces of `src` into LDS (coalesced; every thread of the block participates).
-__device__ __forceinline__ void stage_strip(const Strip& S, const P4* __restrict__ src)
-{
+    Chunk C;
+    C.p0 = p0;
+    C.np = np;
+    int n = 0;
+    int kst0 = 0, kend0 = 0, kst1 = 0, kend1 = 0, kst2 = 0, kend2 = 0;
 #pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        int len = S.plen[q], st = S.pstart[q], of = S.poff[q];
-        for (int k = threadIdx.x; k < len; k += kStripThreads) s_lds[of + k] = src[st + k];
+    for (int d = 0; d < 3; ++d) {
+        if (lo[d] >= hi[d]) continue;
+        int last_end = n == 1 ? kend0 : kend1;
+        if (n > 0 && lo[d] <= last_end) {
+            if (n == 1)
+                kend0 = max(kend0, hi[d]);
+            else
+                kend1 = max(kend1, hi[d]);
+        } else {
+            if (n == 0) { kst0 = lo[d]; kend0 = hi[d]; }
+            else if (n == 1) { kst1 = lo[d]; kend1 = hi[d]; }
+            else { kst2 = lo[d]; kend2 = hi[d]; }
+            ++n;
+        }
     }
+    C.npieces = n;
+    C.kst[0] = kst0; C.kend[0] = kend0;
+    C.kst[1] = kst1; C.kend[1] = kend1;
+    C.kst[2] = kst2; C.kend[2] = kend2;
+    int off = 0, big = 0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        int ps = 0, pl = 0;
+        if (q < n) {
+            ps = start[C.kst[q]];
+            pl = start[C.kend[q]] - ps;
+        }
+        C.pst[q] = ps;
+        C.plen[q] = pl;
+        C.poff[q] = off;
+        off += pl;
+        big |= pl > kPieceMax;
+    }
+    C.fb = (off > kLdsCap || big) ? 1 : 0;
+    chunks[b] = C;
 }
 
-// LDS offset of the first particle of cell column c (wrapped) in stencil row r, and the slot it
-// corresponds to.
-__device__ __forceinline__ int column_lds(const Strip& S, int r, int c, int first_slot)
+// Copy a chunk's pieces of `src` into LDS: every load is issued before the first LDS store.  (Six
+// named records, not an array: an array indexed in a loop could be placed in scratch memory.)
+__device__ __forceinline__ void stage_chunk(const Chunk& C, const P4* __restrict__ src)
 {
-    if (c >= S.lo[r] && c <= S.hi[r]) return S.main_off[r] + (first_slot - S.main_start[r]);
-    return S.wrap_off[r];
+    const int t = threadIdx.x, u = t + kChunk;
+    const bool v0 = t < C.plen[0], w0 = u < C.plen[0];
+    const bool v1 = t < C.plen[1], w1 = u < C.plen[1];
+    const bool v2 = t < C.plen[2], w2 = u < C.plen[2];
+    // unconditional loads (slot 0 stands in for lanes past a piece): conditionally assigned
+    // records would be kept in scratch memory
+    const P4 a0 = src[v0 ? C.pst[0] + t : 0], b0 = src[w0 ? C.pst[0] + u : 0];
+    const P4 a1 = src[v1 ? C.pst[1] + t : 0], b1 = src[w1 ? C.pst[1] + u : 0];
+    const P4 a2 = src[v2 ? C.pst[2] + t : 0], b2 = src[w2 ? C.pst[2] + u : 0];
+    if (v0) s_lds[C.poff[0] + t] = a0;
+    if (w0) s_lds[C.poff[0] + u] = b0;
+    if (v1) s_lds[C.poff[1] + t] = a1;
+    if (w1) s_lds[C.poff[1] + u] = b1;
+    if (v2) s_lds[C.poff[2] + t] = a2;
+    if (w2) s_lds[C.poff[2] + u] = b2;
+}
+
+// LDS offset minus slot of the piece holding the whole cell-key range [k_lo, k_hi], or INT_MIN
+// (selects only: an indexed read of the descriptor would move it to scratch memory).
+__device__ __forceinline__ int piece_shift(const Chunk& C, int k_lo, int k_hi)
+{
+    int r = INT_MIN;
+    if (2 < C.npieces && k_lo >= C.kst[2] && k_hi < C.kend[2]) r = C.poff[2] - C.pst[2];
+    if (1 < C.npieces && k_lo >= C.kst[1] && k_hi < C.kend[1]) r = C.poff[1] - C.pst[1];
+    if (0 < C.npieces && k_lo >= C.kst[0] && k_hi < C.kend[0]) r = C.poff[0] - C.pst[0];
+    return r;
+}
+
+// Chunks are numbered along the slot order; consecutive chunks share neighbour pieces, so the
+// remap keeps runs of consecutive chunks on one XCD (one L2).
+__device__ __forceinline__ int chunk_block()
+{
+    int nblk = gridDim.x, b = blockIdx.x;
+    if ((nblk & 7) == 0) return (b & 7) * (nblk >> 3) + (b >> 3);
+    return b;
 }
 
 struct NlistArgs {
-    int N, cap;
+    int N, cap, nchunks;
     Grid G;
-    StripGeom SG;
     double T_list;  // sqrt(d2) < r_list  <=>  d2 < T_list
     const P4* pos;
     const int* start;
-    const Strip* strips;
+    const Chunk* chunks;
     uint16_t* nbr16;
     int* nbr32;
     int* ncount;
@@ -424,54 +452,60 @@ struct NlistArgs {
     int* blknnz;
 };
 
-__device__ __forceinline__ int strip_block()
-{
-    int nblk = gridDim.x, b = blockIdx.x;
-    if ((nblk & 7) == 0) return (b & 7) * (nblk >> 3) + (b >> 3);
-    return b;
-}
-
-// The candidate record of slot j (LDS offset l) for a strip in LDS (FB = false) or not.
-template <bool FB>
-__device__ __forceinline__ P4 strip_record(const P4* __restrict__ pos, int j, int l)
-{
-    if (FB) return pos[j];
-    return s_lds[l];
-}
-
 // Cell::Neighbor_List (shared/cell.h:74-103) over the 3x3 periodic stencil, full list, predicate
-// sqrt(d2) < rv (cell.h:84-85) evaluated as d2 < T(rv), candidates read from the strip's LDS copy.
-template <bool FB>
-__device__ __forceinline__ void nlist_strip(const NlistArgs& a, const Strip& S, int& mlen, int& tnnz)
+// sqrt(d2) < rv (cell.h:84-85) evaluated as d2 < T(rv); the same candidate order as k_nlist_flat
+// (per stencil row: the contiguous columns, then the wrapped column), candidates read from the
+// chunk's LDS copy where staged.
+__global__ __launch_bounds__(kChunk) void k_nlist_chunk(NlistArgs a)
 {
+    __shared__ int red_len[kChunk / 64], red_nnz[kChunk / 64];
+    int b = chunk_block();
+    if (b >= a.nchunks) return;  // grid padding (uniform)
+    const Chunk& C = a.chunks[b];
+    if (!C.fb) stage_chunk(C, a.pos);
+    __syncthreads();
     const Grid& G = a.G;
-    for (int base = 0; base < S.np; base += kStripThreads) {
-        int t = base + threadIdx.x;
-        if (t >= S.np) continue;
-        int i = S.p0 + t;
-        P4 me = strip_record<FB>(a.pos, i, S.main_off[1] + (i - S.main_start[1]));
+    int c = 0;
+    if ((int)threadIdx.x < C.np) {
+        int i = C.p0 + threadIdx.x;
+        P4 me = a.pos[i];
         int cx = cell_coord(me.x, G.Lx, G.Lx2, G.ncx, G.quirk);
         int cy = cell_coord(me.y, G.Ly, G.Ly2, G.ncy, G.quirk);
-        int c = 0;
-        for (int r = 0; r < 3; ++r) {
-            int row = S.yrow[r] * G.ncx;
-            for (int ox = -1; ox <= 1; ++ox) {
-                int x = cx + ox;
-                x = x < 0 ? x + G.ncx : (x >= G.ncx ? x - G.ncx : x);
-                int js = a.start[row + x], je = a.start[row + x + 1];
-                int lbase = FB ? 0 : column_lds(S, r, x, js);
+        int x0 = cx - 1, x1 = cx + 1, xw = -1;
+        if (x0 < 0) {
+            xw = G.ncx - 1;
+            x0 = 0;
+        } else if (x1 >= G.ncx) {
+            xw = 0;
+            x1 = G.ncx - 1;
+        }
+        for (int oy = -1; oy <= 1; ++oy) {
+            int y = cy + oy;
+            y = y < 0 ? y + G.ncy : (y >= G.ncy ? y - G.ncy : y);
+            int row = y * G.ncx;
+            for (int seg = 0; seg < 2; ++seg) {
+                int ka, kb;
+                if (seg == 0) {
+                    ka = row + x0;
+                    kb = row + x1;
+                } else {
+                    if (xw < 0) break;
+                    ka = kb = row + xw;
+                }
+                int js = a.start[ka], je = a.start[kb + 1];
+                int sh = C.fb ? INT_MIN : piece_shift(C, ka, kb);
+                bool staged = sh != INT_MIN;
+                int lbase = staged ? sh + js : 0;
                 for (int j = js; j < je; ++j) {
                     if (j == i) continue;
-                    P4 pj = strip_record<FB>(a.pos, j, lbase + (j - js));
+                    P4 pj = staged ? s_lds[lbase + (j - js)] : a.pos[j];
                     double dx = spp::min_image(me.x - pj.x, G.Lx2, G.invLx2);
                     double dy = spp::min_image(me.y - pj.y, G.Ly2, G.invLy2);
                     double d2 = dx * dx + dy * dy;
                     if (d2 < a.T_list) {
                         if (c < a.cap) {
-                            if (FB)
-                                a.nbr32[(size_t)c * a.N + i] = j;
-                            else
-                                a.nbr16[(size_t)c * a.N + i] = (uint16_t)(lbase + (j - js));
+                            a.nbr32[(size_t)c * a.N + i] = j;
+                            a.nbr16[(size_t)c * a.N + i] = staged ? (uint16_t)(lbase + (j - js)) : kGlobalNbr;
                         }
                         ++c;
                     }
@@ -481,46 +515,29 @@ __device__ __forceinline__ void nlist_strip(const NlistArgs& a, const Strip& S, 
         a.ncount[i] = c < a.cap ? c : a.cap;
         a.build[i] = make_float2((float)me.x, (float)me.y);
         a.cellxy[i] = cx | (cy << 16);
-        mlen = max(mlen, c);
-        tnnz += c;
     }
-}
-
-__global__ __launch_bounds__(kStripThreads) void k_nlist(NlistArgs a)
-{
-    __shared__ int red_len[kStripThreads / 64], red_nnz[kStripThreads / 64];
-    int s = strip_block();
-    if (s >= a.SG.nstrips) return;  // grid padding (uniform)
-    const Strip S = a.strips[s];
-    const bool fb = S.total > kLdsCap;
-    if (!fb) stage_strip(S, a.pos);
-    __syncthreads();
-    int mlen = 0, tnnz = 0;
-    if (fb)
-        nlist_strip<true>(a, S, mlen, tnnz);
-    else
-        nlist_strip<false>(a, S, mlen, tnnz);
+    int m = c;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) mlen = max(mlen, __shfl_xor(mlen, o));
-    tnnz = wave_sum(tnnz);
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
+    int t = wave_sum(c);
     if ((threadIdx.x & 63) == 0) {
-        red_len[threadIdx.x >> 6] = mlen;
-        red_nnz[threadIdx.x >> 6] = tnnz;
+        red_len[threadIdx.x >> 6] = m;
+        red_nnz[threadIdx.x >> 6] = t;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         int mm = 0, tt = 0;
-        for (int k = 0; k < kStripThreads / 64; ++k) {
+        for (int k = 0; k < kChunk / 64; ++k) {
             mm = max(mm, red_len[k]);
             tt += red_nnz[k];
         }
-        a.blklen[s] = mm;
-        a.blknnz[s] = tt;
+        a.blklen[b] = mm;
+        a.blknnz[b] = tt;
     }
 }
 
 // ------------------------------------------------------------------------------------------------
-// Stage kernels: interact + Move fused, one particle per lane, one strip per workgroup.
+// Stage kernels: interact + Move fused, one particle per lane, one chunk per workgroup.
 // ------------------------------------------------------------------------------------------------
 
 // Constants of the single-bead active Brownian model (zero for the Vicsek models).
@@ -539,8 +556,8 @@ struct StageArgs {
     const int* inv;        // caller index -> slot (ring neighbours of the active Brownian model)
     const SysInfo* sys_tab;  // batched engine (null: single system, constants in P)
     const int* sys_of;
-    StripGeom SG;
-    const Strip* strips;
+    int nchunks;
+    const Chunk* chunks;
     const P4* gather;      // stage input: neighbours' (and own) position and heading
     const double* th_in;   // own heading angle of the stage input
     P4* out;               // stage output
@@ -560,13 +577,68 @@ struct StageArgs {
     int64_t step;          // noise-stream step
     int kstep;             // step index inside the current interval
     double thresh2;        // displacement trigger, squared
-    int* blkbad;           // per-strip first bad step
-    double* blockmax;      // per-strip max displacement^2 (last step of an interval)
+    int* blkbad;           // per-workgroup first bad step
+    double* blockmax;      // per-workgroup max displacement^2 (last step of an interval)
     int record;
 };
 
-// Sum of particle i's pair terms from the strip's LDS copy (FB = false) or, for a fallback
-// strip, from global memory; four neighbours per round with their loads issued together.
+// The first kPfList rows of a staged chunk's LDS-offset list, requested before the staging loads so
+// that the pair loop does not wait on global memory for them.
+constexpr int kPfList = 12;
+
+struct ListHead {
+    int cnt;
+    int l[kPfList];
+};
+
+__device__ __forceinline__ void load_list_head(const StageArgs& a, int i, bool live, ListHead& h)
+{
+    h.cnt = live ? a.ncount[i] : 0;
+#pragma unroll
+    for (int u = 0; u < kPfList; ++u) h.l[u] = live ? a.nbr16[(size_t)u * a.N + i] : 0;
+}
+
+// Neighbour record of list entry k (LDS offset l, or global through the slot list when marked).
+__device__ __forceinline__ P4 chunk_record(const StageArgs& a, int i, int k, int l)
+{
+    if (l != kGlobalNbr) return s_lds[l];
+    return a.gather[a.nbr32[(size_t)k * a.N + i]];
+}
+
+template <int MODEL, int REP>
+__device__ __forceinline__ void gather_forces_lds(const StageArgs& a, int i, const P4& me, const ListHead& h,
+                                                  double& fx, double& fy, double& tq)
+{
+    const int cnt = h.cnt;
+    double s1 = 0, s2 = 0;
+    fx = me.c * a.P.v0;  // Reset: f = v (speed v0), particle.h:559-564
+    fy = me.s * a.P.v0;
+#pragma unroll
+    for (int k0 = 0; k0 < kPfList; k0 += 4) {
+        if (k0 >= cnt) break;
+        P4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = chunk_record(a, i, k0 + u, k0 + u < cnt ? h.l[k0 + u] : 0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (k0 + u < cnt) spp::pair_accumulate<MODEL, REP>(a.P, me, q[u], fx, fy, s1, s2);
+    }
+    for (int k = kPfList; k < cnt; k += 4) {
+        int l[4];
+        P4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) l[u] = k + u < cnt ? a.nbr16[(size_t)(k + u) * a.N + i] : 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = chunk_record(a, i, k + u, l[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (k + u < cnt) spp::pair_accumulate<MODEL, REP>(a.P, me, q[u], fx, fy, s1, s2);
+    }
+    tq = spp::drift_torque<MODEL>(a.P, s1, s2);
+}
+
+// Pair sums of slot i over its list: LDS offsets (global slot indices where marked) for a staged
+// chunk, global slot indices otherwise (FB).
 template <int MODEL, int REP, bool FB>
 __device__ __forceinline__ void gather_forces(const StageArgs& a, int i, const P4& me, double& fx,
                                               double& fy, double& tq)
@@ -578,11 +650,16 @@ __device__ __forceinline__ void gather_forces(const StageArgs& a, int i, const P
     for (int k = 0; k < cnt; k += 4) {
         P4 q[4];
         if (!FB) {
-            int j[4];
+            int l[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) j[u] = k + u < cnt ? a.nbr16[(size_t)(k + u) * a.N + i] : 0;
+            for (int u = 0; u < 4; ++u) l[u] = k + u < cnt ? a.nbr16[(size_t)(k + u) * a.N + i] : 0;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) q[u] = s_lds[j[u]];
+            for (int u = 0; u < 4; ++u) {
+                if (l[u] != kGlobalNbr)
+                    q[u] = s_lds[l[u]];
+                else
+                    q[u] = a.gather[a.nbr32[(size_t)(k + u) * a.N + i]];
+            }
         } else {
             int j[4];
 #pragma unroll
@@ -597,8 +674,8 @@ __device__ __forceinline__ void gather_forces(const StageArgs& a, int i, const P
     tq = spp::drift_torque<MODEL>(a.P, s1, s2);
 }
 
-// Displacement trigger: marks the step (per strip, atomics stay inside one workgroup's slot) and,
-// on the last step of an interval, records the strip's largest squared displacement.
+// Displacement trigger: marks the step (per workgroup, atomics stay inside one workgroup's slot)
+// and, on the last step of an interval, records the workgroup's largest squared displacement.
 __device__ __forceinline__ void check_disp(const StageArgs& a, const Phys& P, int s, bool live, int i,
                                            double x, double y, int mark, double& dmax)
 {
@@ -615,14 +692,14 @@ __device__ __forceinline__ void check_disp(const StageArgs& a, const Phys& P, in
 
 __device__ __forceinline__ void finish_disp(const StageArgs& a, int s, double dmax)
 {
-    __shared__ double red[kStripThreads / 64];
+    __shared__ double red[kChunk / 64];
     if (!a.record) return;
     double m = wave_max(dmax);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
         double r = red[0];
-        for (int k = 1; k < kStripThreads / 64; ++k) r = fmax(r, red[k]);
+        for (int k = 1; k < kChunk / 64; ++k) r = fmax(r, red[k]);
         a.blockmax[s] = r;
     }
 }
@@ -630,18 +707,18 @@ __device__ __forceinline__ void finish_disp(const StageArgs& a, int s, double dm
 // INTEG/STAGE: Euler (1,0); RK2 (2,1..2); RK4 (4,1..4).  Reference updates: Move
 // (particle.h:571-587), Move_Runge_Kutta2_1/2 (:589-631), Move_Runge_Kutta4_1..4 (:634-725).
 template <int MODEL, int REP, int INTEG, int STAGE, bool FB>
-__device__ __forceinline__ void stage_body(const StageArgs& a, int s, const Strip& S)
+__device__ __forceinline__ void stage_body(const StageArgs& a, int s, const Chunk& C, const ListHead& h)
 {
     const Phys& P = a.P;
     constexpr bool final_stage = (INTEG == SPP_EULER) || (INTEG == SPP_RK2 && STAGE == 2) || (STAGE == 4);
     double dmax = 0;
-    for (int base = 0; base < S.np; base += kStripThreads) {
-        int t = base + threadIdx.x;
-        bool live = t < S.np;
-        int i = S.p0 + t;
+    {
+        int t = threadIdx.x;
+        bool live = t < C.np;
+        int i = C.p0 + t;
         double x = 0, y = 0;
         if (live) {
-            P4 me = FB ? a.gather[i] : s_lds[S.main_off[1] + (i - S.main_start[1])];
+            P4 me = a.gather[i];
             double dn = 0, th_in = 0, thold = 0;
             double2 u = make_double2(0, 0);
             P4 kk{0, 0, 0, 0};
@@ -658,7 +735,10 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int s, const Stri
                 if (INTEG == SPP_RK4) kk = a.kacc[i];
             }
             double fx, fy, tq;
-            gather_forces<MODEL, REP, FB>(a, i, me, fx, fy, tq);
+            if (FB)
+                gather_forces<MODEL, REP, true>(a, i, me, fx, fy, tq);
+            else
+                gather_forces_lds<MODEL, REP>(a, i, me, h, fx, fy, tq);
             double th;
             if (STAGE <= 1) {
                 if (INTEG == SPP_EULER) {
@@ -709,18 +789,21 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int s, const Stri
 }
 
 template <int MODEL, int REP, int INTEG, int STAGE>
-__global__ __launch_bounds__(kStripThreads) void k_stage(StageArgs a)
+__global__ __launch_bounds__(kChunk) void k_stage(StageArgs a)
 {
-    int s = strip_block();
-    if (s >= a.SG.nstrips) return;  // grid padding (uniform)
-    const Strip S = a.strips[s];
-    const bool fb = S.total > kLdsCap;
-    if (!fb) stage_strip(S, a.gather);
+    int s = chunk_block();
+    if (s >= a.nchunks) return;  // grid padding (uniform)
+    const Chunk& C = a.chunks[s];
+    ListHead h;
+    if (!C.fb) {
+        load_list_head(a, C.p0 + (int)threadIdx.x, (int)threadIdx.x < C.np, h);
+        stage_chunk(C, a.gather);
+    }
     __syncthreads();
-    if (fb)
-        stage_body<MODEL, REP, INTEG, STAGE, true>(a, s, S);
+    if (C.fb)
+        stage_body<MODEL, REP, INTEG, STAGE, true>(a, s, C, h);
     else
-        stage_body<MODEL, REP, INTEG, STAGE, false>(a, s, S);
+        stage_body<MODEL, REP, INTEG, STAGE, false>(a, s, C, h);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -991,27 +1074,6 @@ __global__ __launch_bounds__(kBlock) void k_interact_flat(StageArgs a, P4* force
 // current state set in place like the reference's particle array.
 // ------------------------------------------------------------------------------------------------
 
-template <int MODEL, int REP>
-__global__ __launch_bounds__(kStripThreads) void k_interact(StageArgs a, P4* force)
-{
-    int s = strip_block();
-    if (s >= a.SG.nstrips) return;
-    const Strip S = a.strips[s];
-    const bool fb = S.total > kLdsCap;
-    if (!fb) stage_strip(S, a.gather);
-    __syncthreads();
-    for (int t = threadIdx.x; t < S.np; t += kStripThreads) {
-        int i = S.p0 + t;
-        double fx, fy, tq;
-        if (fb) {
-            gather_forces<MODEL, REP, true>(a, i, a.gather[i], fx, fy, tq);
-        } else {
-            P4 me = s_lds[S.main_off[1] + (i - S.main_start[1])];
-            gather_forces<MODEL, REP, false>(a, i, me, fx, fy, tq);
-        }
-        force[i] = P4{fx, fy, tq, 0.0};
-    }
-}
 
 struct MoveArgs {
     int N;
@@ -1307,7 +1369,6 @@ StageFn pick_stage(int model, int rep, int integ, int stage, int variant)
 template <int MODEL, int REP>
 InteractFn interact_fn(int variant)
 {
-    if (variant == SPP_KERNEL_STRIP_LDS) return k_interact<MODEL, REP>;
     if (variant == kVariantBatch) return k_interact_flat<MODEL, REP, true>;
     return k_interact_flat<MODEL, REP, false>;
 }
@@ -1429,14 +1490,14 @@ struct spp_engine {
     double *dth = nullptr, *dth_alt = nullptr, *inj_host_order = nullptr, *inj = nullptr;
     P4 *kacc = nullptr, *force = nullptr, *old = nullptr;
     P4* aux_alt[3] = {nullptr, nullptr, nullptr};
-    uint16_t* nbr16 = nullptr;   // list of LDS offsets (strips that fit in LDS)
-    int* nbr32 = nullptr;        // list of slot indices (fallback strips)
+    uint16_t* nbr16 = nullptr;   // list of LDS offsets (LDS variant)
+    int* nbr32 = nullptr;        // list of slot indices (always complete)
     int *ncount = nullptr, *cellxy = nullptr;
-    StripGeom SG{};
-    int strip_grid = 0;          // nstrips rounded up to a multiple of 8 (XCD remap)
+    int nchunks = 0;             // 256-slot chunks of the LDS variant
+    int chunk_grid = 0;          // nchunks rounded up to a multiple of 8 (XCD remap)
     int variant = SPP_KERNEL_SLOT;
     int nstat = 0;               // entries of the per-workgroup statistics arrays
-    Strip* strips = nullptr;
+    Chunk* chunks = nullptr;
     float2* build = nullptr;
     int *key = nullptr, *perm = nullptr, *cell_count = nullptr, *cell_start = nullptr,
         *block_sums = nullptr;
@@ -1539,9 +1600,9 @@ int launch(spp_engine* e, const char* name, F kernel, int grid, Args... args)
     return SPP_OK;
 }
 
-// Strip kernels: one workgroup of kStripThreads per strip.
+// Chunk kernels: one workgroup of kChunk threads per chunk.
 template <typename F, typename... Args>
-int launch_strips(spp_engine* e, const char* name, F kernel, Args... args)
+int launch_chunks(spp_engine* e, const char* name, F kernel, Args... args)
 {
     ProfRec rec{name, nullptr, nullptr};
     if (e->profiling) {
@@ -1549,7 +1610,7 @@ int launch_strips(spp_engine* e, const char* name, F kernel, Args... args)
         rec.b = take_event(e);
         HIPC(e, hipEventRecord(rec.a, e->stream));
     }
-    hipLaunchKernelGGL(kernel, dim3(e->strip_grid), dim3(kStripThreads), 0, e->stream, args...);
+    hipLaunchKernelGGL(kernel, dim3(e->chunk_grid), dim3(kChunk), 0, e->stream, args...);
     HIPC(e, hipGetLastError());
     if (e->profiling) {
         HIPC(e, hipEventRecord(rec.b, e->stream));
@@ -1616,10 +1677,10 @@ int launch_nlist(spp_engine* e, int slot)
                          e->ncount, e->build, e->cellxy, e->slotb[slot].blklen, e->slotb[slot].blknnz};
         return launch(e, "nlist", k_nlist_flat, e->nblk, fa);
     }
-    NlistArgs na{e->N, e->cap, e->G, e->SG, e->T_list, e->set[e->cur].pos, e->cell_start,
-                 e->strips, e->nbr16, e->nbr32, e->ncount, e->build, e->cellxy,
+    NlistArgs na{e->N, e->cap, e->nchunks, e->G, e->T_list, e->set[e->cur].pos, e->cell_start,
+                 e->chunks, e->nbr16, e->nbr32, e->ncount, e->build, e->cellxy,
                  e->slotb[slot].blklen, e->slotb[slot].blknnz};
-    return launch_strips(e, "nlist", k_nlist, na);
+    return launch_chunks(e, "nlist", k_nlist_chunk, na);
 }
 
 int launch_report(spp_engine* e, int slot, bool with_list)
@@ -1636,13 +1697,13 @@ int launch_rebuild(spp_engine* e, int dst, int slot, bool aux)
                e->block_sums));
     TRY(launch(e, "scan_final", k_scan_final, e->nscan_blocks, (const int*)e->cell_count, nk,
                (const int*)e->block_sums, e->cell_start));
-    if (e->variant == SPP_KERNEL_STRIP_LDS)
-        TRY(launch(e, "strips", k_strips, nblocks(e->SG.nstrips), e->G, e->SG, (const int*)e->cell_start,
-                   e->strips));
     TRY(launch(e, "scatter", k_scatter, nblocks(N), N, (const int*)e->key,
                (const int*)e->cell_start, e->cell_count, e->perm));
     TRY(launch(e, "cell_sort", k_cell_sort, nblocks(nk), nk, (const int*)e->cell_start, e->perm,
                (const int*)e->set[src].id));
+    if (e->variant == SPP_KERNEL_STRIP_LDS)
+        TRY(launch(e, "chunks", k_chunks, nblocks(e->nchunks), e->N, e->nchunks, e->G.ncx, e->nkeys,
+                   (const int*)e->cell_start, (const int*)e->key, (const int*)e->perm, e->chunks));
     PermuteArgs pa{};
     pa.N = N;
     pa.perm = e->perm;
@@ -1718,8 +1779,8 @@ int launch_step(spp_engine* e, int S, int W, int kstep, int slot, bool record)
     a.sys_of = e->sys_of;
     a.abp = e->abp;
     a.inv = e->inv;
-    a.SG = e->SG;
-    a.strips = e->strips;
+    a.nchunks = e->nchunks;
+    a.chunks = e->chunks;
     a.dth = e->dth;
     a.kacc = e->kacc;
     a.nbr16 = e->nbr16;
@@ -1767,7 +1828,7 @@ int launch_step(spp_engine* e, int S, int W, int kstep, int slot, bool record)
         }
         StageFn fn = pick_stage(e->p.model, e->p.repulsion, integ, stage, e->variant);
         if (e->variant == SPP_KERNEL_STRIP_LDS)
-            TRY(launch_strips(e, stage_name(integ, stage), fn, a));
+            TRY(launch_chunks(e, stage_name(integ, stage), fn, a));
         else
             TRY(launch(e, stage_name(integ, stage), fn, e->nblk, a));
         a.inj = nullptr;
@@ -2191,21 +2252,12 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
     e->nkeys = (int)keys;
     e->nscan_blocks = (e->nkeys + kScanTile - 1) / kScanTile;
     e->nblk = nblocks(e->N);
-    {
-        // strips of ~220 particles (most fit one pass of a 256-thread workgroup); their 3-row
-        // neighbourhood (~3 x 220 records) then fits the LDS budget with room for clustering
-        int ncx = e->G.ncx, ncy = e->G.ncy;
-        double occ = (double)e->N / ((double)ncx * ncy);
-        int W = (int)std::lround(220.0 / std::max(occ, 1e-9));
-        W = std::max(1, std::min(ncx, W));
-        e->SG.W = W;
-        e->SG.nsx = (ncx + W - 1) / W;
-        e->SG.nstrips = e->SG.nsx * ncy;
-        e->strip_grid = (e->SG.nstrips + 7) / 8 * 8;
-    }
+    e->nchunks = (e->N + kChunk - 1) / kChunk;
+    e->chunk_grid = (e->nchunks + 7) / 8 * 8;
     e->variant = systems ? kVariantBatch : p->kernel_variant;
-    e->nstat = std::max(e->strip_grid, e->nblk);
-    e->cap = p->ell_capacity > 0 ? p->ell_capacity : 32;
+    e->nstat = std::max(e->chunk_grid, e->nblk);
+    // the LDS stage kernels read list rows 0..kPfList-1 unconditionally
+    e->cap = std::max(kPfList, p->ell_capacity > 0 ? p->ell_capacity : 32);
     {
         spp_params q = *p;
         q.Lx = sd[0].Lx;
@@ -2300,7 +2352,7 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
         A(dalloc(e, &e->slotb[sl].blklen, (size_t)e->nstat));
         A(dalloc(e, &e->slotb[sl].blknnz, (size_t)e->nstat));
     }
-    A(dalloc(e, &e->strips, (size_t)e->SG.nstrips));
+    A(dalloc(e, &e->chunks, (size_t)e->nchunks));
     A(dalloc(e, &e->dpolar, (size_t)e->nblk));
     A(alloc_nbr(e));
     if (systems) {
@@ -2368,7 +2420,7 @@ void spp_destroy(spp_engine* e)
     for (int k = 0; k < 3; ++k) free_set(e->set[k]);
     void* bufs[] = {e->pos1, e->pos2, e->th1, e->th2, e->dth, e->dth_alt, e->inj_host_order,
                     e->inj, e->kacc, e->force, e->old, e->aux_alt[0], e->aux_alt[1], e->aux_alt[2],
-                    e->nbr16, e->nbr32, e->strips, e->ncount, e->cellxy, e->build, e->key, e->perm, e->inv, e->cell_count,
+                    e->nbr16, e->nbr32, e->chunks, e->ncount, e->cellxy, e->build, e->key, e->perm, e->inv, e->cell_count,
                     e->cell_start, e->block_sums, e->dpolar, e->dsys, e->sys_of, e->dpolar_sys};
     for (void* b : bufs) dfree(b);
     for (auto& b : e->slotb) {
@@ -2473,15 +2525,18 @@ int spp_get_neighbors(spp_engine* e, int32_t* offsets, int32_t* idx, int64_t cap
 {
     TRY(check_ready(e));
     if (!e->list_valid) TRY(spp_build_neighbors(e));
-    int N = e->N, C = e->cap, NS = e->SG.nstrips;
-    std::vector<uint16_t> n16((size_t)C * N);
+    const bool lds = e->variant == SPP_KERNEL_STRIP_LDS;
+    int N = e->N, C = e->cap, NC = e->nchunks;
+    std::vector<uint16_t> n16(lds ? (size_t)C * N : 0);
     std::vector<int> n32((size_t)C * N), cnt(N), id(N);
-    std::vector<Strip> st(NS);
-    HIPC(e, hipMemcpyAsync(n16.data(), e->nbr16, n16.size() * sizeof(uint16_t), hipMemcpyDeviceToHost, e->stream));
+    std::vector<Chunk> ch(lds ? NC : 0);
+    if (lds) {
+        HIPC(e, hipMemcpyAsync(n16.data(), e->nbr16, n16.size() * sizeof(uint16_t), hipMemcpyDeviceToHost, e->stream));
+        HIPC(e, hipMemcpyAsync(ch.data(), e->chunks, NC * sizeof(Chunk), hipMemcpyDeviceToHost, e->stream));
+    }
     HIPC(e, hipMemcpyAsync(n32.data(), e->nbr32, n32.size() * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipMemcpyAsync(cnt.data(), e->ncount, N * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipMemcpyAsync(id.data(), e->set[e->cur].id, N * sizeof(int), hipMemcpyDeviceToHost, e->stream));
-    HIPC(e, hipMemcpyAsync(st.data(), e->strips, NS * sizeof(Strip), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipStreamSynchronize(e->stream));
     std::vector<int64_t> rowlen(N + 1, 0);
     for (int k = 0; k < N; ++k) rowlen[id[k] + 1] = cnt[k];
@@ -2491,28 +2546,24 @@ int spp_get_neighbors(spp_engine* e, int32_t* offsets, int32_t* idx, int64_t cap
     if (!offsets || !idx || cap < total)
         return fail(e, SPP_ERR_CAPACITY, "neighbour list has %lld entries", (long long)total);
     for (int i = 0; i <= N; ++i) offsets[i] = (int32_t)rowlen[i];
-    if (e->variant != SPP_KERNEL_STRIP_LDS) {
-        for (int k = 0; k < N; ++k) {
-            int64_t o = rowlen[id[k]];
-            for (int c = 0; c < cnt[k]; ++c) idx[o + c] = id[n32[(size_t)c * N + k]];
-        }
-        return SPP_OK;
-    }
-    for (const Strip& S : st) {
-        bool fb = S.total > kLdsCap;
-        for (int k = S.p0; k < S.p0 + S.np; ++k) {
-            int64_t o = rowlen[id[k]];
-            for (int c = 0; c < cnt[k]; ++c) {
-                int slot;
-                if (fb) {
-                    slot = n32[(size_t)c * N + k];
-                } else {
-                    int l = n16[(size_t)c * N + k], q = 0;
-                    while (q < 5 && !(l >= S.poff[q] && l < S.poff[q] + S.plen[q])) ++q;
-                    slot = S.pstart[q] + (l - S.poff[q]);
+    for (int k = 0; k < N; ++k) {
+        int64_t o = rowlen[id[k]];
+        for (int c = 0; c < cnt[k]; ++c) {
+            int slot = n32[(size_t)c * N + k];
+            if (lds) {
+                // the LDS-offset list must name the same neighbour
+                const Chunk& Q = ch[k / kChunk];
+                int l = n16[(size_t)c * N + k];
+                if (!Q.fb && l != kGlobalNbr) {
+                    int q = 0;
+                    while (q < Q.npieces && !(l >= Q.poff[q] && l < Q.poff[q] + Q.plen[q])) ++q;
+                    int s16 = q < Q.npieces ? Q.pst[q] + (l - Q.poff[q]) : -1;
+                    if (s16 != slot)
+                        return fail(e, SPP_ERR_STATE, "LDS list entry %d of slot %d names slot %d, not %d", c, k,
+                                    s16, slot);
                 }
-                idx[o + c] = id[slot];
             }
+            idx[o + c] = id[slot];
         }
     }
     return SPP_OK;
@@ -2544,15 +2595,14 @@ int spp_interact(spp_engine* e)
     a.sys_of = e->sys_of;
     a.abp = e->abp;
     a.inv = e->inv;
-    a.SG = e->SG;
-    a.strips = e->strips;
+    a.nchunks = e->nchunks;
+    a.chunks = e->chunks;
     a.gather = e->set[e->cur].pos;
     a.id = e->set[e->cur].id;
     a.nbr16 = e->nbr16;
     a.nbr32 = e->nbr32;
     a.ncount = e->ncount;
     InteractFn fn = pick_interact(e->p.model, e->p.repulsion, e->variant);
-    if (e->variant == SPP_KERNEL_STRIP_LDS) return launch_strips(e, "interact", fn, a, e->force);
     return launch(e, "interact", fn, e->nblk, a, e->force);
 }
 
@@ -2912,8 +2962,8 @@ int spp_time_stage(spp_engine* e, int32_t stage, int32_t reps, double* avg_us)
     a.sys_of = e->sys_of;
     a.abp = e->abp;
     a.inv = e->inv;
-    a.SG = e->SG;
-    a.strips = e->strips;
+    a.nchunks = e->nchunks;
+    a.chunks = e->chunks;
     a.nbr16 = e->nbr16;
     a.nbr32 = e->nbr32;
     a.ncount = e->ncount;
@@ -2940,7 +2990,7 @@ int spp_time_stage(spp_engine* e, int32_t stage, int32_t reps, double* avg_us)
     bool prof = e->profiling;
     e->profiling = false;
     auto go = [&]() -> int {
-        if (e->variant == SPP_KERNEL_STRIP_LDS) return launch_strips(e, "time", fn, a);
+        if (e->variant == SPP_KERNEL_STRIP_LDS) return launch_chunks(e, "time", fn, a);
         return launch(e, "time", fn, e->nblk, a);
     };
     int rc = go();  // warm

@@ -326,9 +326,9 @@ def test_kernel_variants_match_oracle(oracle_mod, variant):
 
 @pytest.mark.parametrize("variant", [S.KERNEL_SLOT, S.KERNEL_SLOT_SPLIT, S.KERNEL_STRIP_LDS])
 def test_kernel_variants_dense_clusters(oracle_mod, variant):
-    """A strip whose neighbourhood overflows the LDS budget takes the global-gather fallback."""
+    """A chunk whose neighbourhood overflows the LDS budget takes the global-gather fallback."""
     n, L = 6000, 16.0
-    p = c2_params(n=n, L=L, kernel_variant=variant)
+    p = c2_params(n=n, L=L, kernel_variant=variant, dt=0.0005)  # strong overlaps: small steps
     x, y, th = oracle_mod.random_state(14, n, L, L)
     x = x * 0.3
     e, o = _pair(p, x, y, th, oracle_mod)
@@ -337,3 +337,26 @@ def test_kernel_variants_dense_clusters(oracle_mod, variant):
     gf, of = e.get_forces(), o.get_forces()
     assert vec_rel_err(gf[0], gf[1], of[0], of[1]) < FORCE_TOL
     assert rel_err(gf[2], of[2]) < FORCE_TOL
+    e.multi_step(20)
+    o.multi_step(20)
+    gs, os_ = e.get_state(), o.get_state()
+    assert np.abs(gs["xu"] - os_["xu"]).max() < TRAJ_TOL
+
+
+@pytest.mark.parametrize("n,L,squeeze,dt", [(65536, 128.0, 1.0, 0.005), (20000, 40.0, 1.0, 0.005),
+                                             (6000, 16.0, 0.3, 0.0005)])
+def test_lds_variant_bitwise_equals_slot(oracle_mod, n, L, squeeze, dt):
+    """The LDS chunk kernels read the same neighbours in the same order as the slot kernels (16-bit
+    LDS offsets, wrapped neighbours from global memory), so the trajectories are bit-for-bit equal;
+    the squeezed case has chunks over the LDS budget (global-gather fallback)."""
+    x, y, th = oracle_mod.random_state(5, n, L, L)
+    x = x * squeeze
+    out = []
+    for v in (S.KERNEL_SLOT, S.KERNEL_STRIP_LDS):
+        e = S.Engine(c2_params(n=n, L=L, kernel_variant=v, D=0.2, dt=dt))
+        e.set_state(x, y, th)
+        e.multi_step(60)
+        out.append(e.get_state())
+        off, idx = e.get_neighbors()  # decodes and cross-checks the LDS-offset list
+    for k in ("xw", "yw", "xu", "yu", "theta"):
+        assert np.array_equal(out[0][k], out[1][k]), k
