@@ -24,7 +24,7 @@ HIP_FLAGS = [
 
 
 def build_engine(verbose: bool = False) -> str:
-    srcs = [os.path.join(CSRC, "engine.hip")]
+    srcs = [os.path.join(CSRC, "engine.hip"), os.path.join(CSRC, "surrogate.hip")]
     tmp = LIB + ".tmp"
     cmd = [HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs]
     if verbose:
