@@ -224,6 +224,13 @@ int spp_polarization(spp_engine* e, double* p, double* sum_x, double* sum_y);
 /* Polarization of each system of a batched engine: p[s] = |sum vhat| / n_s.                     */
 int spp_batch_polarization(spp_engine* e, double* p, int32_t nsys);
 
+/* Isotropic pair distribution g(r) over [0, r_max) in nbins bins (r_max <= r_list), from the Verlet
+   list: counts[k] = ordered pairs (i, j) with distance in bin k; g[k] = counts[k] / (N rho pi
+   (r_{k+1}^2 - r_k^2)), rho = N / (4 Lx Ly).  Either output may be null.  Single-box engines.
+   (The reference has a body-frame 2-D pair distribution, analyze/analyze.h:365-423, and a radial
+   density about the origin, :332-362; the isotropic g(r) is new.)                              */
+int spp_pair_distribution(spp_engine* e, double r_max, int32_t nbins, int64_t* counts, double* g);
+
 /* ---- Lyapunov exponents by finite-difference clones ----------------------------------------------
    The reference evolves perturbed copies of the state with identical noise and re-orthonormalises
    their deviations by Gram-Schmidt (parallel/lyapunovbox.h:128-194, shared/vector-set.h:107-127,

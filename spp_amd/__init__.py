@@ -143,6 +143,7 @@ SIGNATURES = {
     "spp_multi_step": (C.c_int, [C.c_void_p, C.c_int32]),
     "spp_polarization": (C.c_int, [C.c_void_p, _D, _D, _D]),
     "spp_batch_polarization": (C.c_int, [C.c_void_p, _D, C.c_int32]),
+    "spp_pair_distribution": (C.c_int, [C.c_void_p, C.c_double, C.c_int32, _I64, _D]),
     "spp_lyapunov_init": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_uint64]),
     "spp_lyapunov_renormalize": (C.c_int, [C.c_void_p, _D]),
     "spp_lyapunov_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
@@ -329,6 +330,14 @@ class Engine:
         p = np.empty(len(self.sizes))
         self._check(self._lib.spp_batch_polarization(self._h, _ptr(p), len(self.sizes)))
         return p
+
+    def pair_distribution(self, r_max: float, nbins: int):
+        """(bin centres, g(r), ordered-pair counts) over [0, r_max)."""
+        counts = np.empty(nbins, dtype=np.int64)
+        g = np.empty(nbins)
+        self._check(self._lib.spp_pair_distribution(self._h, r_max, nbins, _ptr(counts, _I64), _ptr(g)))
+        dr = r_max / nbins
+        return (np.arange(nbins) + 0.5) * dr, g, counts
 
     # -- Lyapunov clones (batched engine: replicas x (base + vectors) systems) -------------------
     def lyapunov_init(self, replicas: int, vectors: int, epsilon: float, rescale: bool = True, seed: int = 1):

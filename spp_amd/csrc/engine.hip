@@ -1253,6 +1253,32 @@ __global__ __launch_bounds__(kBlock) void k_polar_sys(const SysInfo* __restrict_
     }
 }
 
+// Pair-distance histogram over the full Verlet list (each pair counted from both ends), integer
+// counts so the result does not depend on the order of the atomic additions.
+__global__ __launch_bounds__(kBlock) void k_pair_hist(int N, const P4* __restrict__ pos, const int* __restrict__ nbr,
+                                                      const int* __restrict__ ncount, Grid G, double inv_dr,
+                                                      int nbins, unsigned long long* __restrict__ hist)
+{
+    extern __shared__ unsigned int sh[];
+    for (int k = threadIdx.x; k < nbins; k += kBlock) sh[k] = 0;
+    __syncthreads();
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) {
+        P4 me = pos[i];
+        int c = ncount[i];
+        for (int k = 0; k < c; ++k) {
+            P4 q = pos[nbr[(size_t)k * N + i]];
+            double dx = spp::min_image(me.x - q.x, G.Lx2, G.invLx2);
+            double dy = spp::min_image(me.y - q.y, G.Ly2, G.invLy2);
+            int b = (int)(sqrt(dx * dx + dy * dy) * inv_dr);
+            if (b < nbins) atomicAdd(&sh[b], 1u);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < nbins; k += kBlock)
+        if (sh[k]) atomicAdd(&hist[k], (unsigned long long)sh[k]);
+}
+
 // Reduces one interval's per-block partials into the host-mapped report (one block).
 __global__ __launch_bounds__(kBlock) void k_report(int nblk, SlotBufs b, int with_list, Report* out)
 {
@@ -2731,6 +2757,40 @@ int spp_batch_polarization(spp_engine* e, double* p, int32_t nsys)
     for (int s = 0; s < nsys; ++s) {
         double ax = v[s].x / e->hsys[s].n, ay = v[s].y / e->hsys[s].n;
         p[s] = std::sqrt(ax * ax + ay * ay);
+    }
+    return SPP_OK;
+}
+
+int spp_pair_distribution(spp_engine* e, double r_max, int32_t nbins, int64_t* counts, double* g)
+{
+    TRY(check_ready(e));
+    if (e->dsys) return fail(e, SPP_ERR_UNSUPPORTED, "pair distribution of a batched engine");
+    if (nbins <= 0 || nbins > 8192 || !(r_max > 0)) return fail(e, SPP_ERR_ARG, "need 0 < nbins <= 8192, r_max > 0");
+    if (r_max > e->p.r_list) return fail(e, SPP_ERR_ARG, "r_max %g beyond the list radius %g", r_max, e->p.r_list);
+    if (!e->list_valid) TRY(rebuild_sync(e, false));
+    unsigned long long* dh = nullptr;
+    TRY(dalloc(e, &dh, (size_t)nbins));
+    int rc = launch(e, "fill", k_fill<unsigned long long>, nblocks(nbins), dh, 0ull, nbins);
+    const double dr = r_max / nbins;
+    if (rc == SPP_OK) {
+        hipLaunchKernelGGL(k_pair_hist, dim3(e->nblk), dim3(kBlock), nbins * sizeof(unsigned int), e->stream, e->N,
+                           (const P4*)e->set[e->cur].pos, (const int*)e->nbr32, (const int*)e->ncount, e->G,
+                           1.0 / dr, nbins, dh);
+        if (hipGetLastError() != hipSuccess) rc = fail(e, SPP_ERR_HIP, "pair histogram launch failed");
+    }
+    std::vector<unsigned long long> h(nbins);
+    if (rc == SPP_OK && (hipMemcpyAsync(h.data(), dh, nbins * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                        e->stream) != hipSuccess ||
+                         hipStreamSynchronize(e->stream) != hipSuccess))
+        rc = fail(e, SPP_ERR_HIP, "pair histogram read-back failed");
+    dfree(dh);
+    if (rc != SPP_OK) return rc;
+    // g(r_k) = C_k / (N rho pi (r_{k+1}^2 - r_k^2)), C_k counting each pair from both ends
+    const double rho = e->N / (4.0 * e->p.Lx * e->p.Ly);
+    for (int k = 0; k < nbins; ++k) {
+        if (counts) counts[k] = (int64_t)h[k];
+        const double a = spp::kPi * ((k + 1) * dr * (k + 1) * dr - k * dr * k * dr);
+        if (g) g[k] = (double)h[k] / (e->N * rho * a);
     }
     return SPP_OK;
 }

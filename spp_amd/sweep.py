@@ -1,0 +1,349 @@
+"""Parameter sweeps over the order-disorder transition of the continuous-time Vicsek model.
+
+Three ways to produce a polarisation-versus-noise curve P(D) at fixed density, speed and size,
+every point resolved to a given error bar (the reference's autocorrelation estimator,
+polarization-analysis.py:29-62, restated in observables.polarity_error):
+
+* ``plain_curve``   -- the reference's way: one simulation per point on a uniform grid of D, run
+                       until its error bar is small enough (serial/mpi-main.cpp:157-160 runs such
+                       points as independent processes);
+* ``batched_curve`` -- the same points as systems of one batched engine (one kernel launch advances
+                       every point);
+* ``steered_curve`` -- the batched engine on points placed by the surrogate network: the network,
+                       trained on simulator output (``make_dataset``), predicts the curve, and the
+                       points are spread evenly along its predicted arc length, so that they crowd
+                       where P changes fast.  Every reported value is a simulated one; where
+                       neighbouring simulated values still differ by more than the resolution, the
+                       sweep adds simulated midpoints (the plain fallback).
+
+The curve counts as resolved when every point's error bar is at most ``tol`` and neighbouring
+points differ by at most ``gap`` (default 3 tol: two values each within tol of the truth can differ
+by 2 tol through noise alone) or are already ``dD_min`` apart (the finest noise resolution asked
+for: in a finite system the transition can be a jump, which no spacing resolves).  Runs start from
+aligned headings (see ``initial_state``).
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import (MODEL_CONTINUOUS, REBUILD_DISPLACEMENT, RK2, Engine, default_params, random_state, system)
+from .observables import polarity_error
+
+FEATURES = ("D", "rho", "v0", "logN")
+TARGETS = ("mean_P", "var_P", "lambda1")
+
+
+def model_params(dt=0.01):
+    """Continuous-time Vicsek: unit interaction radius, g = 1, RK2, Verlet radius 1.3."""
+    p = default_params(MODEL_CONTINUOUS)
+    return p.update(dt=dt, R_align=1.0, g=1.0, alpha=0.0, integrator=RK2, r_list=1.3,
+                    rebuild=REBUILD_DISPLACEMENT)
+
+
+def half_width(n, rho):
+    return 0.5 * math.sqrt(n / rho)
+
+
+def features(D, rho, v0, n):
+    """Network inputs, scaled to order one."""
+    return np.stack([4.0 * np.asarray(D) - 1.0, np.log(np.asarray(rho)), 2.0 * np.asarray(v0) - 1.0,
+                     np.log(np.asarray(n, dtype=np.float64)) - 7.0], axis=-1).astype(np.float32)
+
+
+@dataclass
+class Timer:
+    seconds: float = 0.0
+    particle_steps: float = 0.0
+
+    def run(self, fn, *a, particles=0, steps=0):
+        t0 = time.perf_counter()
+        r = fn(*a)
+        self.seconds += time.perf_counter() - t0
+        self.particle_steps += particles * steps
+        return r
+
+
+# -------------------------------------------------------------------------------------------------
+# Simulation of many points in one batched engine
+# -------------------------------------------------------------------------------------------------
+
+def initial_state(seed, n, L, ordered):
+    """Uniform positions; headings uniform, or all equal (`ordered`: the ordered phase is then
+    reached at once and the disordered one within a few rotational-diffusion times)."""
+    x, y, t = random_state(seed, int(n), L, L)
+    if ordered:
+        t = np.zeros_like(t)
+    return x, y, t
+
+
+def run_points(points, steps, sample_every, seed, timer=None, dt=0.01, state=None, ordered=True):
+    """Simulate points [(D, rho, v0, n)] as one batched engine for `steps` steps, sampling every
+    system's polarisation every `sample_every` steps.  Returns (series [samples, points], final
+    state dict per point) ; `state` continues from a previous call's final states."""
+    timer = timer or Timer()
+    p = model_params(dt)
+    systems = [system(int(n), half_width(n, rho), v0=v0, g=1.0, D_rot=D, seed=seed + 7919 * k)
+               for k, (D, rho, v0, n) in enumerate(points)]
+    e = Engine(p, systems)
+    xs, ys, ts = [], [], []
+    for k, (D, rho, v0, n) in enumerate(points):
+        if state is not None:
+            s = state[k]
+            xs.append(s["xw"])
+            ys.append(s["yw"])
+            ts.append(s["theta"])
+        else:
+            L = half_width(n, rho)
+            x, y, t = initial_state(seed * 1000003 + k, n, L, ordered)
+            xs.append(x)
+            ys.append(y)
+            ts.append(t)
+    e.set_state(np.concatenate(xs), np.concatenate(ys), np.concatenate(ts))
+    ntot = int(sum(n for *_, n in points))
+    series = []
+    for _ in range(steps // sample_every):
+        timer.run(e.multi_step, sample_every, particles=ntot, steps=sample_every)
+        series.append(timer.run(e.batch_polarization))
+    st = e.get_state()
+    off = e.offsets
+    final = [{k: v[off[i]:off[i + 1]] for k, v in st.items()} for i in range(len(points))]
+    e.close()
+    return np.array(series), final
+
+
+def run_lyapunov(points, warm, steps, seed, sample_every=10, interval=10, eps=1e-6, dt=0.01, timer=None,
+                 ordered=True):
+    """Polarisation series and largest Lyapunov exponent of each point (one base + one clone per
+    point, all points of one size n in one engine): `warm` steps of the pairs, then `steps` steps
+    with renormalisation every `interval` steps and a polarisation sample of every base every
+    `sample_every` steps (a multiple of `interval`).  Returns (series [samples, points], lambda)."""
+    timer = timer or Timer()
+    ns = {int(n) for *_, n in points}
+    if len(ns) != 1:
+        raise ValueError("one system size per Lyapunov batch")
+    if sample_every % interval:
+        raise ValueError("sample_every must be a multiple of interval")
+    n = ns.pop()
+    p = model_params(dt)
+    systems = []
+    xs, ys, ts = [], [], []
+    for k, (D, rho, v0, _) in enumerate(points):
+        L = half_width(n, rho)
+        x, y, t = initial_state(seed * 1000003 + k, n, L, ordered)
+        for _ in range(2):
+            systems.append(system(n, L, v0=v0, g=1.0, D_rot=D, seed=seed + 7919 * k))
+            xs.append(x)
+            ys.append(y)
+            ts.append(t)
+    e = Engine(p, systems)
+    e.set_state(np.concatenate(xs), np.concatenate(ys), np.concatenate(ts))
+    ntot = 2 * n * len(points)
+    timer.run(e.multi_step, warm, particles=ntot, steps=warm)
+    e.lyapunov_init(len(points), 1, eps, True, seed)
+    series = []
+    for _ in range(steps // sample_every):
+        timer.run(e.lyapunov_run, sample_every, interval, particles=ntot, steps=sample_every)
+        series.append(timer.run(e.batch_polarization)[0::2])
+    sums, T = e.lyapunov_get()
+    e.close()
+    return np.array(series), sums[:, 0] / T
+
+
+# -------------------------------------------------------------------------------------------------
+# Training data and the surrogate
+# -------------------------------------------------------------------------------------------------
+
+@dataclass
+class Dataset:
+    x: np.ndarray                    # [n, 4] raw parameters (D, rho, v0, n)
+    y: np.ndarray                    # [n, 3] mean P, var P, lambda1
+    seconds: float = 0.0
+
+    def inputs(self):
+        return features(self.x[:, 0], self.x[:, 1], self.x[:, 2], self.x[:, 3])
+
+
+def make_dataset(count, sizes=(256, 512, 1024), warm=6000, steps=6000, sample_every=20, seed=1,
+                 D_range=(0.0, 1.5), rho_range=(0.25, 4.0), v0_range=(0.25, 1.0), per_batch=64):
+    """Seeded parameter draws simulated in batches of `per_batch` points of one size each: mean and
+    variance of the polarisation over the `steps` after `warm` and the largest exponent."""
+    rng = np.random.default_rng(seed)
+    timer = Timer()
+    xs, ys = [], []
+    k = 0
+    while k < count:
+        n = int(sizes[(k // per_batch) % len(sizes)])
+        m = min(per_batch, count - k)
+        pts = [(rng.uniform(*D_range), rng.uniform(*rho_range), rng.uniform(*v0_range), n) for _ in range(m)]
+        series, lam = run_lyapunov(pts, warm, steps, seed + 31 * k, sample_every=sample_every, timer=timer)
+        for j, pt in enumerate(pts):
+            xs.append(pt)
+            ys.append((series[:, j].mean(), series[:, j].var(), lam[j]))
+        k += m
+    return Dataset(np.array(xs, dtype=np.float64), np.array(ys, dtype=np.float64), timer.seconds)
+
+
+@dataclass
+class Surrogate:
+    """The network plus the target scaling it was trained with."""
+    net: object
+    mu: np.ndarray
+    sd: np.ndarray
+    train_ms: float = 0.0
+
+    def predict(self, D, rho, v0, n):
+        y = self.net.forward(features(D, rho, v0, n)).astype(np.float64)
+        return y * self.sd + self.mu
+
+
+def train_surrogate(data: Dataset, epochs=400, batch=256, seed=1):
+    from .surrogate import Mlp
+
+    y = data.y.copy()
+    y[:, 1] = np.log(np.maximum(y[:, 1], 1e-8))  # variance spans decades: fit its logarithm
+    mu, sd = y.mean(axis=0), y.std(axis=0) + 1e-12
+    net = Mlp(seed=seed, max_batch=max(batch, 4096))
+    net.train(data.inputs(), ((y - mu) / sd).astype(np.float32), epochs=epochs, batch=batch, seed=seed)
+    s = Surrogate(net, mu, sd, net.last_train_ms())
+    return s
+
+
+def predicted_curve(s: Surrogate, D, rho, v0, n):
+    y = s.predict(D, np.full_like(D, rho), np.full_like(D, v0), np.full_like(D, n))
+    return y[:, 0], np.exp(y[:, 1]), y[:, 2]
+
+
+# -------------------------------------------------------------------------------------------------
+# Curves
+# -------------------------------------------------------------------------------------------------
+
+@dataclass
+class Curve:
+    D: np.ndarray
+    P: np.ndarray
+    err: np.ndarray
+    seconds: float
+    particle_steps: float
+    points_simulated: int
+    rounds: int = 1
+    log: list = field(default_factory=list)
+
+    def resolved(self, tol, gap, dD_min):
+        o = np.argsort(self.D)
+        ok_gap = (np.abs(np.diff(self.P[o])) <= gap) | (np.diff(self.D[o]) <= 2 * dD_min)
+        return bool(np.all(self.err <= tol) and np.all(ok_gap))
+
+
+def _converge(points, tol, chunk, sample_every, max_steps, seed, timer, min_samples=40):
+    """Run `points` in one batch until every error bar is <= tol (systems that are done leave the
+    batch at the next chunk boundary).  Returns (mean, error) per point."""
+    n = len(points)
+    series = [np.zeros(0) for _ in range(n)]
+    state = None
+    active = list(range(n))
+    done_mean = np.zeros(n)
+    done_err = np.full(n, np.inf)
+    steps = 0
+    seg = 0
+    while active and steps < max_steps:
+        pts = [points[i] for i in active]
+        # a new segment re-keys the noise so it does not repeat the previous segment's stream
+        s, state = run_points(pts, chunk, sample_every, seed + 104729 * seg, timer, state=state)
+        seg += 1
+        steps += chunk
+        keep = []
+        keep_state = []
+        for j, i in enumerate(active):
+            series[i] = np.concatenate([series[i], s[:, j]])
+            if series[i].size >= min_samples:
+                m, err, _ = polarity_error(np.arange(series[i].size), series[i])
+                done_mean[i], done_err[i] = m, err
+                if err <= tol:
+                    continue
+            keep.append(i)
+            keep_state.append(state[j])
+        active = keep
+        state = keep_state
+    for i in active:  # out of steps: report what was measured
+        m, err, _ = polarity_error(np.arange(series[i].size), series[i])
+        done_mean[i], done_err[i] = m, err
+    return done_mean, done_err
+
+
+def plain_curve(D_grid, rho, v0, n, tol=0.02, chunk=2000, sample_every=10, max_steps=200000, seed=1):
+    """One engine per point, sequentially (the reference's independent runs)."""
+    timer = Timer()
+    P, E = [], []
+    for k, D in enumerate(D_grid):
+        m, e = _converge([(D, rho, v0, n)], tol, chunk, sample_every, max_steps, seed + 11 * k, timer)
+        P.append(m[0])
+        E.append(e[0])
+    return Curve(np.array(D_grid), np.array(P), np.array(E), timer.seconds, timer.particle_steps, len(D_grid))
+
+
+def _refine(D, P, E, points_fn, tol, gap, dD_min, chunk, sample_every, max_steps, seed, timer, rounds, log,
+            max_rounds):
+    """Add simulated midpoints between neighbours that differ by more than `gap` and are more than
+    2 dD_min apart."""
+    while rounds < max_rounds:
+        o = np.argsort(D)
+        D, P, E = D[o], P[o], E[o]
+        bad = np.nonzero((np.abs(np.diff(P)) > gap) & (np.diff(D) > 2 * dD_min))[0]
+        if bad.size == 0:
+            break
+        mids = 0.5 * (D[bad] + D[bad + 1])
+        m, e = _converge(points_fn(mids), tol, chunk, sample_every, max_steps, seed + 1009 * rounds, timer)
+        log.append(("refine", len(mids)))
+        D, P, E = np.concatenate([D, mids]), np.concatenate([P, m]), np.concatenate([E, e])
+        rounds += 1
+    o = np.argsort(D)
+    return D[o], P[o], E[o], rounds
+
+
+def batched_curve(D_grid, rho, v0, n, tol=0.02, gap=None, dD_min=None, chunk=2000, sample_every=10,
+                  max_steps=200000, seed=1, max_rounds=6):
+    """All points of a uniform grid in one batched engine, refined where neighbours differ by more
+    than `gap` (no network)."""
+    gap = 3 * tol if gap is None else gap
+    D = np.asarray(D_grid, dtype=np.float64)
+    dD_min = (D.max() - D.min()) / 64 if dD_min is None else dD_min
+    timer = Timer()
+    pts = lambda Ds: [(float(D), rho, v0, n) for D in Ds]
+    m, e = _converge(pts(D), tol, chunk, sample_every, max_steps, seed, timer)
+    log = [("grid", len(D))]
+    D, P, E, rounds = _refine(D, m, e, pts, tol, gap, dD_min, chunk, sample_every, max_steps, seed, timer, 1, log,
+                              max_rounds)
+    return Curve(D, P, E, timer.seconds, timer.particle_steps, len(D), rounds, log)
+
+
+def steered_points(s: Surrogate, D_lo, D_hi, rho, v0, n, count, tol, fine=512):
+    """`count` noise values spread evenly along the predicted curve's arc length in the metric
+    (dD / (D_hi - D_lo), dP / tol): points crowd where the predicted P changes fast."""
+    Dg = np.linspace(D_lo, D_hi, fine)
+    Pm, _, _ = predicted_curve(s, Dg, rho, v0, n)
+    ds = np.hypot(np.diff(Dg) / (D_hi - D_lo) * count, np.diff(Pm) / tol)
+    arc = np.concatenate([[0.0], np.cumsum(ds)])
+    return np.interp(np.linspace(0, arc[-1], count), arc, Dg)
+
+
+def steered_curve(s: Surrogate, D_lo, D_hi, rho, v0, n, count=None, tol=0.02, gap=None, dD_min=None, chunk=2000,
+                  sample_every=10, max_steps=200000, seed=1, max_rounds=6):
+    """Surrogate-placed points in one batched engine, then simulated refinement where needed."""
+    gap = 3 * tol if gap is None else gap
+    dD_min = (D_hi - D_lo) / 64 if dD_min is None else dD_min
+    timer = Timer()
+    Dg = np.linspace(D_lo, D_hi, 512)
+    Pm, _, _ = predicted_curve(s, Dg, rho, v0, n)
+    if count is None:  # enough points for the predicted total variation plus the end points
+        count = int(np.ceil(np.abs(np.diff(Pm)).sum() / gap)) + 3
+    D = steered_points(s, D_lo, D_hi, rho, v0, n, count, gap)
+    pts = lambda Ds: [(float(x), rho, v0, n) for x in Ds]
+    m, e = _converge(pts(D), tol, chunk, sample_every, max_steps, seed, timer)
+    log = [("steered", len(D))]
+    D, P, E, rounds = _refine(D, m, e, pts, tol, gap, dD_min, chunk, sample_every, max_steps, seed, timer, 1, log,
+                              max_rounds)
+    return Curve(D, P, E, timer.seconds, timer.particle_steps, len(D), rounds, log)
