@@ -221,6 +221,12 @@ int spp_multi_step(spp_engine* e, int32_t steps);
 
 /* Box::Compute_Polarization (serial/box.h:531-538): P = |sum vhat| / N, plus the summed vector.   */
 int spp_polarization(spp_engine* e, double* p, double* sum_x, double* sum_y);
+/* spp_multi_step plus a polarisation sample of every system after each `every` steps, recorded on
+   the device (no extra host round trip): polar[k * nsys + s] after step k * every of this call
+   (Box::Save_Polarization, parallel/box.h:467-473, called in the data_gathering loop).  nsys = 1
+   for a single-box engine; steps must be a multiple of every; len = (steps / every) * nsys.      */
+int spp_multi_step_sampled(spp_engine* e, int32_t steps, int32_t every, double* polar, int64_t len);
+
 /* Polarization of each system of a batched engine: p[s] = |sum vhat| / n_s.                     */
 int spp_batch_polarization(spp_engine* e, double* p, int32_t nsys);
 

@@ -143,6 +143,7 @@ SIGNATURES = {
     "spp_multi_step": (C.c_int, [C.c_void_p, C.c_int32]),
     "spp_polarization": (C.c_int, [C.c_void_p, _D, _D, _D]),
     "spp_batch_polarization": (C.c_int, [C.c_void_p, _D, C.c_int32]),
+    "spp_multi_step_sampled": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _D, C.c_int64]),
     "spp_pair_distribution": (C.c_int, [C.c_void_p, C.c_double, C.c_int32, _I64, _D]),
     "spp_lyapunov_init": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_uint64]),
     "spp_lyapunov_renormalize": (C.c_int, [C.c_void_p, _D]),
@@ -325,6 +326,14 @@ class Engine:
         p, sx, sy = C.c_double(), C.c_double(), C.c_double()
         self._check(self._lib.spp_polarization(self._h, C.byref(p), C.byref(sx), C.byref(sy)))
         return p.value, sx.value, sy.value
+
+    def multi_step_sampled(self, steps: int, every: int):
+        """Run `steps` steps, sampling every system's polarisation every `every` steps on the
+        device; returns [steps // every, systems]."""
+        ns = len(self.sizes)
+        out = np.empty((steps // every) * ns)
+        self._check(self._lib.spp_multi_step_sampled(self._h, steps, every, _ptr(out), out.size))
+        return out.reshape(steps // every, ns)
 
     def batch_polarization(self):
         p = np.empty(len(self.sizes))

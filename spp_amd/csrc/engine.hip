@@ -252,22 +252,21 @@ __global__ __launch_bounds__(kBlock) void k_scatter(int N, const int* __restrict
 
 // Order each cell's particles by caller index, so the sorted layout is a pure function of the
 // positions (the atomic scatter leaves arrival order).
-__global__ __launch_bounds__(kBlock) void k_cell_sort(int nkeys, const int* __restrict__ start,
-                                                      int* __restrict__ perm,
-                                                      const int* __restrict__ id)
+// Canonical order inside each cell (the scatter's atomics leave it arbitrary): the slot of a
+// particle is its cell's start plus the number of cell mates with a smaller id.  One thread per
+// particle, so a crowded cell costs each of its particles one pass over the cell.
+__global__ __launch_bounds__(kBlock) void k_cell_rank(int N, const int* __restrict__ key, const int* __restrict__ start,
+                                                      const int* __restrict__ perm, const int* __restrict__ id,
+                                                      int* __restrict__ perm_out)
 {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nkeys) return;
-    int s = start[c], e = start[c + 1];
-    for (int a = s + 1; a < e; ++a) {
-        int v = perm[a], kv = id[v];
-        int b = a - 1;
-        while (b >= s && id[perm[b]] > kv) {
-            perm[b + 1] = perm[b];
-            --b;
-        }
-        perm[b + 1] = v;
-    }
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    const int me = perm[j], c = key[me], mine = id[me];
+    const int s = start[c], e = start[c + 1];
+    int r = 0;
+#pragma unroll 4
+    for (int q = s; q < e; ++q) r += id[perm[q]] < mine;
+    perm_out[s + r] = me;
 }
 
 struct SetPtrs {
@@ -1528,6 +1527,7 @@ struct spp_engine {
     int *key = nullptr, *perm = nullptr, *cell_count = nullptr, *cell_start = nullptr,
         *block_sums = nullptr;
     int* inv = nullptr;          // caller index -> slot of the current set
+    int* perm2 = nullptr;        // the scatter's order, canonicalised within cells
     AbpConst abp{};
     SlotBufs slotb[2] = {};      // per-block partials of the (at most two) intervals in flight
     Report* hrep = nullptr;      // host-mapped reports, 2 slots
@@ -1539,6 +1539,10 @@ struct spp_engine {
     SysInfo* dsys = nullptr;
     int* sys_of = nullptr;
     double2* dpolar_sys = nullptr;
+    SysInfo* dsys1 = nullptr;    // one-entry table describing a single-box engine (sampling)
+    double2* dsamples = nullptr; // heading sums recorded during spp_multi_step_sampled
+    int64_t samples_alloc = 0, sample_cap = 0, sample_base = 0;
+    int sample_every = 0;
     hipEvent_t slot_event[2] = {nullptr, nullptr};
     std::map<int, Snapshot> slots;
     LyapState ly;
@@ -1725,14 +1729,14 @@ int launch_rebuild(spp_engine* e, int dst, int slot, bool aux)
                (const int*)e->block_sums, e->cell_start));
     TRY(launch(e, "scatter", k_scatter, nblocks(N), N, (const int*)e->key,
                (const int*)e->cell_start, e->cell_count, e->perm));
-    TRY(launch(e, "cell_sort", k_cell_sort, nblocks(nk), nk, (const int*)e->cell_start, e->perm,
-               (const int*)e->set[src].id));
+    TRY(launch(e, "cell_rank", k_cell_rank, nblocks(N), N, (const int*)e->key, (const int*)e->cell_start,
+               (const int*)e->perm, (const int*)e->set[src].id, e->perm2));
     if (e->variant == SPP_KERNEL_STRIP_LDS)
         TRY(launch(e, "chunks", k_chunks, nblocks(e->nchunks), e->N, e->nchunks, e->G.ncx, e->nkeys,
-                   (const int*)e->cell_start, (const int*)e->key, (const int*)e->perm, e->chunks));
+                   (const int*)e->cell_start, (const int*)e->key, (const int*)e->perm2, e->chunks));
     PermuteArgs pa{};
     pa.N = N;
-    pa.perm = e->perm;
+    pa.perm = e->perm2;
     pa.in = e->set[src];
     pa.out = e->set[dst];
     pa.inv = e->inv;
@@ -1868,6 +1872,15 @@ int launch_step(spp_engine* e, int S, int W, int kstep, int slot, bool record)
     e->step++;
     e->t += e->p.dt;
     e->steps_since_build++;
+    if (e->sample_every > 0 && (e->step - e->sample_base) % e->sample_every == 0) {
+        // heading sums of every system after this step (a replay after a rollback rewrites them)
+        int64_t k = (e->step - e->sample_base) / e->sample_every - 1;
+        if (k >= 0 && k < e->sample_cap) {
+            const int ns = e->dsys ? e->nsys : 1;
+            TRY(launch(e, "polar_sample", k_polar_sys, ns, (const SysInfo*)(e->dsys ? e->dsys : e->dsys1),
+                       (const P4*)e->set[W].pos, e->dsamples + k * ns));
+        }
+    }
     return SPP_OK;
 }
 
@@ -2368,6 +2381,7 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
     A(dalloc(e, &e->build, N));
     A(dalloc(e, &e->key, N));
     A(dalloc(e, &e->perm, N));
+    A(dalloc(e, &e->perm2, N));
     A(dalloc(e, &e->inv, N));
     A(dalloc(e, &e->cell_count, (size_t)e->nkeys));
     A(dalloc(e, &e->cell_start, (size_t)e->nkeys + 1));
@@ -2446,8 +2460,8 @@ void spp_destroy(spp_engine* e)
     for (int k = 0; k < 3; ++k) free_set(e->set[k]);
     void* bufs[] = {e->pos1, e->pos2, e->th1, e->th2, e->dth, e->dth_alt, e->inj_host_order,
                     e->inj, e->kacc, e->force, e->old, e->aux_alt[0], e->aux_alt[1], e->aux_alt[2],
-                    e->nbr16, e->nbr32, e->chunks, e->ncount, e->cellxy, e->build, e->key, e->perm, e->inv, e->cell_count,
-                    e->cell_start, e->block_sums, e->dpolar, e->dsys, e->sys_of, e->dpolar_sys};
+                    e->nbr16, e->nbr32, e->chunks, e->ncount, e->cellxy, e->build, e->key, e->perm, e->perm2, e->inv, e->cell_count,
+                    e->cell_start, e->block_sums, e->dpolar, e->dsys, e->sys_of, e->dpolar_sys, e->dsys1, e->dsamples};
     for (void* b : bufs) dfree(b);
     for (auto& b : e->slotb) {
         dfree(b.blkbad);
@@ -2720,6 +2734,46 @@ int spp_multi_step(spp_engine* e, int32_t steps)
 }
 
 int spp_one_step(spp_engine* e) { return spp_multi_step(e, 1); }
+
+int spp_multi_step_sampled(spp_engine* e, int32_t steps, int32_t every, double* polar, int64_t len)
+{
+    TRY(check_ready(e));
+    if (every <= 0 || steps < 0 || steps % every) return fail(e, SPP_ERR_ARG, "steps must be a multiple of every > 0");
+    const int ns = e->dsys ? e->nsys : 1;
+    const int64_t nsamp = steps / every;
+    if (!polar || len != nsamp * ns)
+        return fail(e, SPP_ERR_ARG, "polar needs %lld entries", (long long)(nsamp * ns));
+    if (!e->dsys && !e->dsys1) {
+        SysInfo S{};
+        S.G = e->G;
+        S.n = e->N;
+        TRY(dalloc(e, &e->dsys1, 1));
+        HIPC(e, hipMemcpy(e->dsys1, &S, sizeof S, hipMemcpyHostToDevice));
+    }
+    if (e->samples_alloc < nsamp * ns) {
+        dfree(e->dsamples);
+        e->dsamples = nullptr;
+        e->samples_alloc = 0;
+        TRY(dalloc(e, &e->dsamples, (size_t)(nsamp * ns)));
+        e->samples_alloc = nsamp * ns;
+    }
+    e->sample_every = every;
+    e->sample_base = e->step;
+    e->sample_cap = nsamp;
+    int rc = spp_multi_step(e, steps);
+    e->sample_every = 0;
+    if (rc != SPP_OK) return rc;
+    std::vector<double2> h(nsamp * ns);
+    HIPC(e, hipMemcpyAsync(h.data(), e->dsamples, h.size() * sizeof(double2), hipMemcpyDeviceToHost, e->stream));
+    HIPC(e, hipStreamSynchronize(e->stream));
+    for (int64_t k = 0; k < nsamp; ++k)
+        for (int s = 0; s < ns; ++s) {
+            const double n = e->dsys ? e->hsys[s].n : e->N;
+            const double ax = h[k * ns + s].x / n, ay = h[k * ns + s].y / n;
+            polar[k * ns + s] = std::sqrt(ax * ax + ay * ay);
+        }
+    return SPP_OK;
+}
 
 int spp_polarization(spp_engine* e, double* p, double* sum_x, double* sum_y)
 {

@@ -104,15 +104,13 @@ def run_points(points, steps, sample_every, seed, timer=None, dt=0.01, state=Non
             ts.append(t)
     e.set_state(np.concatenate(xs), np.concatenate(ys), np.concatenate(ts))
     ntot = int(sum(n for *_, n in points))
-    series = []
-    for _ in range(steps // sample_every):
-        timer.run(e.multi_step, sample_every, particles=ntot, steps=sample_every)
-        series.append(timer.run(e.batch_polarization))
+    steps -= steps % sample_every
+    series = timer.run(e.multi_step_sampled, steps, sample_every, particles=ntot, steps=steps)
     st = e.get_state()
     off = e.offsets
     final = [{k: v[off[i]:off[i + 1]] for k, v in st.items()} for i in range(len(points))]
     e.close()
-    return np.array(series), final
+    return series, final
 
 
 def run_lyapunov(points, warm, steps, seed, sample_every=10, interval=10, eps=1e-6, dt=0.01, timer=None,
@@ -238,39 +236,50 @@ class Curve:
         return bool(np.all(self.err <= tol) and np.all(ok_gap))
 
 
-def _converge(points, tol, chunk, sample_every, max_steps, seed, timer, min_samples=40):
-    """Run `points` in one batch until every error bar is <= tol (systems that are done leave the
-    batch at the next chunk boundary).  Returns (mean, error) per point."""
+def _converge(points, tol, chunk, sample_every, max_steps, seed, timer, min_samples=40, reps=None):
+    """Run `points` in one batch until every error bar is <= tol.  Point i runs as reps[i]
+    independent replicas (other noise keys and initial positions); its value is the mean of the
+    replicas' means and its error bar sqrt(sum of squared replica errors) / replicas.  A point whose
+    error bar is small enough leaves the batch at the next chunk boundary.  Returns (mean, error)."""
     n = len(points)
-    series = [np.zeros(0) for _ in range(n)]
+    reps = [1] * n if reps is None else [max(1, int(r)) for r in reps]
+    owner = [i for i in range(n) for _ in range(reps[i])]
+    series = [np.zeros(0) for _ in owner]
     state = None
-    active = list(range(n))
+    active = list(range(len(owner)))  # systems
     done_mean = np.zeros(n)
     done_err = np.full(n, np.inf)
     steps = 0
     seg = 0
+
+    def estimate(i):
+        ms, es = [], []
+        for q, o in enumerate(owner):
+            if o == i:
+                m, e, _ = polarity_error(np.arange(series[q].size), series[q])
+                ms.append(m)
+                es.append(e)
+        return float(np.mean(ms)), float(np.sqrt(np.sum(np.square(es))) / len(es))
+
     while active and steps < max_steps:
-        pts = [points[i] for i in active]
+        pts = [points[owner[q]] for q in active]
         # a new segment re-keys the noise so it does not repeat the previous segment's stream
         s, state = run_points(pts, chunk, sample_every, seed + 104729 * seg, timer, state=state)
         seg += 1
         steps += chunk
-        keep = []
-        keep_state = []
-        for j, i in enumerate(active):
-            series[i] = np.concatenate([series[i], s[:, j]])
-            if series[i].size >= min_samples:
-                m, err, _ = polarity_error(np.arange(series[i].size), series[i])
-                done_mean[i], done_err[i] = m, err
-                if err <= tol:
-                    continue
-            keep.append(i)
-            keep_state.append(state[j])
-        active = keep
-        state = keep_state
-    for i in active:  # out of steps: report what was measured
-        m, err, _ = polarity_error(np.arange(series[i].size), series[i])
-        done_mean[i], done_err[i] = m, err
+        for j, q in enumerate(active):
+            series[q] = np.concatenate([series[q], s[:, j]])
+        finished = set()
+        if series[active[0]].size >= min_samples:
+            for i in {owner[q] for q in active}:
+                done_mean[i], done_err[i] = estimate(i)
+                if done_err[i] <= tol:
+                    finished.add(i)
+        keep = [(q, st) for q, st in zip(active, state) if owner[q] not in finished]
+        active = [q for q, _ in keep]
+        state = [st for _, st in keep]
+    for i in {owner[q] for q in active}:  # out of steps: report what was measured
+        done_mean[i], done_err[i] = estimate(i)
     return done_mean, done_err
 
 
@@ -286,9 +295,9 @@ def plain_curve(D_grid, rho, v0, n, tol=0.02, chunk=2000, sample_every=10, max_s
 
 
 def _refine(D, P, E, points_fn, tol, gap, dD_min, chunk, sample_every, max_steps, seed, timer, rounds, log,
-            max_rounds):
+            max_rounds, reps_fn=None):
     """Add simulated midpoints between neighbours that differ by more than `gap` and are more than
-    2 dD_min apart."""
+    2 dD_min apart (`reps_fn(D)`: replicas per new point)."""
     while rounds < max_rounds:
         o = np.argsort(D)
         D, P, E = D[o], P[o], E[o]
@@ -296,7 +305,8 @@ def _refine(D, P, E, points_fn, tol, gap, dD_min, chunk, sample_every, max_steps
         if bad.size == 0:
             break
         mids = 0.5 * (D[bad] + D[bad + 1])
-        m, e = _converge(points_fn(mids), tol, chunk, sample_every, max_steps, seed + 1009 * rounds, timer)
+        reps = reps_fn(mids) if reps_fn else None
+        m, e = _converge(points_fn(mids), tol, chunk, sample_every, max_steps, seed + 1009 * rounds, timer, reps=reps)
         log.append(("refine", len(mids)))
         D, P, E = np.concatenate([D, mids]), np.concatenate([P, m]), np.concatenate([E, e])
         rounds += 1
@@ -330,9 +340,18 @@ def steered_points(s: Surrogate, D_lo, D_hi, rho, v0, n, count, tol, fine=512):
     return np.interp(np.linspace(0, arc[-1], count), arc, Dg)
 
 
+def replicas_for(s: Surrogate, D, rho, v0, n, tol, max_reps=16, per_rep=8.0):
+    """Replicas per point from the predicted polarisation variance: var / (per_rep tol^2), at least
+    one -- the noisy points near the transition run as several shorter independent runs side by
+    side in the batch instead of one long run."""
+    _, var, _ = predicted_curve(s, np.asarray(D, dtype=np.float64), rho, v0, n)
+    return np.clip(np.ceil(var / (per_rep * tol * tol)), 1, max_reps).astype(int)
+
+
 def steered_curve(s: Surrogate, D_lo, D_hi, rho, v0, n, count=None, tol=0.02, gap=None, dD_min=None, chunk=2000,
-                  sample_every=10, max_steps=200000, seed=1, max_rounds=6):
-    """Surrogate-placed points in one batched engine, then simulated refinement where needed."""
+                  sample_every=10, max_steps=200000, seed=1, max_rounds=6, max_reps=16):
+    """Surrogate-placed points in one batched engine, each with as many replicas as its predicted
+    variance asks for, then simulated refinement where needed."""
     gap = 3 * tol if gap is None else gap
     dD_min = (D_hi - D_lo) / 64 if dD_min is None else dD_min
     timer = Timer()
@@ -342,8 +361,10 @@ def steered_curve(s: Surrogate, D_lo, D_hi, rho, v0, n, count=None, tol=0.02, ga
         count = int(np.ceil(np.abs(np.diff(Pm)).sum() / gap)) + 3
     D = steered_points(s, D_lo, D_hi, rho, v0, n, count, gap)
     pts = lambda Ds: [(float(x), rho, v0, n) for x in Ds]
-    m, e = _converge(pts(D), tol, chunk, sample_every, max_steps, seed, timer)
-    log = [("steered", len(D))]
+    reps_fn = lambda Ds: replicas_for(s, Ds, rho, v0, n, tol, max_reps)
+    reps = reps_fn(D)
+    m, e = _converge(pts(D), tol, chunk, sample_every, max_steps, seed, timer, reps=reps)
+    log = [("steered", len(D), int(reps.sum()))]
     D, P, E, rounds = _refine(D, m, e, pts, tol, gap, dD_min, chunk, sample_every, max_steps, seed, timer, 1, log,
-                              max_rounds)
+                              max_rounds, reps_fn)
     return Curve(D, P, E, timer.seconds, timer.particle_steps, len(D), rounds, log)

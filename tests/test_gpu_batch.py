@@ -301,3 +301,29 @@ def test_lyapunov_rejects_inconsistent_groups():
         e.lyapunov_init(1, 1, 1e-6)  # clone has another noise seed
     with pytest.raises(S.SppError):
         e.lyapunov_init(2, 1, 1e-6)  # 2 x (1 + 1) != 2 systems
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_sampled_steps_equal_stepwise_polarization(batched):
+    """Samples recorded on the device during one call equal polarisations read after each chunk of
+    a step-by-step run (fixed rebuild period: identical rebuild steps, so identical states)."""
+    p = vicsek_params(rebuild=S.REBUILD_PERIOD, rebuild_period=7, D_rot=0.3)
+    systems = ragged_systems(3, seed=4) if batched else None
+    if batched:
+        x, y, th = initial_states(systems, 9)
+    else:
+        p.update(n=2000, Lx=20.0, Ly=20.0, seed=5)
+        x, y, th = S.random_state(9, 2000, 20.0, 20.0)
+    a, b = S.Engine(p, systems), S.Engine(p, systems)
+    a.set_state(x, y, th)
+    b.set_state(x, y, th)
+    got = a.multi_step_sampled(300, 20)
+    want = []
+    for _ in range(15):
+        b.multi_step(20)
+        want.append(b.batch_polarization() if batched else [b.polarization()[0]])
+    assert got.shape == (15, len(a.sizes))
+    if batched:  # the same per-system reduction
+        assert np.array_equal(got, np.array(want))
+    else:  # one workgroup's sum against the block-partial sum of spp_polarization
+        assert np.abs(got - np.array(want)).max() < 1e-14
