@@ -291,6 +291,19 @@ int spp_time_stage(spp_engine* e, int32_t stage, int32_t reps, double* avg_us);
    (Random_Formation, shared/set-up.h:152-163, with a counter-based generator).                   */
 int spp_random_state(uint64_t seed, int32_t n, double Lx, double Ly, double* x, double* y,
                      double* theta);
+
+/* Triangle_Lattice_Formation (shared/set-up.h:116-145): n sites of a triangular lattice with
+   spacing (2 Lx - sigma) / Nx, Nx = (int)sqrt(Lx n sqrt(3) / (2 Ly)) + 1, offset by -(L - sigma)
+   and wrapped into the box, in the reference's site order.                                      */
+int spp_triangle_lattice(int32_t n, double Lx, double Ly, double sigma, double* x, double* y);
+/* Ring_Membrane (shared/set-up.h:292-316): nm beads on a circle about the origin; radius
+   0.5 bead_diameter / sin(pi / nm) (touching beads) unless radius > 0 is given.                 */
+int spp_ring_membrane(int32_t nm, double bead_diameter, double radius, double* x, double* y);
+/* n points uniform in the disk of the given radius about the origin, headings uniform in
+   [0, 2 pi), from the Philox stream keyed by seed (swimmers placed inside a ring membrane; the
+   reference's Confined_In_Ring_Membrane, set-up.h:319-366, reads an uninitialised swimmer length
+   and is not restated).                                                                          */
+int spp_disk_state(uint64_t seed, int32_t n, double radius, double* x, double* y, double* theta);
 /* Standard-normal variate of the angular-noise stream for (seed, particle id, step); the engine's
    d_theta is sqrt(2 D_rot dt) times this value.                                                   */
 double spp_noise_normal(uint64_t seed, int32_t id, int64_t step);

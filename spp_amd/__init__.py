@@ -158,6 +158,9 @@ SIGNATURES = {
     "spp_time_stage": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _D]),
     "spp_random_state": (C.c_int, [C.c_uint64, C.c_int32, C.c_double, C.c_double, _D, _D, _D]),
     "spp_noise_normal": (C.c_double, [C.c_uint64, C.c_int32, C.c_int64]),
+    "spp_triangle_lattice": (C.c_int, [C.c_int32, C.c_double, C.c_double, C.c_double, _D, _D]),
+    "spp_ring_membrane": (C.c_int, [C.c_int32, C.c_double, C.c_double, _D, _D]),
+    "spp_disk_state": (C.c_int, [C.c_uint64, C.c_int32, C.c_double, _D, _D, _D]),
 }
 
 _lib = None
@@ -205,6 +208,30 @@ def random_state(seed: int, n: int, Lx: float, Ly: float):
     rc = lib().spp_random_state(seed, n, Lx, Ly, _ptr(x), _ptr(y), _ptr(th))
     if rc:
         raise SppError(rc, "spp_random_state")
+    return x, y, th
+
+
+def triangle_lattice(n: int, Lx: float, Ly: float, sigma: float = 0.0):
+    x, y = np.empty(n), np.empty(n)
+    rc = lib().spp_triangle_lattice(n, Lx, Ly, sigma, _ptr(x), _ptr(y))
+    if rc:
+        raise SppError(rc, "spp_triangle_lattice")
+    return x, y
+
+
+def ring_membrane(nm: int, bead_diameter: float = 1.0, radius: float = 0.0):
+    x, y = np.empty(nm), np.empty(nm)
+    rc = lib().spp_ring_membrane(nm, bead_diameter, radius, _ptr(x), _ptr(y))
+    if rc:
+        raise SppError(rc, "spp_ring_membrane")
+    return x, y
+
+
+def disk_state(seed: int, n: int, radius: float):
+    x, y, th = np.empty(n), np.empty(n), np.empty(n)
+    rc = lib().spp_disk_state(seed, n, radius, _ptr(x), _ptr(y), _ptr(th))
+    if rc:
+        raise SppError(rc, "spp_disk_state")
     return x, y, th
 
 

@@ -3139,6 +3139,59 @@ int spp_random_state(uint64_t seed, int32_t n, double Lx, double Ly, double* x, 
     return SPP_OK;
 }
 
+int spp_triangle_lattice(int32_t n, double Lx, double Ly, double sigma, double* x, double* y)
+{
+    if (n <= 0 || !x || !y || !(Lx > 0) || !(Ly > 0)) return SPP_ERR_ARG;
+    // Triangle_Lattice_Formation, shared/set-up.h:116-145: rows of Nx sites along basis_1, rows
+    // stacked along basis_2, every second pair of rows shifted back by basis_1
+    const double Lx2 = 2 * Lx, Ly2 = 2 * Ly;
+    const int nx = (int)std::sqrt((Lx * n * std::sqrt(3.0)) / (2 * Ly)) + 1;
+    const double s = (Lx2 - sigma) / nx;
+    const double b1x = 1 * s, b1y = 0 * s, b2x = 0.5 * s, b2y = std::sqrt(3.0) / 2 * s;
+    for (int32_t i = 0; i < n; ++i) {
+        const int a1 = i % nx, a2 = i / nx, a3 = i / (2 * nx);
+        double rx = b1x * a1 + b2x * a2 - b1x * a3;
+        double ry = b1y * a1 + b2y * a2 - b1y * a3;
+        rx -= (Lx - sigma);
+        ry -= (Ly - sigma);
+        x[i] = rx - Lx2 * (double)(int)std::floor(rx / Lx2 + 0.5);
+        y[i] = ry - Ly2 * (double)(int)std::floor(ry / Ly2 + 0.5);
+    }
+    return SPP_OK;
+}
+
+int spp_ring_membrane(int32_t nm, double bead_diameter, double radius, double* x, double* y)
+{
+    if (nm < 3 || !x || !y) return SPP_ERR_ARG;
+    // Ring_Membrane, shared/set-up.h:292-316: radius 0.5 d / sin(pi / Nm) unless given; the angle
+    // accumulates 2 pi / Nm per bead
+    const double R = radius > 0 ? radius : 0.5 * bead_diameter / std::sin(spp::kPi / nm);
+    if (!(R > 0)) return SPP_ERR_ARG;
+    double theta = 0;
+    for (int32_t i = 0; i < nm; ++i) {
+        x[i] = R * std::cos(theta);
+        y[i] = R * std::sin(theta);
+        theta += 2 * spp::kPi / nm;
+    }
+    return SPP_OK;
+}
+
+int spp_disk_state(uint64_t seed, int32_t n, double radius, double* x, double* y, double* theta)
+{
+    if (n < 0 || !x || !y || !theta || !(radius > 0)) return SPP_ERR_ARG;
+    for (int32_t i = 0; i < n; ++i) {
+        uint32_t c[4] = {(uint32_t)i, 1u, 0u, 0x53505069u};
+        spp::philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+        const double u0 = (double)c[0] * (1.0 / 4294967296.0), u1 = (double)c[1] * (1.0 / 4294967296.0);
+        const double u2 = (double)c[2] * (1.0 / 4294967296.0);
+        const double r = radius * std::sqrt(u0), phi = spp::kTwoPi * u1;
+        x[i] = r * std::cos(phi);
+        y[i] = r * std::sin(phi);
+        theta[i] = spp::kTwoPi * u2;
+    }
+    return SPP_OK;
+}
+
 double spp_noise_normal(uint64_t seed, int32_t id, int64_t step)
 {
     return spp::noise_normal(seed, (uint32_t)id, step);
