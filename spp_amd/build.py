@@ -37,18 +37,24 @@ def build_engine(verbose: bool = False) -> str:
     return LIB
 
 
-def build_demo(verbose: bool = False) -> str:
-    """The C++ Box facade driver (host code only, linked against the engine library)."""
-    out = os.path.join(HERE, "box_demo")
-    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "box_demo.cpp"),
-           "-L", HERE, "-lspp_engine", "-Wl,-rpath,$ORIGIN", "-o", out]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        sys.stderr.write(r.stderr)
-        raise RuntimeError("g++ failed for box_demo")
-    return out
+DEMOS = ("box_demo", "vesicle_demo")
+
+
+def build_demo(verbose: bool = False) -> list:
+    """The C++ Box facade drivers (host code only, linked against the engine library)."""
+    outs = []
+    for name in DEMOS:
+        out = os.path.join(HERE, name)
+        cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, name + ".cpp"),
+               "-L", HERE, "-lspp_engine", "-Wl,-rpath,$ORIGIN", "-o", out]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stderr)
+            raise RuntimeError("g++ failed for " + name)
+        outs.append(out)
+    return outs
 
 
 if __name__ == "__main__":

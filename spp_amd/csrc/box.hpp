@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/spp_engine.h"
+#include "membrane.hpp"
 #include "trajectory.hpp"
 
 namespace spp_gpu {
@@ -60,7 +61,7 @@ public:
     {
         std::ifstream is(name, std::ios::binary);
         if (!is) return false;
-        auto frames = spp_traj::read_healthy_frames(is, true);
+        auto frames = spp_traj::read_healthy_frames(is, writes_theta());
         if (frames.empty()) return false;
         const spp_traj::Frame& f = frames.back();
         if (f.Ns() + f.Nm() != N) return false;
@@ -115,6 +116,42 @@ public:
         os << t << "\t" << polarization << std::endl;
     }
 
+    // Vesicle observables (serial/beadbox.h:419-506): Save_Particles_Positions keeps the
+    // unwrapped positions and headings as the reference for the next finite-difference velocity;
+    // Compute_All_Variables evaluates the membrane/swimmer moments and saves the positions again;
+    // Save_All_Variables appends one row (and the header, once, to an empty stream).
+    MembraneVariables variables;
+
+    void Save_Particles_Positions()
+    {
+        std::vector<double> xw, yw, th;
+        Get_State(xw, yw, x_old_, y_old_, th);
+        t_old_ = t;
+    }
+    void Compute_All_Variables()
+    {
+        if (Nm <= 0) throw Error(SPP_ERR_ARG, "Compute_All_Variables needs ring beads (n_membrane > 0)");
+        if ((int)x_old_.size() != N) Save_Particles_Positions();
+        std::vector<double> xw, yw, xu, yu, th;
+        Get_State(xw, yw, xu, yu, th);
+        variables = membrane_variables(xu.data(), yu.data(), x_old_.data(), y_old_.data(), Nm, Ns, t - t_old_);
+        x_old_.swap(xu);
+        y_old_.swap(yu);
+        t_old_ = t;
+    }
+    void Save_All_Variables(std::ostream& os)
+    {
+        Compute_All_Variables();
+        if (!header_written_) {
+            // the header goes only into an empty file: an appending stream reports position 0
+            // until its first write, so ask the buffer for the end (no stream state on failure)
+            auto end = os.rdbuf() ? os.rdbuf()->pubseekoff(0, std::ios::end, std::ios::out) : std::streampos(-1);
+            if (end == std::streampos(0)) write_variables_header(os);
+            header_written_ = true;
+        }
+        write_variables_row(os, t, variables);
+    }
+
     // Current state in caller order.
     void Get_State(std::vector<double>& xw, std::vector<double>& yw, std::vector<double>& xu,
                    std::vector<double>& yu, std::vector<double>& th)
@@ -138,8 +175,10 @@ public:
         f.Ly = box->params_.Ly;
         f.nb = 1;
         // continuous particles write the wrapped r (BasicDynamicParticle::Write), the others
-        // r_original (RepulsiveParticle::Write, ActiveBrownianChain::Write)
+        // r_original (RepulsiveParticle::Write, ActiveBrownianChain::Write); swimmers without
+        // self-propulsion write no heading (particle.h:1038-1045)
         bool wrapped = box->params_.model == SPP_MODEL_CONTINUOUS;
+        bool with_theta = box->writes_theta();
         for (int i = 0; i < box->Nm; ++i) {
             f.mx.push_back((float)xu[i]);
             f.my.push_back((float)yu[i]);
@@ -147,11 +186,14 @@ public:
         for (int i = box->Nm; i < box->N; ++i) {
             f.x.push_back((float)(wrapped ? xw[i] : xu[i]));
             f.y.push_back((float)(wrapped ? yw[i] : yu[i]));
-            f.theta.push_back((float)th[i]);
+            if (with_theta) f.theta.push_back((float)th[i]);
         }
         spp_traj::write_frame(os, f);
         return os;
     }
+
+    // Whether frames carry the particles' headings: all models except swimmers with F0 == 0.
+    bool writes_theta() const { return !(params_.model == SPP_MODEL_ABP && params_.v0 == 0.0); }
 
 private:
     void stage(int s) { check(spp_move_stage(e_, s)); sync_time(); }
@@ -177,6 +219,9 @@ private:
     spp_params params_;
     spp_engine* e_ = nullptr;
     double t0_ = 0, base_ = 0;
+    std::vector<double> x_old_, y_old_;
+    double t_old_ = 0;
+    bool header_written_ = false;
 };
 
 }  // namespace spp_gpu
