@@ -120,7 +120,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_abp(StageArgs a)
         if (INTEG == SPP_EULER) {
             th = th_in + A.mob_rot * P.dt * (tq);
             th += dn;
-            sincos(th, &sn, &cs);
+            spp::sincos_heading(th, &sn, &cs);
             u.x += fx * (P.dt * A.mob_perp);
             u.y += fy * (P.dt * A.mob_perp);
             double fv = fx * cs + fy * sn;
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_abp(StageArgs a)
             yy += me.s * (fv * (P.half_dt * dm));
             x = spp::periodic_wrap(xx, P.Lx, P.Lx2);
             y = spp::periodic_wrap(yy, P.Ly, P.Ly2);
-            sincos(th, &sn, &cs);
+            spp::sincos_heading(th, &sn, &cs);
         } else {
             th = thold + A.mob_rot * P.dt * (tq);
             th += dn;
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_abp(StageArgs a)
             a.U_out[i] = u;
             x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
             y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
-            sincos(th, &sn, &cs);
+            spp::sincos_heading(th, &sn, &cs);
         }
         a.out[i] = P4{x, y, cs, sn};
         a.th_out[i] = th;
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(kBlock) void k_move_abp(MoveArgs a)
         if (a.integ == SPP_EULER) {
             th = th_in + A.mob_rot * P.dt * (f.c);
             th += dn;
-            sincos(th, &sn, &cs);
+            spp::sincos_heading(th, &sn, &cs);
             u.x += f.x * (P.dt * A.mob_perp);
             u.y += f.y * (P.dt * A.mob_perp);
             double fv = f.x * cs + f.y * sn;
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(kBlock) void k_move_abp(MoveArgs a)
             yy += me.s * (fv * (P.half_dt * dm));
             x = spp::periodic_wrap(xx, P.Lx, P.Lx2);
             y = spp::periodic_wrap(yy, P.Ly, P.Ly2);
-            sincos(th, &sn, &cs);
+            spp::sincos_heading(th, &sn, &cs);
         }
     } else {
         P4 o = a.old[i];
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(kBlock) void k_move_abp(MoveArgs a)
         a.cur.U[i] = u;
         x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
         y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
-        sincos(th, &sn, &cs);
+        spp::sincos_heading(th, &sn, &cs);
     }
     a.cur.pos[i] = P4{x, y, cs, sn};
     a.cur.th[i] = th;

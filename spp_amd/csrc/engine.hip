@@ -778,7 +778,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int s, const Chun
                 y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
             }
             double sn, cs;
-            sincos(th, &sn, &cs);
+            spp::sincos_heading(th, &sn, &cs);
             a.out[i] = P4{x, y, cs, sn};
             a.th_out[i] = th;
         }
@@ -1047,7 +1047,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_flat(StageArgs a)
             y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
         }
         double sn, cs;
-        sincos(th, &sn, &cs);
+        spp::sincos_heading(th, &sn, &cs);
         a.out[i] = P4{x, y, cs, sn};
         a.th_out[i] = th;
     }
@@ -1153,21 +1153,26 @@ __global__ __launch_bounds__(kBlock) void k_move(MoveArgs a)
         y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
     }
     double sn, cs;
-    sincos(th, &sn, &cs);
+    spp::sincos_heading(th, &sn, &cs);
     a.cur.pos[i] = P4{x, y, cs, sn};
     a.cur.th[i] = th;
 }
 
 #include "abp_kernels.hpp"
 
-// Headings -> (cos, sin) after a host upload, computed by the same device sincos as the stages.
+// Headings -> (cos, sin) after a host upload, computed by the same routine as the stages (the
+// library's for uploaded angles beyond its range).
 __global__ __launch_bounds__(kBlock) void k_init_heading(int N, P4* __restrict__ pos,
                                                          const double* __restrict__ th)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     double sn, cs;
-    sincos(th[i], &sn, &cs);
+    const double t = th[i];
+    if (fabs(t) <= 524288.0)
+        spp::sincos_heading(t, &sn, &cs);
+    else
+        sincos(t, &sn, &cs);
     pos[i].c = cs;
     pos[i].s = sn;
 }

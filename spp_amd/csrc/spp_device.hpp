@@ -9,6 +9,7 @@
 #include <stdint.h>
 
 #include "../../include/spp_engine.h"
+#include "trig.hpp"
 
 namespace spp {
 
@@ -110,7 +111,9 @@ __host__ __device__ __forceinline__ double noise_normal(uint64_t seed, uint32_t 
     philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
     double ua = ((double)(c[0] >> 5) * 67108864.0 + (double)(c[1] >> 6)) * (1.0 / 9007199254740992.0);
     double ub = ((double)(c[2] >> 5) * 67108864.0 + (double)(c[3] >> 6)) * (1.0 / 9007199254740992.0);
-    return sqrt(-2.0 * log(1.0 - ua)) * cos(kTwoPi * ub);
+    double sn, cs;
+    sincos_heading(kTwoPi * ub, &sn, &cs);  // cos(2 pi u_b), the heading routine (|arg| < 2 pi)
+    return sqrt(-2.0 * log(1.0 - ua)) * cs;
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -1,0 +1,36 @@
+"""The heading sine/cosine of the stage kernels and of the noise transform
+(spp_amd/csrc/trig.hpp), on the CPU: the host build of the same code the device runs.
+
+Pinned against extended-precision sinl/cosl (x87, 64-bit significand): at most 1.5 ulp over
+[-4, 4] and near multiples of pi/2 (where the reduction cancels), absolute error below 1.2e-16,
+and the exact points sin 0 = 0, cos 0 = 1, sin(pi/2) = 1, cos(+-pi) = -1.
+"""
+import math
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def out(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("trig") / "trig_accuracy")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off",
+                    os.path.join(ROOT, "tests", "cpp", "trig_accuracy.cpp"), "-o", exe], check=True)
+    return subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")
+
+
+def test_error_bound(out):
+    es, ec, ea = (float(v) for v in out[0].split())
+    assert es <= 1.5 and ec <= 1.5 and ea < 1.2e-16
+
+
+def test_exact_points(out):
+    vals = {float(l.split()[0]): tuple(float(v) for v in l.split()[1:]) for l in out[1:] if l}
+    assert vals[0.0] == (0.0, 1.0)
+    assert vals[math.pi / 2][0] == 1.0 and abs(vals[math.pi / 2][1]) < 1e-16
+    assert vals[math.pi][1] == -1.0 and vals[-math.pi][1] == -1.0
+    for x in (0.5, -2.5):
+        assert vals[x] == pytest.approx((math.sin(x), math.cos(x)), rel=3e-16, abs=1e-17)
