@@ -4,7 +4,8 @@ The knobs do not change the physics; they trade gather strategy against list len
 frequency:
 
 * ``kernel_variant`` -- the interaction tile strategy: one lane per slot with L2 gathers (0), the
-  split-record gather (1), or 256-slot chunks staged in LDS (2; single box only);
+  split-record gather (1), 256-slot chunks staged in LDS (2; single box only), or one workgroup per
+  system running the whole call in one launch (3; systems of at most 4096 particles);
 * ``r_list`` -- the Verlet radius: a wider skin lengthens every list but rebuilds less often;
 * the cell size -- ``ncell_x/y`` below floor(2L / r_list) makes cells wider than the list radius
   (more candidates per list build, fewer empty cells).
@@ -19,8 +20,8 @@ import math
 import time
 from dataclasses import dataclass, field
 
-from . import (KERNEL_SLOT, KERNEL_SLOT_SPLIT, KERNEL_STRIP_LDS, MODEL_ABP, MODEL_REPULSIVE, REP_NONE, Engine,
-               SppError, System)
+from . import (KERNEL_PERSIST, KERNEL_SLOT, KERNEL_SLOT_SPLIT, KERNEL_STRIP_LDS, MODEL_ABP, MODEL_REPULSIVE,
+               PERSIST_MAX_N, REP_NONE, Engine, SppError, System)
 
 
 def interaction_cutoff(p):
@@ -73,6 +74,9 @@ def autotune(params, x, y, theta, systems=None, steps=200, warm=50, variants=Non
         variants = [KERNEL_SLOT, KERNEL_SLOT_SPLIT]
         if systems is None and params.model != MODEL_ABP:
             variants.append(KERNEL_STRIP_LDS)
+        sizes = [s.n for s in systems] if systems is not None else [params.n]
+        if params.model != MODEL_ABP and max(sizes) <= PERSIST_MAX_N:
+            variants.append(KERNEL_PERSIST)
     n = int(len(x))
     trials = []
     best = None

@@ -828,8 +828,64 @@ struct NlistFlatArgs {
     int* blknnz;
 };
 
-// Cell::Neighbor_List (shared/cell.h:74-103) over the 3x3 periodic stencil; each stencil row is
-// scanned as one or two contiguous slot ranges, four candidates at a time.
+// Cell::Neighbor_List (shared/cell.h:74-103) for slot i over the 3x3 periodic stencil of grid G
+// (cell keys from koff); each stencil row is scanned as one or two contiguous slot ranges, four
+// candidates at a time.  Returns the list length before the capacity cut.
+__device__ __forceinline__ int nlist_particle(const NlistFlatArgs& a, int i, const Grid& G, int koff)
+{
+    int c = 0;
+    P4 me = a.pos[i];
+    int cx = cell_coord(me.x, G.Lx, G.Lx2, G.ncx, G.quirk);
+    int cy = cell_coord(me.y, G.Ly, G.Ly2, G.ncy, G.quirk);
+    int x0 = cx - 1, x1 = cx + 1, xw = -1;
+    if (x0 < 0) {
+        xw = G.ncx - 1;
+        x0 = 0;
+    } else if (x1 >= G.ncx) {
+        xw = 0;
+        x1 = G.ncx - 1;
+    }
+    for (int oy = -1; oy <= 1; ++oy) {
+        int y = cy + oy;
+        y = y < 0 ? y + G.ncy : (y >= G.ncy ? y - G.ncy : y);
+        int row = koff + y * G.ncx;
+        for (int seg = 0; seg < 2; ++seg) {
+            int s, e;
+            if (seg == 0) {
+                s = a.start[row + x0];
+                e = a.start[row + x1 + 1];
+            } else {
+                if (xw < 0) break;
+                s = a.start[row + xw];
+                e = a.start[row + xw + 1];
+            }
+            for (int j = s; j < e; j += 4) {
+                double2 q[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (j + u < e) q[u] = *reinterpret_cast<const double2*>(&a.pos[j + u].x);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    int jj = j + u;
+                    if (jj < e && jj != i) {
+                        double dx = spp::min_image(me.x - q[u].x, G.Lx2, G.invLx2);
+                        double dy = spp::min_image(me.y - q[u].y, G.Ly2, G.invLy2);
+                        double d2 = dx * dx + dy * dy;
+                        if (d2 < a.T_list) {
+                            if (c < a.cap) a.nbr32[(size_t)c * a.N + i] = jj;
+                            ++c;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    a.ncount[i] = c < a.cap ? c : a.cap;
+    a.build[i] = make_float2((float)me.x, (float)me.y);
+    a.cellxy[i] = cx | (cy << 16);
+    return c;
+}
+
 __global__ __launch_bounds__(kBlock) void k_nlist_flat(NlistFlatArgs a)
 {
     __shared__ int red_len[kBlock / 64], red_nnz[kBlock / 64];
@@ -844,55 +900,7 @@ __global__ __launch_bounds__(kBlock) void k_nlist_flat(NlistFlatArgs a)
             G = S.G;
             koff = S.key_off;
         }
-        P4 me = a.pos[i];
-        int cx = cell_coord(me.x, G.Lx, G.Lx2, G.ncx, G.quirk);
-        int cy = cell_coord(me.y, G.Ly, G.Ly2, G.ncy, G.quirk);
-        int x0 = cx - 1, x1 = cx + 1, xw = -1;
-        if (x0 < 0) {
-            xw = G.ncx - 1;
-            x0 = 0;
-        } else if (x1 >= G.ncx) {
-            xw = 0;
-            x1 = G.ncx - 1;
-        }
-        for (int oy = -1; oy <= 1; ++oy) {
-            int y = cy + oy;
-            y = y < 0 ? y + G.ncy : (y >= G.ncy ? y - G.ncy : y);
-            int row = koff + y * G.ncx;
-            for (int seg = 0; seg < 2; ++seg) {
-                int s, e;
-                if (seg == 0) {
-                    s = a.start[row + x0];
-                    e = a.start[row + x1 + 1];
-                } else {
-                    if (xw < 0) break;
-                    s = a.start[row + xw];
-                    e = a.start[row + xw + 1];
-                }
-                for (int j = s; j < e; j += 4) {
-                    double2 q[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u)
-                        if (j + u < e) q[u] = *reinterpret_cast<const double2*>(&a.pos[j + u].x);
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        int jj = j + u;
-                        if (jj < e && jj != i) {
-                            double dx = spp::min_image(me.x - q[u].x, G.Lx2, G.invLx2);
-                            double dy = spp::min_image(me.y - q[u].y, G.Ly2, G.invLy2);
-                            double d2 = dx * dx + dy * dy;
-                            if (d2 < a.T_list) {
-                                if (c < a.cap) a.nbr32[(size_t)c * a.N + i] = jj;
-                                ++c;
-                            }
-                        }
-                    }
-                }
-            }
-        }
-        a.ncount[i] = c < a.cap ? c : a.cap;
-        a.build[i] = make_float2((float)me.x, (float)me.y);
-        a.cellxy[i] = cx | (cy << 16);
+        c = nlist_particle(a, i, G, koff);
     }
     int m = c;
 #pragma unroll
@@ -971,6 +979,78 @@ __device__ __forceinline__ void gather_forces_flat(const StageArgs& a, const Phy
     tq = spp::drift_torque<MODEL>(P, s1, s2);
 }
 
+// One particle's fused stage (interaction at the stage input + the move), for slot i of a system
+// whose constants P already hold and whose particle indices start at pid_off.  Returns the wrapped
+// output position for the displacement check.
+template <int MODEL, int REP, int INTEG, int STAGE, bool SPLIT>
+__device__ __forceinline__ void stage_particle(const StageArgs& a, const Phys& P, int i, int pid_off, double& x,
+                                               double& y)
+{
+    P4 me = a.gather[i];
+    double dn = 0, th_in = 0, thold = 0;
+    double2 u = make_double2(0, 0);
+    P4 kk{0, 0, 0, 0};
+    if (STAGE <= 1) {
+        int pid = a.id[i];
+        if (a.id_out) a.id_out[i] = pid;
+        th_in = a.th_in[i];
+        if (INTEG == SPP_EULER) u = a.U_in[i];
+        // the stream is addressed by the particle's index within its system, so systems with
+        // equal seeds (Lyapunov clones) draw identical noise
+        dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)(pid - pid_off), a.step);
+    } else {
+        thold = a.th0[i];
+        dn = a.dth[i];
+        u = a.U_in[i];
+        if (INTEG == SPP_RK4) kk = a.kacc[i];
+    }
+    double fx, fy, tq;
+    gather_forces_flat<MODEL, REP, SPLIT>(a, P, i, me, fx, fy, tq);
+    double th;
+    if (STAGE <= 1) {
+        if (INTEG == SPP_EULER) {
+            th = th_in + P.dt * (tq);
+            th += dn;
+            u.x += fx * P.dt;
+            u.y += fy * P.dt;
+            a.U_out[i] = u;
+            x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
+            y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
+        } else {
+            a.dth[i] = dn;
+            th = th_in;
+            th += P.half_dt * tq;
+            th += dn / 2;
+            th = spp::wrap_angle(th);
+            x = spp::periodic_wrap(me.x + fx * P.half_dt, P.Lx, P.Lx2);
+            y = spp::periodic_wrap(me.y + fy * P.half_dt, P.Ly, P.Ly2);
+            if (INTEG == SPP_RK4) a.kacc[i] = P4{fx, fy, tq, 0.0};
+        }
+    } else {
+        double h = (INTEG == SPP_RK4 && STAGE == 2) ? P.half_dt : P.dt;
+        if (INTEG == SPP_RK4 && STAGE == 4) {
+            th = thold + (kk.c + tq) * P.dt6;
+            th += dn;
+            u.x = u.x + (kk.x + fx) * P.dt6;
+            u.y = u.y + (kk.y + fy) * P.dt6;
+        } else {
+            th = thold + h * tq;
+            th += (INTEG == SPP_RK4 && STAGE == 2) ? dn / 2 : dn;
+            u.x = u.x + fx * h;
+            u.y = u.y + fy * h;
+            if (INTEG == SPP_RK4) a.kacc[i] = P4{kk.x + fx * 2, kk.y + fy * 2, kk.c + tq * 2, 0.0};
+        }
+        th = spp::wrap_angle(th);
+        if (INTEG == SPP_RK2 || STAGE == 4) a.U_out[i] = u;
+        x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
+        y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
+    }
+    double sn, cs;
+    spp::sincos_heading(th, &sn, &cs);
+    a.out[i] = P4{x, y, cs, sn};
+    a.th_out[i] = th;
+}
+
 template <int MODEL, int REP, int INTEG, int STAGE, bool SPLIT, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_stage_flat(StageArgs a)
 {
@@ -986,71 +1066,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_flat(StageArgs a)
     }
     constexpr bool final_stage = (INTEG == SPP_EULER) || (INTEG == SPP_RK2 && STAGE == 2) || (STAGE == 4);
     double x = 0, y = 0, dmax = 0;
-    if (live) {
-        P4 me = a.gather[i];
-        double dn = 0, th_in = 0, thold = 0;
-        double2 u = make_double2(0, 0);
-        P4 kk{0, 0, 0, 0};
-        if (STAGE <= 1) {
-            int pid = a.id[i];
-            if (a.id_out) a.id_out[i] = pid;
-            th_in = a.th_in[i];
-            if (INTEG == SPP_EULER) u = a.U_in[i];
-            // the stream is addressed by the particle's index within its system, so systems with
-            // equal seeds (Lyapunov clones) draw identical noise
-            dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)(pid - pid_off), a.step);
-        } else {
-            thold = a.th0[i];
-            dn = a.dth[i];
-            u = a.U_in[i];
-            if (INTEG == SPP_RK4) kk = a.kacc[i];
-        }
-        double fx, fy, tq;
-        gather_forces_flat<MODEL, REP, SPLIT>(a, P, i, me, fx, fy, tq);
-        double th;
-        if (STAGE <= 1) {
-            if (INTEG == SPP_EULER) {
-                th = th_in + P.dt * (tq);
-                th += dn;
-                u.x += fx * P.dt;
-                u.y += fy * P.dt;
-                a.U_out[i] = u;
-                x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
-                y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
-            } else {
-                a.dth[i] = dn;
-                th = th_in;
-                th += P.half_dt * tq;
-                th += dn / 2;
-                th = spp::wrap_angle(th);
-                x = spp::periodic_wrap(me.x + fx * P.half_dt, P.Lx, P.Lx2);
-                y = spp::periodic_wrap(me.y + fy * P.half_dt, P.Ly, P.Ly2);
-                if (INTEG == SPP_RK4) a.kacc[i] = P4{fx, fy, tq, 0.0};
-            }
-        } else {
-            double h = (INTEG == SPP_RK4 && STAGE == 2) ? P.half_dt : P.dt;
-            if (INTEG == SPP_RK4 && STAGE == 4) {
-                th = thold + (kk.c + tq) * P.dt6;
-                th += dn;
-                u.x = u.x + (kk.x + fx) * P.dt6;
-                u.y = u.y + (kk.y + fy) * P.dt6;
-            } else {
-                th = thold + h * tq;
-                th += (INTEG == SPP_RK4 && STAGE == 2) ? dn / 2 : dn;
-                u.x = u.x + fx * h;
-                u.y = u.y + fy * h;
-                if (INTEG == SPP_RK4) a.kacc[i] = P4{kk.x + fx * 2, kk.y + fy * 2, kk.c + tq * 2, 0.0};
-            }
-            th = spp::wrap_angle(th);
-            if (INTEG == SPP_RK2 || STAGE == 4) a.U_out[i] = u;
-            x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
-            y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
-        }
-        double sn, cs;
-        spp::sincos_heading(th, &sn, &cs);
-        a.out[i] = P4{x, y, cs, sn};
-        a.th_out[i] = th;
-    }
+    if (live) stage_particle<MODEL, REP, INTEG, STAGE, SPLIT>(a, P, i, pid_off, x, y);
     check_disp(a, P, b, live, i, x, y, final_stage ? a.kstep + 1 : a.kstep, dmax);
     finish_disp(a, b, dmax);
 }
@@ -1460,6 +1476,7 @@ double sqrt_threshold(double R)
 }  // namespace
 
 #include "lyapunov_kernels.hpp"
+#include "persist_kernels.hpp"
 
 // ==================================================================================================
 // Engine
@@ -1565,6 +1582,11 @@ struct spp_engine {
     int max_nbrs = 0;
     double mean_nbrs = 0;
     int unfused_stage = 0;       // next expected stage of the split-call path (0 = step boundary)
+    // kernel_variant SPP_KERNEL_PERSIST: one workgroup per system runs whole intervals
+    bool persist = false;
+    bool persist_valid = false;  // the device statuses describe the current state and lists
+    persist::Status* pstatus = nullptr;
+    int64_t persist_rebuild_base = 0;
 
     bool profiling = false;
     std::vector<ProfRec> prof;
@@ -1762,6 +1784,7 @@ int launch_rebuild(spp_engine* e, int dst, int slot, bool aux)
     }
     e->cur = dst;
     TRY(launch_nlist(e, slot));
+    e->persist_valid = false;
     e->rebuilds++;
     e->steps_since_build = 0;
     e->list_valid = true;
@@ -2021,6 +2044,152 @@ int run_intervals(spp_engine* e, int steps)
     return SPP_OK;
 }
 
+using PersistFn = void (*)(persist::Args);
+
+template <int MODEL, int REP>
+PersistFn persist_integ(int integ)
+{
+    if (integ == SPP_EULER) return persist::k_persist<MODEL, REP, SPP_EULER>;
+    if (integ == SPP_RK2) return persist::k_persist<MODEL, REP, SPP_RK2>;
+    return persist::k_persist<MODEL, REP, SPP_RK4>;
+}
+
+PersistFn pick_persist(int model, int rep, int integ)
+{
+    if (model == SPP_MODEL_CONTINUOUS) return persist_integ<SPP_MODEL_CONTINUOUS, SPP_REP_NONE>(integ);
+    switch (rep) {
+    case SPP_REP_NONE: return persist_integ<SPP_MODEL_REPULSIVE, SPP_REP_NONE>(integ);
+    case SPP_REP_R12_TRUNCATED: return persist_integ<SPP_MODEL_REPULSIVE, SPP_REP_R12_TRUNCATED>(integ);
+    case SPP_REP_HARMONIC: return persist_integ<SPP_MODEL_REPULSIVE, SPP_REP_HARMONIC>(integ);
+    case SPP_REP_R12: return persist_integ<SPP_MODEL_REPULSIVE, SPP_REP_R12>(integ);
+    case SPP_REP_YUKAWA_TRUNCATED: return persist_integ<SPP_MODEL_REPULSIVE, SPP_REP_YUKAWA_TRUNCATED>(integ);
+    default: return persist_integ<SPP_MODEL_REPULSIVE, SPP_REP_YUKAWA>(integ);
+    }
+}
+
+// Steps through the persistent kernel: every system runs to the common target step in one launch;
+// a list longer than the capacity stops that system at the step it was building for, the lists are
+// reallocated and the launch repeated (finished systems return at once).
+int run_persist(spp_engine* e, int steps)
+{
+    const int ns = e->dsys ? e->nsys : 1;
+    std::vector<persist::Status> h(ns);
+    if (!e->persist_valid) {
+        for (auto& x : h) {
+            x = persist::Status{};
+            x.step = e->step;
+            x.since = -1;
+        }
+        HIPC(e, hipMemcpy(e->pstatus, h.data(), ns * sizeof(persist::Status), hipMemcpyHostToDevice));
+        e->persist_valid = true;
+        e->persist_rebuild_base = e->rebuilds;
+    }
+    const long long target = e->step + steps;
+    const int spare = free_set_other_than(e->cur, -1);
+    PersistFn fn = pick_persist(e->p.model, e->p.repulsion, e->p.integrator);
+    for (;;) {
+        persist::Args a{};
+        StageArgs& st = a.st;
+        st.N = e->N;
+        st.P = e->P;
+        st.abp = e->abp;
+        st.inv = e->inv;
+        st.sys_tab = e->dsys;
+        st.sys_of = e->sys_of;
+        st.dth = e->dth;
+        st.kacc = e->kacc;
+        st.nbr32 = e->nbr32;
+        st.ncount = e->ncount;
+        st.build = e->build;
+        a.nl = NlistFlatArgs{e->N, e->cap, e->G, e->dsys, e->sys_of, e->T_list, nullptr, e->cell_start, e->nbr32,
+                             e->ncount, e->build, e->cellxy, nullptr, nullptr};
+        a.sys_tab = e->dsys ? e->dsys : e->dsys1;
+        a.cur = e->set[e->cur];
+        a.spare = e->set[spare];
+        a.pos1 = e->pos1;
+        a.pos2 = e->pos2;
+        a.th1 = e->th1;
+        a.th2 = e->th2;
+        a.key = e->key;
+        a.count = e->cell_count;
+        a.start = e->cell_start;
+        a.perm = e->perm;
+        a.perm2 = e->perm2;
+        a.inv = e->inv;
+        a.target = target;
+        a.period = e->p.rebuild == SPP_REBUILD_PERIOD ? e->p.rebuild_period : 0;
+        a.nsys = ns;
+        a.thresh2 = e->thresh * e->thresh;
+        a.samples = e->sample_every > 0 ? e->dsamples : nullptr;
+        a.sample_every = e->sample_every;
+        a.sample_base = e->sample_base;
+        a.sample_cap = e->sample_cap;
+        a.status = e->pstatus;
+        ProfRec rec{"persist", nullptr, nullptr};
+        if (e->profiling) {
+            rec.a = take_event(e);
+            rec.b = take_event(e);
+            HIPC(e, hipEventRecord(rec.a, e->stream));
+        }
+        hipLaunchKernelGGL(fn, dim3(ns), dim3(persist::kBlock1024), 0, e->stream, a);
+        HIPC(e, hipGetLastError());
+        if (e->profiling) {
+            HIPC(e, hipEventRecord(rec.b, e->stream));
+            e->prof.push_back(rec);
+        }
+        HIPC(e, hipMemcpyAsync(h.data(), e->pstatus, ns * sizeof(persist::Status), hipMemcpyDeviceToHost, e->stream));
+        HIPC(e, hipStreamSynchronize(e->stream));
+        e->host_syncs++;
+        int need = 0;
+        long long nnz = 0;
+        for (int s = 0; s < ns; ++s) {
+            if (h[s].code == persist::kSkinJump)
+                return fail(e, SPP_ERR_STATE,
+                            "a particle moves farther than the half skin (%g) within one step: "
+                            "raise r_list or lower dt", e->half_skin);
+            if (h[s].code == persist::kNeedCap) need = std::max(need, h[s].need_cap);
+            e->max_nbrs = std::max(e->max_nbrs, h[s].max_len);
+            nnz += h[s].nnz;
+        }
+        e->mean_nbrs = (double)nnz / e->N;
+        if (need == 0) break;
+        // Longer lists.  Row k of slot i sits at k N + i, so the old array is a prefix of the new
+        // one: the other systems keep their lists (and so their rebuild steps); the systems that
+        // overflowed rebuild where they stopped.
+        const int old_cap = e->cap;
+        int* old32 = e->nbr32;
+        e->nbr32 = nullptr;
+        e->cap = ((need + 7) / 8) * 8 + 8;
+        dfree(e->nbr16);
+        e->nbr16 = nullptr;
+        TRY(dalloc(e, &e->nbr16, (size_t)e->cap * e->N));
+        TRY(dalloc(e, &e->nbr32, (size_t)e->cap * e->N));
+        HIPC(e, hipMemcpyAsync(e->nbr32, old32, (size_t)old_cap * e->N * sizeof(int), hipMemcpyDeviceToDevice,
+                               e->stream));
+        HIPC(e, hipStreamSynchronize(e->stream));
+        dfree(old32);
+        for (auto& x : h)
+            if (x.code == persist::kNeedCap) {
+                x.code = persist::kRunning;
+                x.since = -1;
+            }
+        HIPC(e, hipMemcpy(e->pstatus, h.data(), ns * sizeof(persist::Status), hipMemcpyHostToDevice));
+    }
+    int64_t rb = 0;
+    bool lists = true;
+    for (auto& x : h) {
+        rb += x.rebuilds;
+        lists = lists && x.since >= 0;
+    }
+    e->rebuilds = e->persist_rebuild_base + rb;
+    for (int k = 0; k < steps; ++k) e->t += e->p.dt;  // the per-step sum the multi-launch path forms
+    e->step = target;
+    // every system's list covers its state unless one still has to rebuild
+    e->list_valid = lists;
+    e->steps_since_build = 0;
+    return SPP_OK;
+}
+
 void lyap_free(LyapState& L)
 {
     void* bufs[] = {L.inv, L.status, L.dev, L.part, L.coef, L.rdiag, L.logsum, L.norm0};
@@ -2232,10 +2401,13 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
         return fail(nullptr, SPP_ERR_ARG, "noise_mode %d", p->noise_mode);
     if (p->rebuild != SPP_REBUILD_DISPLACEMENT && p->rebuild != SPP_REBUILD_PERIOD)
         return fail(nullptr, SPP_ERR_ARG, "rebuild %d", p->rebuild);
-    if (p->kernel_variant < SPP_KERNEL_SLOT || p->kernel_variant > SPP_KERNEL_STRIP_LDS)
+    if (p->kernel_variant < SPP_KERNEL_SLOT || p->kernel_variant > SPP_KERNEL_PERSIST)
         return fail(nullptr, SPP_ERR_ARG, "kernel_variant %d", p->kernel_variant);
-    if (systems && p->kernel_variant != SPP_KERNEL_SLOT)
-        return fail(nullptr, SPP_ERR_UNSUPPORTED, "batched engines run the slot kernels (kernel_variant 0)");
+    if (systems && p->kernel_variant != SPP_KERNEL_SLOT && p->kernel_variant != SPP_KERNEL_PERSIST)
+        return fail(nullptr, SPP_ERR_UNSUPPORTED, "batched engines run the slot kernels (kernel_variant 0 or 3)");
+    const bool persist = p->kernel_variant == SPP_KERNEL_PERSIST;
+    if (persist && p->noise_mode != SPP_NOISE_STREAM)
+        return fail(nullptr, SPP_ERR_UNSUPPORTED, "the persistent kernel draws its noise from the stream");
     if (p->rebuild == SPP_REBUILD_PERIOD && p->rebuild_period <= 0)
         return fail(nullptr, SPP_ERR_ARG, "rebuild_period must be positive");
     double rcut = max_cutoff(*p);
@@ -2264,6 +2436,9 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
             return fail(nullptr, SPP_ERR_ARG, "system %d: cell grid %dx%d invalid for r_list %g in box %gx%g",
                         s, d.ncx, d.ncy, p->r_list, 2 * d.Lx, 2 * d.Ly);
         if (d.ncx >= 32768 || d.ncy >= 32768) return fail(nullptr, SPP_ERR_ARG, "cell grid too large");
+        if (persist && d.n > persist::kMaxN)
+            return fail(nullptr, SPP_ERR_UNSUPPORTED, "system %d: %d particles, the persistent kernel takes up to %d",
+                        s, d.n, persist::kMaxN);
         ntot += d.n;
         keys += (int64_t)d.ncx * d.ncy;
         Lmax = std::max(Lmax, std::max(d.Lx, d.Ly));
@@ -2298,7 +2473,8 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
     e->nblk = nblocks(e->N);
     e->nchunks = (e->N + kChunk - 1) / kChunk;
     e->chunk_grid = (e->nchunks + 7) / 8 * 8;
-    e->variant = systems ? kVariantBatch : p->kernel_variant;
+    e->variant = systems ? kVariantBatch : (persist ? SPP_KERNEL_SLOT : p->kernel_variant);
+    e->persist = persist;
     e->nstat = std::max(e->chunk_grid, e->nblk);
     // the LDS stage kernels read list rows 0..kPfList-1 unconditionally
     e->cap = std::max(kPfList, p->ell_capacity > 0 ? p->ell_capacity : 32);
@@ -2405,6 +2581,21 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
         A(dalloc(e, &e->sys_of, N));
         A(dalloc(e, &e->dpolar_sys, (size_t)nsys));
     }
+    if (persist) {
+        A(dalloc(e, &e->pstatus, (size_t)nsys));
+        if (!systems) {  // the persistent kernel reads every constant from a system table
+            SysInfo S{};
+            S.G = e->G;
+            S.n = e->N;
+            S.v0 = e->P.v0;
+            S.c_align = e->P.c_align;
+            S.g = e->P.g;
+            S.noise_amp = e->P.noise_amp;
+            S.seed = e->P.seed;
+            A(dalloc(e, &e->dsys1, 1));
+            if (rc == SPP_OK && hipMemcpy(e->dsys1, &S, sizeof S, hipMemcpyHostToDevice) != hipSuccess) rc = SPP_ERR_HIP;
+        }
+    }
     if (rc == SPP_OK && hipHostMalloc((void**)&e->hrep, 2 * sizeof(Report), hipHostMallocMapped) != hipSuccess)
         rc = SPP_ERR_HIP;
     if (rc == SPP_OK && hipHostGetDevicePointer((void**)&e->drep, e->hrep, 0) != hipSuccess) rc = SPP_ERR_HIP;
@@ -2466,7 +2657,8 @@ void spp_destroy(spp_engine* e)
     void* bufs[] = {e->pos1, e->pos2, e->th1, e->th2, e->dth, e->dth_alt, e->inj_host_order,
                     e->inj, e->kacc, e->force, e->old, e->aux_alt[0], e->aux_alt[1], e->aux_alt[2],
                     e->nbr16, e->nbr32, e->chunks, e->ncount, e->cellxy, e->build, e->key, e->perm, e->perm2, e->inv, e->cell_count,
-                    e->cell_start, e->block_sums, e->dpolar, e->dsys, e->sys_of, e->dpolar_sys, e->dsys1, e->dsamples};
+                    e->cell_start, e->block_sums, e->dpolar, e->dsys, e->sys_of, e->dpolar_sys, e->dsys1, e->dsamples,
+                    e->pstatus};
     for (void* b : bufs) dfree(b);
     for (auto& b : e->slotb) {
         dfree(b.blkbad);
@@ -2735,6 +2927,7 @@ int spp_multi_step(spp_engine* e, int32_t steps)
     if (e->p.noise_mode == SPP_NOISE_INJECTED && (steps > 1 || (steps == 1 && !e->inj_pending)))
         return fail(e, SPP_ERR_STATE, "injected noise covers one step: inject before each step");
     if (steps == 0) return SPP_OK;
+    if (e->persist) return run_persist(e, steps);
     return run_intervals(e, steps);
 }
 
@@ -2915,6 +3108,7 @@ int spp_lyapunov_init(spp_engine* e, int32_t replicas, int32_t vectors, double e
     TRY(launch(e, "lyap_random", lyap::k_random_dev, nblocks(replicas * n), a, seed));
     TRY(lyap_qr(e, true));
     e->list_valid = false;  // clones moved: the next step rebuilds
+    e->persist_valid = false;
     return lyap_check_status(e);
 }
 
@@ -2929,6 +3123,7 @@ int spp_lyapunov_renormalize(spp_engine* e, double* log_growth)
     TRY(launch(e, "lyap_deviation", lyap::k_deviation, nblocks(L.R * L.n), a));
     TRY(lyap_qr(e, false));
     e->list_valid = false;
+    e->persist_valid = false;
     L.renorms++;
     L.t_last = e->t;
     if (log_growth) {
