@@ -6,6 +6,8 @@
   coarse lags, error = sqrt(var / (n // ic))); where the reference script would raise (no lag
   below 0.01) it returns an infinite error.
 Both return (mean, error, correlation length in samples).
+* ``order_parameters`` and ``quantities_time`` restate the trajectory analyses of
+  analyze/analyze.h:71-148 over frames of headings and positions (host numpy).
 """
 from __future__ import annotations
 
@@ -56,3 +58,38 @@ def polarity_error(t, p):
         return ave, math.inf, None
     ic = int(idx[0]) * di
     return ave, math.sqrt(var / (n // ic)), ic
+
+
+def _polar_per_frame(theta):
+    th = np.asarray(theta, dtype=np.float64)
+    return np.hypot(np.cos(th).mean(axis=-1), np.sin(th).mean(axis=-1))
+
+
+def order_parameters(theta_frames, Lx, Ly):
+    """Compute_Order_Parameters (analyze/analyze.h:127-148): for frames of headings (shape
+    [frames, particles]) and the box half-widths, returns (mean polarisation, its autocorrelation
+    error, sigma2 = var(p) 4 Lx Ly, G = 1 - <p^4> / (3 <p^2>)).  G keeps the reference's
+    normalisation: the Binder cumulant would divide by 3 <p^2>^2."""
+    p = _polar_per_frame(theta_frames)
+    mean, err, _ = stat_error(p)
+    mean_square = float(np.dot(p, p) / p.size)
+    var = mean_square - mean * mean
+    p4 = float(np.mean(p ** 4))
+    return mean, err, var * 4.0 * Lx * Ly, 1.0 - p4 / (3.0 * mean_square)
+
+
+def quantities_time(theta_frames, x_frames, y_frames, start=0):
+    """Quantities_Time (analyze/analyze.h:71-100): for every frame from `start` on, the
+    polarisation |<(cos, sin)>|, the nematic order S = |<(cos 2 theta, sin 2 theta)>| and the mean
+    squared displacement from frame `start` (positions as the frames store them: unwrapped for the
+    repulsive and active Brownian models).  Returns three arrays."""
+    th = np.asarray(theta_frames, dtype=np.float64)[start:]
+    c, s_ = np.cos(th), np.sin(th)
+    p = np.hypot(c.mean(axis=1), s_.mean(axis=1))
+    c2 = (2 * c * c - 1).mean(axis=1)
+    s2 = (2 * c * s_).mean(axis=1)
+    S = np.sqrt(c2 * c2 + s2 * s2)
+    x = np.asarray(x_frames, dtype=np.float64)
+    y = np.asarray(y_frames, dtype=np.float64)
+    dr2 = ((x[start:] - x[start]) ** 2 + (y[start:] - y[start]) ** 2).mean(axis=1)
+    return p, S, dr2
