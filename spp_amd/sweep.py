@@ -210,6 +210,62 @@ def train_surrogate(data: Dataset, epochs=400, batch=256, seed=1):
     return s
 
 
+def merge(a: Dataset, b: Dataset) -> Dataset:
+    return Dataset(np.concatenate([a.x, b.x]), np.concatenate([a.y, b.y]), a.seconds + b.seconds)
+
+
+def simulate_points(points, warm=6000, steps=6000, sample_every=20, seed=1, per_batch=64):
+    """Plain simulations of arbitrary (D, rho, v0, n) points, grouped by size into Lyapunov-pair
+    batches as make_dataset runs them: mean and variance of P and the largest exponent."""
+    timer = Timer()
+    xs, ys = [], []
+    by_n = {}
+    for pt in points:
+        by_n.setdefault(int(pt[3]), []).append(tuple(pt))
+    k = 0
+    for n, pts in sorted(by_n.items()):
+        for b in range(0, len(pts), per_batch):
+            chunk = pts[b:b + per_batch]
+            series, lam = run_lyapunov(chunk, warm, steps, seed + 31 * k, sample_every=sample_every, timer=timer)
+            for j, pt in enumerate(chunk):
+                xs.append(pt)
+                ys.append((series[:, j].mean(), series[:, j].var(), lam[j]))
+            k += len(chunk)
+    return Dataset(np.array(xs, dtype=np.float64).reshape(-1, 4), np.array(ys, dtype=np.float64).reshape(-1, 3),
+                   timer.seconds)
+
+
+def active_round(s: Surrogate, data: Dataset, per_round=128, candidates=4096, seed=1, sizes=(256, 512, 1024),
+                 D_range=(0.0, 1.5), rho_range=(0.25, 4.0), v0_range=(0.25, 1.0), band=(0.15, 0.85), warm=6000,
+                 steps=6000, sample_every=20, epochs=400, batch=256):
+    """One round of variance-steered sampling: of `candidates` seeded draws of (D, rho, v0, N),
+    the `per_round` with the highest predicted polarisation variance among those whose predicted
+    mean polarisation lies in `band` (the transition; topped up from outside it by variance when
+    the band holds too few) are simulated plainly and added to the training set, and the network
+    is retrained.  Returns (surrogate, enlarged data set, the picked points, their predicted
+    variances, the candidates' predicted variances)."""
+    rng = np.random.default_rng(seed)
+    cand = np.stack([rng.uniform(*D_range, candidates), rng.uniform(*rho_range, candidates),
+                     rng.uniform(*v0_range, candidates), rng.choice(np.asarray(sizes, dtype=np.float64), candidates)],
+                    axis=1)
+    y = s.predict(cand[:, 0], cand[:, 1], cand[:, 2], cand[:, 3])
+    P, var = y[:, 0], np.exp(y[:, 1])
+    in_band = (P >= band[0]) & (P <= band[1])
+    order = np.lexsort((-var, ~in_band))  # in-band first, each group by decreasing variance
+    pick = order[:per_round]
+    new = simulate_points([tuple(r) for r in cand[pick]], warm=warm, steps=steps, sample_every=sample_every,
+                          seed=seed + 17)
+    data2 = merge(data, new)
+    s2 = train_surrogate(data2, epochs=epochs, batch=batch, seed=seed)
+    return s2, data2, cand[pick], var[pick], var
+
+
+def held_out_error(s: Surrogate, held: Dataset):
+    """Root-mean-square error of the predicted mean polarisation against plain simulations."""
+    pred = s.predict(held.x[:, 0], held.x[:, 1], held.x[:, 2], held.x[:, 3])[:, 0]
+    return float(np.sqrt(np.mean((pred - held.y[:, 0]) ** 2)))
+
+
 def predicted_curve(s: Surrogate, D, rho, v0, n):
     y = s.predict(D, np.full_like(D, rho), np.full_like(D, v0), np.full_like(D, n))
     return y[:, 0], np.exp(y[:, 1]), y[:, 2]

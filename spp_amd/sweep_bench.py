@@ -14,7 +14,10 @@ estimator (polarization-analysis.py:29-62).
              same way by simulation.
 Times are wall-clock seconds of the engine calls (the host only waits for the device between
 them); the steered figure is reported without and with the cost of producing the training set and
-training the network (reusable across curves).
+training the network (reusable across curves).  The network's own prediction is also judged
+against the plain runs' points, which it never saw (`surrogate_rms_error_vs_plain`).
+`--active-rounds` adds rounds of 128 points of highest predicted variance near the transition,
+simulated and added to the training set before the curve is drawn.
 """
 from __future__ import annotations
 
@@ -46,6 +49,8 @@ def main(argv=None):
     ap.add_argument("--max-steps", type=int, default=200000)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--active-rounds", type=int, default=0,
+                    help="variance-steered rounds of 128 simulated points added to the training set")
     a = ap.parse_args(argv)
 
     sizes = tuple(int(s) for s in a.train_sizes.split(","))
@@ -56,6 +61,9 @@ def main(argv=None):
     print(f"dataset: {len(data.y)} points in {t_data:.1f} s", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     s = SW.train_surrogate(data, epochs=a.epochs, batch=256, seed=a.seed)
+    for r in range(a.active_rounds):
+        s, data, _, _, _ = SW.active_round(s, data, per_round=128, seed=a.seed + 101 * (r + 1), sizes=sizes,
+                                           epochs=a.epochs)
     t_train = time.perf_counter() - t0
     print(f"training: {t_train:.1f} s ({s.train_ms:.0f} ms on the device)", file=sys.stderr, flush=True)
     kw = dict(tol=a.tol, chunk=a.chunk, sample_every=20, max_steps=a.max_steps, seed=a.seed + 100)
@@ -94,6 +102,9 @@ def main(argv=None):
         print(f"plain: {D.size} points, {timer_s:.2f} s", file=sys.stderr, flush=True)
         out["plain"] = {"seconds": timer_s, "points": int(D.size), "rounds": rounds, "particle_steps": steps,
                         "max_err": float(E.max())}
+        # the network judged against the held-out plain runs (it never saw these points)
+        pred, _, _ = SW.predicted_curve(s, D, a.rho, a.v0, a.n)
+        out["surrogate_rms_error_vs_plain"] = float(np.sqrt(np.mean((pred - P) ** 2)))
         out["value"] = st.seconds
         out["speedup_vs_plain"] = timer_s / st.seconds
         out["speedup_vs_plain_incl_training"] = timer_s / (st.seconds + t_data + t_train)

@@ -53,3 +53,28 @@ def test_steered_sweep_needs_fewer_points_than_uniform():
     assert st.points_simulated <= un.points_simulated, (st.log, un.log)
     # an ordered-to-disordered curve
     assert st.P[0] > 0.8 and st.P[-1] < 0.2
+
+
+def test_variance_steered_round_and_held_out_error():
+    """One round of variance-steered sampling: the picked points are the candidates of highest
+    predicted variance inside the predicted transition band, every pick is simulated and joins the
+    training set, and the retrained network is judged against held-out plain runs (its error is
+    measured, and stays a fraction of the curve's span)."""
+    from spp_amd import sweep as SW
+
+    data = SW.make_dataset(96, sizes=(256,), warm=3000, steps=3000, sample_every=20, seed=11, per_batch=48,
+                           D_range=(0.0, 1.5), rho_range=(0.5, 2.0), v0_range=(0.5, 0.5))
+    s = SW.train_surrogate(data, epochs=300, batch=48, seed=4)
+    held = SW.simulate_points([(D, 1.0, 0.5, 256) for D in np.linspace(0.1, 1.4, 12)], warm=3000, steps=3000,
+                              seed=99)
+    before = SW.held_out_error(s, held)
+    s2, data2, picks, pv, cv = SW.active_round(s, data, per_round=32, candidates=512, seed=12, sizes=(256,),
+                                               rho_range=(0.5, 2.0), v0_range=(0.5, 0.5), warm=3000, steps=3000,
+                                               epochs=300, batch=48)
+    assert picks.shape == (32, 4) and len(data2.y) == len(data.y) + 32
+    assert np.array_equal(data2.x[-32:], picks)  # every pick was simulated and added
+    assert np.all(np.isfinite(data2.y[-32:]))
+    assert pv.min() >= np.median(cv)  # the most variable candidates were chosen
+    after = SW.held_out_error(s2, held)
+    print("held-out rms error of P: %.3f before, %.3f after the round" % (before, after))
+    assert after < 0.25 and before < 0.25
