@@ -139,9 +139,11 @@ __device__ __forceinline__ double wave_max(double v)
 // (Box::Update_Cells + Update_Neighbor_List, serial/box.h:107-179)
 // ------------------------------------------------------------------------------------------------
 
+// Cell key of every particle and its arrival rank among its cell mates (the atomic's old count),
+// so that the scatter needs no atomics.
 __global__ __launch_bounds__(kBlock) void k_bin(int N, Grid G0, const P4* __restrict__ pos,
                                                 int* __restrict__ key, int* __restrict__ count,
-                                                const SysInfo* __restrict__ sys_tab,
+                                                int* __restrict__ rank, const SysInfo* __restrict__ sys_tab,
                                                 const int* __restrict__ sys_of)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -158,7 +160,7 @@ __global__ __launch_bounds__(kBlock) void k_bin(int N, Grid G0, const P4* __rest
     int cy = cell_coord(p.y, G.Ly, G.Ly2, G.ncy, G.quirk);
     int k = koff + cy * G.ncx + cx;
     key[i] = k;
-    atomicAdd(&count[k], 1);
+    rank[i] = atomicAdd(&count[k], 1);
 }
 
 // Block-wide exclusive scan of kScanTile ints; returns the tile total.
@@ -211,9 +213,9 @@ __global__ __launch_bounds__(kBlock) void k_scan_partial(const int* __restrict__
     }
 }
 
-// out[0..n] = exclusive scan of in[0..n), out[n] = total.  Each block first sums the partial
-// sums of the blocks before it (few: n / 1024).
-__global__ __launch_bounds__(kBlock) void k_scan_final(const int* __restrict__ in, int n,
+// out[0..n] = exclusive scan of in[0..n), out[n] = total; in[] is left zero.  Each block first
+// sums the partial sums of the blocks before it (few: n / 1024).
+__global__ __launch_bounds__(kBlock) void k_scan_final(int* __restrict__ in, int n,
                                                        const int* __restrict__ block_sums,
                                                        int* __restrict__ out)
 {
@@ -234,20 +236,21 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(const int* __restrict__ i
     int total = block_exclusive_scan(v, lds_wave);
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k)
-        if (base + k < n) out[base + k] = v[k] + prefix;
+        if (base + k < n) {
+            out[base + k] = v[k] + prefix;
+            in[base + k] = 0;  // the counts' last reader: zero for the next rebuild
+        }
     if (b == gridDim.x - 1 && threadIdx.x == 0) out[n] = prefix + total;
 }
 
-// Counting-sort scatter.  Counting down returns every cell count to zero, ready for the next
-// rebuild without a clearing pass.
+// Counting-sort scatter to the cell start plus the arrival rank recorded by the binning.
 __global__ __launch_bounds__(kBlock) void k_scatter(int N, const int* __restrict__ key,
                                                     const int* __restrict__ start,
-                                                    int* __restrict__ count, int* __restrict__ perm)
+                                                    const int* __restrict__ rank, int* __restrict__ perm)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    int k = key[i];
-    perm[start[k] + atomicSub(&count[k], 1) - 1] = i;
+    perm[start[key[i]] + rank[i]] = i;
 }
 
 // Order each cell's particles by caller index, so the sorted layout is a pure function of the
@@ -1299,7 +1302,8 @@ __global__ __launch_bounds__(kBlock) void k_pair_hist(int N, const P4* __restric
         if (sh[k]) atomicAdd(&hist[k], (unsigned long long)sh[k]);
 }
 
-// Reduces one interval's per-block partials into the host-mapped report (one block).
+// Reduces one interval's per-block partials into the host-mapped report (one block) and clears
+// the per-block bad steps for the slot's next interval.
 __global__ __launch_bounds__(kBlock) void k_report(int nblk, SlotBufs b, int with_list, Report* out)
 {
     __shared__ int r_bad[kBlock / 64], r_len[kBlock / 64];
@@ -1310,6 +1314,7 @@ __global__ __launch_bounds__(kBlock) void k_report(int nblk, SlotBufs b, int wit
     double mx = 0;
     for (int k = threadIdx.x; k < nblk; k += kBlock) {
         bad = min(bad, b.blkbad[k]);
+        b.blkbad[k] = INT_MAX;  // the slot's next interval starts with no bad step
         mx = fmax(mx, b.blkmax[k]);
         if (with_list) {
             len = max(len, b.blklen[k]);
@@ -1748,14 +1753,15 @@ int launch_report(spp_engine* e, int slot, bool with_list)
 int launch_rebuild(spp_engine* e, int dst, int slot, bool aux)
 {
     int N = e->N, nk = e->nkeys, src = e->cur;
+    // the arrival ranks live in perm2 until the per-cell ordering overwrites it
     TRY(launch(e, "bin", k_bin, nblocks(N), N, e->G, (const P4*)e->set[src].pos, e->key,
-               e->cell_count, (const SysInfo*)e->dsys, (const int*)e->sys_of));
+               e->cell_count, e->perm2, (const SysInfo*)e->dsys, (const int*)e->sys_of));
     TRY(launch(e, "scan_partial", k_scan_partial, e->nscan_blocks, (const int*)e->cell_count, nk,
                e->block_sums));
-    TRY(launch(e, "scan_final", k_scan_final, e->nscan_blocks, (const int*)e->cell_count, nk,
-               (const int*)e->block_sums, e->cell_start));
+    TRY(launch(e, "scan_final", k_scan_final, e->nscan_blocks, e->cell_count, nk, (const int*)e->block_sums,
+               e->cell_start));
     TRY(launch(e, "scatter", k_scatter, nblocks(N), N, (const int*)e->key,
-               (const int*)e->cell_start, e->cell_count, e->perm));
+               (const int*)e->cell_start, (const int*)e->perm2, e->perm));
     TRY(launch(e, "cell_rank", k_cell_rank, nblocks(N), N, (const int*)e->key, (const int*)e->cell_start,
                (const int*)e->perm, (const int*)e->set[src].id, e->perm2));
     if (e->variant == SPP_KERNEL_STRIP_LDS)
@@ -1931,7 +1937,7 @@ int enqueue_interval(spp_engine* e, int m, bool rebuild, int protect, Interval& 
     iv.slot = slot;
     iv.m = m;
     iv.rebuilt = rebuild;
-    TRY(launch(e, "fill", k_fill<int>, nblocks(e->nstat), e->slotb[slot].blkbad, INT_MAX, e->nstat));
+    // slot's per-block bad steps: INT_MAX from creation or from the report that closed its last use
     if (rebuild) {
         int dst = free_set_other_than(e->cur, protect);
         TRY(launch_rebuild(e, dst, slot, false));
@@ -3311,6 +3317,8 @@ int spp_time_stage(spp_engine* e, int32_t stage, int32_t reps, double* avg_us)
     if (rc == SPP_OK) rc = hipEventRecord(t0, e->stream) == hipSuccess ? SPP_OK : SPP_ERR_HIP;
     for (int r = 0; r < reps && rc == SPP_OK; ++r) rc = go();
     if (rc == SPP_OK) rc = hipEventRecord(t1, e->stream) == hipSuccess ? SPP_OK : SPP_ERR_HIP;
+    // the timed launches may have marked slot 0's per-block bad steps: clear them for its next interval
+    if (rc == SPP_OK) rc = launch(e, "fill", k_fill<int>, nblocks(e->nstat), e->slotb[0].blkbad, INT_MAX, e->nstat);
     float ms = 0;
     if (rc == SPP_OK) rc = hipEventSynchronize(t1) == hipSuccess ? SPP_OK : SPP_ERR_HIP;
     if (rc == SPP_OK) rc = hipEventElapsedTime(&ms, t0, t1) == hipSuccess ? SPP_OK : SPP_ERR_HIP;
