@@ -132,7 +132,10 @@ typedef struct spp_params {
                                bead i (no reference counterpart; 0 = none)                          */
     int32_t ell_capacity;   /* neighbour slots per particle on the device (0 = default 32)          */
     int32_t kernel_variant; /* SPP_KERNEL_*: how the interaction kernels gather neighbours          */
-    int32_t reserved_i[6];
+    int32_t graph_intervals;/* 1: with SPP_REBUILD_PERIOD (RK2/RK4, no sampling), capture each full
+                               interval (rebuild + rebuild_period steps) once as a HIP graph per
+                               buffer layout and replay it; 0: launch every kernel                  */
+    int32_t reserved_i[5];
     double reserved_d[11];
 } spp_params;
 
@@ -275,6 +278,9 @@ int spp_restore(spp_engine* e, int32_t slot);
 /* ---- timing -------------------------------------------------------------------------------------- */
 
 int spp_synchronize(spp_engine* e);
+/* Interval graphs (spp_params.graph_intervals): intervals captured, intervals launched from a
+   graph (captures included), graphs held.  No reference counterpart.                            */
+int spp_graph_stats(spp_engine* e, int64_t* captures, int64_t* replays, int32_t* cached);
 /* Per-kernel HIP-event timing on the engine's stream: enable, run work, read averages.
    names: '\n'-separated kernel names written into buf (cap bytes).  ms_avg/launches have room
    for max_kernels entries; *nk receives the number of kernels.                                    */

@@ -69,7 +69,8 @@ class Params(C.Structure):
         ("kappa_bend", C.c_double),
         ("ell_capacity", C.c_int32),
         ("kernel_variant", C.c_int32),
-        ("reserved_i", C.c_int32 * 6),
+        ("graph_intervals", C.c_int32),
+        ("reserved_i", C.c_int32 * 5),
         ("reserved_d", C.c_double * 11),
     ]
 
@@ -154,6 +155,7 @@ SIGNATURES = {
     "spp_snapshot": (C.c_int, [C.c_void_p, C.c_int32]),
     "spp_restore": (C.c_int, [C.c_void_p, C.c_int32]),
     "spp_synchronize": (C.c_int, [C.c_void_p]),
+    "spp_graph_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
     "spp_profile_enable": (C.c_int, [C.c_void_p, C.c_int32]),
     "spp_profile_read": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, _D, _I64, C.c_int32, _I32]),
     "spp_time_stage": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _D]),
@@ -417,6 +419,12 @@ class Engine:
 
     def synchronize(self):
         self._check(self._lib.spp_synchronize(self._h))
+
+    def graph_stats(self):
+        """(intervals captured, intervals launched from a graph, graphs held)."""
+        c, r, k = C.c_int64(), C.c_int64(), C.c_int32()
+        self._check(self._lib.spp_graph_stats(self._h, C.byref(c), C.byref(r), C.byref(k)))
+        return c.value, r.value, k.value
 
     # -- timing --------------------------------------------------------------------------------
     def profile(self, on: bool):

@@ -107,7 +107,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_abp(StageArgs a)
             if (a.id_out) a.id_out[i] = pid;
             th_in = a.th_in[i];
             if (INTEG == SPP_EULER) u = a.U_in[i];
-            dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)pid, a.step);
+            dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)pid, noise_step(a));
         } else {
             thold = a.th0[i];
             dn = a.dth[i];

@@ -576,13 +576,21 @@ struct StageArgs {
     int* id_out;           // first step of an interval: the output set's copy of id
     const double* inj;     // injected noise, slot order (null = stream)
     const float2* build;
-    int64_t step;          // noise-stream step
+    int64_t step;          // noise-stream step (relative to *clock when clock is set)
+    const long long* clock;  // graph replays: the interval's first step, set before each replay
     int kstep;             // step index inside the current interval
     double thresh2;        // displacement trigger, squared
     int* blkbad;           // per-workgroup first bad step
     double* blockmax;      // per-workgroup max displacement^2 (last step of an interval)
     int record;
 };
+
+// The noise stream's step of a stage launch: absolute, or relative to the interval's first step
+// held on the device when the launch belongs to a captured (replayed) interval.
+__device__ __forceinline__ int64_t noise_step(const StageArgs& a)
+{
+    return a.clock ? a.step + *a.clock : a.step;
+}
 
 // The first kPfList rows of a staged chunk's LDS-offset list, requested before the staging loads so
 // that the pair loop does not wait on global memory for them.
@@ -729,7 +737,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int s, const Chun
                 if (a.id_out) a.id_out[i] = pid;
                 th_in = a.th_in[i];
                 if (INTEG == SPP_EULER) u = a.U_in[i];
-                dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)pid, a.step);
+                dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)pid, noise_step(a));
             } else {
                 thold = a.th0[i];
                 dn = a.dth[i];
@@ -1000,7 +1008,7 @@ __device__ __forceinline__ void stage_particle(const StageArgs& a, const Phys& P
         if (INTEG == SPP_EULER) u = a.U_in[i];
         // the stream is addressed by the particle's index within its system, so systems with
         // equal seeds (Lyapunov clones) draw identical noise
-        dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)(pid - pid_off), a.step);
+        dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)(pid - pid_off), noise_step(a));
     } else {
         thold = a.th0[i];
         dn = a.dth[i];
@@ -1592,6 +1600,19 @@ struct spp_engine {
     bool persist_valid = false;  // the device statuses describe the current state and lists
     persist::Status* pstatus = nullptr;
     int64_t persist_rebuild_base = 0;
+    // graph_intervals: captured fixed-period intervals, keyed by what their launches embed
+    struct GraphEntry {
+        int cur, protect, slot, m;
+        uint64_t gen;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
+    uint64_t alloc_gen = 0;      // bumped whenever a buffer an interval launch embeds is replaced
+    bool dry = false;            // bookkeeping only: launch() enqueues nothing (graph replays)
+    bool graph_mode = false;     // launches being captured: stage steps relative to *dclock
+    int64_t graph_base = 0;
+    long long* dclock = nullptr; // the replayed interval's first step
+    int64_t graph_replays = 0, graph_captures = 0;
 
     bool profiling = false;
     std::vector<ProfRec> prof;
@@ -1646,7 +1667,7 @@ hipEvent_t take_event(spp_engine* e)
 template <typename F, typename... Args>
 int launch(spp_engine* e, const char* name, F kernel, int grid, Args... args)
 {
-    if (grid <= 0) return SPP_OK;
+    if (grid <= 0 || e->dry) return SPP_OK;
     ProfRec rec{name, nullptr, nullptr};
     if (e->profiling) {
         rec.a = take_event(e);
@@ -1666,6 +1687,7 @@ int launch(spp_engine* e, const char* name, F kernel, int grid, Args... args)
 template <typename F, typename... Args>
 int launch_chunks(spp_engine* e, const char* name, F kernel, Args... args)
 {
+    if (e->dry) return SPP_OK;
     ProfRec rec{name, nullptr, nullptr};
     if (e->profiling) {
         rec.a = take_event(e);
@@ -1701,8 +1723,16 @@ double max_cutoff(const spp_params& p)
     return c;
 }
 
+void drop_graphs(spp_engine* e)
+{
+    for (auto& g : e->graphs) (void)hipGraphExecDestroy(g.exec);
+    e->graphs.clear();
+}
+
 int alloc_nbr(spp_engine* e)
 {
+    e->alloc_gen++;
+    drop_graphs(e);
     dfree(e->nbr16);
     dfree(e->nbr32);
     e->nbr16 = nullptr;
@@ -1783,6 +1813,8 @@ int launch_rebuild(spp_engine* e, int dst, int slot, bool aux)
     }
     TRY(launch(e, "permute", k_permute, nblocks(N), pa));
     if (aux) {
+        e->alloc_gen++;
+        drop_graphs(e);
         std::swap(e->kacc, e->aux_alt[0]);
         std::swap(e->force, e->aux_alt[1]);
         std::swap(e->old, e->aux_alt[2]);
@@ -1852,7 +1884,8 @@ int launch_step(spp_engine* e, int S, int W, int kstep, int slot, bool record)
     a.ncount = e->ncount;
     a.id = e->set[S].id;
     a.build = e->build;
-    a.step = e->step;
+    a.step = e->graph_mode ? e->step - e->graph_base : e->step;
+    a.clock = e->graph_mode ? e->dclock : nullptr;
     a.kstep = kstep;
     a.thresh2 = e->thresh * e->thresh;
     a.blkbad = e->slotb[slot].blkbad;
@@ -1931,12 +1964,9 @@ int check_ready(spp_engine* e)
 }
 
 // Enqueue one interval: [rebuild] + m steps + flag read-back.  Returns the interval record.
-int enqueue_interval(spp_engine* e, int m, bool rebuild, int protect, Interval& iv)
+// The launches of one interval: [rebuild] + m steps + report.  Host bookkeeping only when e->dry.
+int interval_body(spp_engine* e, int m, bool rebuild, int protect, int slot, Interval& iv)
 {
-    int slot = (int)(e->intervals++ & 1);
-    iv.slot = slot;
-    iv.m = m;
-    iv.rebuilt = rebuild;
     // slot's per-block bad steps: INT_MAX from creation or from the report that closed its last use
     if (rebuild) {
         int dst = free_set_other_than(e->cur, protect);
@@ -1946,12 +1976,75 @@ int enqueue_interval(spp_engine* e, int m, bool rebuild, int protect, Interval& 
     iv.step0 = e->step;
     iv.since0 = e->steps_since_build;
     iv.t0 = e->t;
+    e->graph_base = e->step;
     int W = free_set_other_than(e->cur, protect);
     for (int k = 0; k < m; ++k) {
         TRY(launch_step(e, e->cur, k == 0 ? W : e->cur, k, slot, k == m - 1));
         if (k == 0) e->inj_pending = false;
     }
-    TRY(launch_report(e, slot, rebuild));
+    return launch_report(e, slot, rebuild);
+}
+
+__global__ void k_set_i64(long long* p, long long v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// A full fixed-period interval replayed from a captured HIP graph.  Everything its launches embed
+// is fixed by the key: the state set it starts from and the one kept for rollback (which fix the
+// rebuild's destination and the steps' output set), the report slot, the length and the buffer
+// generation; the noise step comes from the device clock set just before the replay.
+bool graphable(const spp_engine* e, int m, bool rebuild)
+{
+    return e->p.graph_intervals && e->p.rebuild == SPP_REBUILD_PERIOD && rebuild && m == e->p.rebuild_period &&
+           e->p.integrator != SPP_EULER && !e->inj_pending && !e->profiling && e->sample_every == 0;
+}
+
+int enqueue_interval(spp_engine* e, int m, bool rebuild, int protect, Interval& iv)
+{
+    int slot = (int)(e->intervals++ & 1);
+    iv.slot = slot;
+    iv.m = m;
+    iv.rebuilt = rebuild;
+    if (!graphable(e, m, rebuild)) {
+        TRY(interval_body(e, m, rebuild, protect, slot, iv));
+    } else {
+        if (!e->dclock) TRY(dalloc(e, &e->dclock, 1));
+        spp_engine::GraphEntry* g = nullptr;
+        for (auto& x : e->graphs)
+            if (x.cur == e->cur && x.protect == protect && x.slot == slot && x.m == m && x.gen == e->alloc_gen) g = &x;
+        const int cur0 = e->cur;
+        if (!g) {
+            // capture: the launches are recorded, not run; the bookkeeping advances as for a launch
+            hipGraph_t graph = nullptr;
+            HIPC(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
+            e->graph_mode = true;
+            int rc = interval_body(e, m, rebuild, protect, slot, iv);
+            e->graph_mode = false;
+            hipError_t ce = hipStreamEndCapture(e->stream, &graph);
+            if (rc != SPP_OK) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            if (ce != hipSuccess) return fail(e, SPP_ERR_HIP, "interval capture: %s", hipGetErrorString(ce));
+            hipGraphExec_t exec = nullptr;
+            hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) return fail(e, SPP_ERR_HIP, "interval graph: %s", hipGetErrorString(ie));
+            e->graphs.push_back(spp_engine::GraphEntry{cur0, protect, slot, m, e->alloc_gen, exec});
+            g = &e->graphs.back();
+            e->graph_captures++;
+        } else {
+            e->dry = true;
+            int rc = interval_body(e, m, rebuild, protect, slot, iv);
+            e->dry = false;
+            TRY(rc);
+        }
+        hipLaunchKernelGGL(k_set_i64, dim3(1), dim3(64), 0, e->stream, e->dclock, (long long)iv.step0);
+        HIPC(e, hipGetLastError());
+        HIPC(e, hipGraphLaunch(g->exec, e->stream));
+        e->graph_replays++;
+    }
     HIPC(e, hipEventRecord(e->slot_event[slot], e->stream));
     iv.done = e->slot_event[slot];
     return SPP_OK;
@@ -1964,7 +2057,9 @@ int predict_m(spp_engine* e, int remaining)
         m = e->force_m;
         e->force_m = 0;
     } else if (e->p.rebuild == SPP_REBUILD_PERIOD) {
-        m = std::max(1, e->p.rebuild_period - (int)e->steps_since_build);
+        // an interval that starts with a rebuild runs a whole period
+        const bool due = !e->list_valid || e->steps_since_build >= e->p.rebuild_period;
+        m = due ? e->p.rebuild_period : std::max(1, e->p.rebuild_period - (int)e->steps_since_build);
     } else if (e->T_cross <= 0) {
         m = 4;  // no history yet: a short interval to measure the displacement rate
     } else {
@@ -2673,6 +2768,8 @@ void spp_destroy(spp_engine* e)
         dfree(b.blknnz);
     }
     for (auto& kv : e->slots) free_set(kv.second.s);
+    drop_graphs(e);
+    dfree(e->dclock);
     lyap_free(e->ly);
     for (auto& r : e->prof) {
         (void)hipEventDestroy(r.a);
@@ -3216,6 +3313,15 @@ int spp_synchronize(spp_engine* e)
 {
     if (!e) return SPP_ERR_ARG;
     HIPC(e, hipStreamSynchronize(e->stream));
+    return SPP_OK;
+}
+
+int spp_graph_stats(spp_engine* e, int64_t* captures, int64_t* replays, int32_t* cached)
+{
+    if (!e) return SPP_ERR_ARG;
+    if (captures) *captures = e->graph_captures;
+    if (replays) *replays = e->graph_replays;
+    if (cached) *cached = (int32_t)e->graphs.size();
     return SPP_OK;
 }
 
