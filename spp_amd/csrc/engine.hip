@@ -253,25 +253,6 @@ __global__ __launch_bounds__(kBlock) void k_scatter(int N, const int* __restrict
     perm[start[key[i]] + rank[i]] = i;
 }
 
-// Order each cell's particles by caller index, so the sorted layout is a pure function of the
-// positions (the atomic scatter leaves arrival order).
-// Canonical order inside each cell (the scatter's atomics leave it arbitrary): the slot of a
-// particle is its cell's start plus the number of cell mates with a smaller id.  One thread per
-// particle, so a crowded cell costs each of its particles one pass over the cell.
-__global__ __launch_bounds__(kBlock) void k_cell_rank(int N, const int* __restrict__ key, const int* __restrict__ start,
-                                                      const int* __restrict__ perm, const int* __restrict__ id,
-                                                      int* __restrict__ perm_out)
-{
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= N) return;
-    const int me = perm[j], c = key[me], mine = id[me];
-    const int s = start[c], e = start[c + 1];
-    int r = 0;
-#pragma unroll 4
-    for (int q = s; q < e; ++q) r += id[perm[q]] < mine;
-    perm_out[s + r] = me;
-}
-
 struct SetPtrs {
     P4* pos;
     double* th;
@@ -291,21 +272,32 @@ struct PermuteArgs {
     double* d_out;
 };
 
-__global__ __launch_bounds__(kBlock) void k_permute(PermuteArgs a)
+// Canonical order inside each cell (the scatter leaves arrival order) and the permutation in one
+// pass: a particle's slot is its cell's start plus the number of cell mates with a smaller id (one
+// pass over the cell per particle, so a crowded cell costs each of its particles one pass), and its
+// state is copied there.  perm2 keeps the order for the chunk builder.
+__global__ __launch_bounds__(kBlock) void k_rank_permute(PermuteArgs a, const int* __restrict__ key,
+                                                         const int* __restrict__ start, const int* __restrict__ perm,
+                                                         int* __restrict__ perm2)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= a.N) return;
-    int o = a.perm[k];
-    a.out.pos[k] = a.in.pos[o];
-    a.out.th[k] = a.in.th[o];
-    a.out.U[k] = a.in.U[o];
-    int pid = a.in.id[o];
-    a.out.id[k] = pid;
-    a.inv[pid] = k;
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= a.N) return;
+    const int me = perm[j], c = key[me], mine = a.in.id[me];
+    const int s = start[c], e = start[c + 1];
+    int r = 0;
+#pragma unroll 4
+    for (int q = s; q < e; ++q) r += a.in.id[perm[q]] < mine;
+    const int k = s + r;
+    perm2[k] = me;
+    a.out.pos[k] = a.in.pos[me];
+    a.out.th[k] = a.in.th[me];
+    a.out.U[k] = a.in.U[me];
+    a.out.id[k] = mine;
+    a.inv[mine] = k;
     if (a.d_in) {
-        a.d_out[k] = a.d_in[o];
+        a.d_out[k] = a.d_in[me];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) a.a_out[q][k] = a.a_in[q][o];
+        for (int q = 0; q < 3; ++q) a.a_out[q][k] = a.a_in[q][me];
     }
 }
 
@@ -1792,11 +1784,6 @@ int launch_rebuild(spp_engine* e, int dst, int slot, bool aux)
                e->cell_start));
     TRY(launch(e, "scatter", k_scatter, nblocks(N), N, (const int*)e->key,
                (const int*)e->cell_start, (const int*)e->perm2, e->perm));
-    TRY(launch(e, "cell_rank", k_cell_rank, nblocks(N), N, (const int*)e->key, (const int*)e->cell_start,
-               (const int*)e->perm, (const int*)e->set[src].id, e->perm2));
-    if (e->variant == SPP_KERNEL_STRIP_LDS)
-        TRY(launch(e, "chunks", k_chunks, nblocks(e->nchunks), e->N, e->nchunks, e->G.ncx, e->nkeys,
-                   (const int*)e->cell_start, (const int*)e->key, (const int*)e->perm2, e->chunks));
     PermuteArgs pa{};
     pa.N = N;
     pa.perm = e->perm2;
@@ -1811,7 +1798,11 @@ int launch_rebuild(spp_engine* e, int dst, int slot, bool aux)
         pa.d_in = e->dth;
         pa.d_out = e->dth_alt;
     }
-    TRY(launch(e, "permute", k_permute, nblocks(N), pa));
+    TRY(launch(e, "rank_permute", k_rank_permute, nblocks(N), pa, (const int*)e->key, (const int*)e->cell_start,
+               (const int*)e->perm, e->perm2));
+    if (e->variant == SPP_KERNEL_STRIP_LDS)
+        TRY(launch(e, "chunks", k_chunks, nblocks(e->nchunks), e->N, e->nchunks, e->G.ncx, e->nkeys,
+                   (const int*)e->cell_start, (const int*)e->key, (const int*)e->perm2, e->chunks));
     if (aux) {
         e->alloc_gen++;
         drop_graphs(e);

@@ -156,7 +156,7 @@ __device__ int rebuild_system(const Args& a, const SysInfo& S, const SetPtrs& fr
         a.perm[a.start[k] + atomicSub(&a.count[k], 1) - 1] = i;
     }
     __syncthreads();
-    for (int j = off + t; j < end; j += kBlock1024) {  // k_cell_rank
+    for (int j = off + t; j < end; j += kBlock1024) {  // k_rank_permute, ranking
         const int me = a.perm[j], c = a.key[me], mine = from.id[me];
         const int s0 = a.start[c], e0 = a.start[c + 1];
         int r = 0;
@@ -164,7 +164,7 @@ __device__ int rebuild_system(const Args& a, const SysInfo& S, const SetPtrs& fr
         a.perm2[s0 + r] = me;
     }
     __syncthreads();
-    for (int k = off + t; k < end; k += kBlock1024) {  // k_permute
+    for (int k = off + t; k < end; k += kBlock1024) {  // k_rank_permute, copying
         const int o = a.perm2[k];
         to.pos[k] = from.pos[o];
         to.th[k] = from.th[o];
