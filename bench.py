@@ -7,7 +7,8 @@ dt = 0.005 in FP64, Verlet list radius 1.3 rebuilt on displacement.  A step is o
 
 Prints one JSON line: particle-steps/s over the timed steps, the roofline of the dominant kernel
 (algorithmic bytes per launch over its HIP-event-timed average duration on the engine stream) and
-the CPU baseline (the C oracle on one host core, a bounded sample of the same workload).
+the CPU baseline (concurrent copies of the C oracle on the host cores, and one copy alone, each
+on a bounded sample of the same workload).
 """
 from __future__ import annotations
 
@@ -205,7 +206,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(dom, args.variant),
+            "traffic": measured_traffic(dom, args.variant) if args.n == 262144 else None,
             "bytes_per_launch": dom_bytes,
             "avg_launch_us": dom_ms * 1e3,
         },

@@ -2800,6 +2800,20 @@ int spp_set_state(spp_engine* e, const double* xw, const double* yw, const doubl
 {
     if (!e || !xw || !yw || !theta) return e ? fail(e, SPP_ERR_ARG, "null array") : SPP_ERR_ARG;
     if (n != e->N) return fail(e, SPP_ERR_ARG, "n = %d, engine has %d particles", n, e->N);
+    // Wrapped positions must lie in their box [-L, L] (the cell binning clamps, so a particle
+    // outside would be binned into a wrong cell and miss neighbours); no value may be NaN or inf.
+    const int ns = e->dsys ? e->nsys : 1;
+    for (int s = 0; s < ns; ++s) {
+        const int lo = e->dsys ? e->hsys[s].pid_off : 0, hi = e->dsys ? lo + e->hsys[s].n : n;
+        const double Lx = e->dsys ? e->hsys[s].G.Lx : e->G.Lx, Ly = e->dsys ? e->hsys[s].G.Ly : e->G.Ly;
+        for (int i = lo; i < hi; ++i) {
+            if (!(std::fabs(xw[i]) <= Lx) || !(std::fabs(yw[i]) <= Ly))
+                return fail(e, SPP_ERR_ARG, "particle %d at (%g, %g) is outside its box [-%g, %g] x [-%g, %g]", i,
+                            xw[i], yw[i], Lx, Lx, Ly, Ly);
+            if (!std::isfinite(theta[i]) || (xu && !std::isfinite(xu[i])) || (yu && !std::isfinite(yu[i])))
+                return fail(e, SPP_ERR_ARG, "particle %d has a non-finite heading or unwrapped position", i);
+        }
+    }
     std::vector<P4> pos(n);
     std::vector<double2> U(n);
     std::vector<int> id(n);

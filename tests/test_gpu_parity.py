@@ -360,3 +360,23 @@ def test_lds_variant_bitwise_equals_slot(oracle_mod, n, L, squeeze, dt):
         off, idx = e.get_neighbors()  # decodes and cross-checks the LDS-offset list
     for k in ("xw", "yw", "xu", "yu", "theta"):
         assert np.array_equal(out[0][k], out[1][k]), k
+
+
+def test_set_state_rejects_positions_outside_the_box_and_non_finite_values(oracle_mod):
+    """Wrapped positions must lie in [-L, L] and every value must be finite; a rejected state
+    leaves the engine's previous state in place."""
+    p = c1_params(n=256, Lx=8.0, Ly=8.0)
+    x, y, th = oracle_mod.random_state(3, 256, 8.0, 8.0)
+    e = S.Engine(p)
+    e.set_state(x, y, th)
+    before = e.get_state()
+    for k, bad in (("x", 8.5), ("y", -9.0), ("x", np.nan), ("th", np.inf)):
+        xb, yb, tb = x.copy(), y.copy(), th.copy()
+        {"x": xb, "y": yb, "th": tb}[k][17] = bad
+        with pytest.raises(S.SppError):
+            e.set_state(xb, yb, tb)
+    edge = x.copy()
+    edge[5] = 8.0  # the box edge itself is accepted
+    e.set_state(edge, y, th)
+    e.set_state(x, y, th)
+    assert np.array_equal(e.get_state()["xw"], before["xw"])
