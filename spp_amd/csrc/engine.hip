@@ -832,14 +832,17 @@ struct NlistFlatArgs {
 };
 
 // Cell::Neighbor_List (shared/cell.h:74-103) for slot i over the 3x3 periodic stencil of grid G
-// (cell keys from koff); each stencil row is scanned as one or two contiguous slot ranges, four
-// candidates at a time.  Returns the list length before the capacity cut.
+// (cell keys from koff).  Each stencil row is one or two contiguous slot ranges (the columns inside
+// the grid, then the column across the periodic edge), so the stencil is at most six ranges; their
+// bounds are read at once and the candidates are streamed eight at a time across range ends, in
+// range order, so the list keeps the row-by-row candidate order while a wave waits on a few rounds
+// of loads instead of one per range.  Returns the list length before the capacity cut.
 __device__ __forceinline__ int nlist_particle(const NlistFlatArgs& a, int i, const Grid& G, int koff)
 {
-    int c = 0;
-    P4 me = a.pos[i];
-    int cx = cell_coord(me.x, G.Lx, G.Lx2, G.ncx, G.quirk);
-    int cy = cell_coord(me.y, G.Ly, G.Ly2, G.ncy, G.quirk);
+    constexpr int kSeg = 6, kBatch = 8;
+    const double2 me = *reinterpret_cast<const double2*>(&a.pos[i].x);
+    const int cx = cell_coord(me.x, G.Lx, G.Lx2, G.ncx, G.quirk);
+    const int cy = cell_coord(me.y, G.Ly, G.Ly2, G.ncy, G.quirk);
     int x0 = cx - 1, x1 = cx + 1, xw = -1;
     if (x0 < 0) {
         xw = G.ncx - 1;
@@ -848,37 +851,50 @@ __device__ __forceinline__ int nlist_particle(const NlistFlatArgs& a, int i, con
         xw = 0;
         x1 = G.ncx - 1;
     }
-    for (int oy = -1; oy <= 1; ++oy) {
-        int y = cy + oy;
-        y = y < 0 ? y + G.ncy : (y >= G.ncy ? y - G.ncy : y);
-        int row = koff + y * G.ncx;
-        for (int seg = 0; seg < 2; ++seg) {
-            int s, e;
-            if (seg == 0) {
-                s = a.start[row + x0];
-                e = a.start[row + x1 + 1];
-            } else {
-                if (xw < 0) break;
-                s = a.start[row + xw];
-                e = a.start[row + xw + 1];
-            }
-            for (int j = s; j < e; j += 4) {
-                double2 q[4];
+    // range k covers flattened candidates [p[k], p[k+1]); candidate t sits at slot t + off[k]
+    int p[kSeg + 1], off[kSeg];
+    {
+        int s[kSeg], e[kSeg];
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    if (j + u < e) q[u] = *reinterpret_cast<const double2*>(&a.pos[j + u].x);
+        for (int r = 0; r < 3; ++r) {
+            int y = cy + r - 1;
+            y = y < 0 ? y + G.ncy : (y >= G.ncy ? y - G.ncy : y);
+            const int row = koff + y * G.ncx;
+            s[2 * r] = a.start[row + x0];
+            e[2 * r] = a.start[row + x1 + 1];
+            s[2 * r + 1] = xw < 0 ? 0 : a.start[row + xw];
+            e[2 * r + 1] = xw < 0 ? 0 : a.start[row + xw + 1];
+        }
+        p[0] = 0;
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    int jj = j + u;
-                    if (jj < e && jj != i) {
-                        double dx = spp::min_image(me.x - q[u].x, G.Lx2, G.invLx2);
-                        double dy = spp::min_image(me.y - q[u].y, G.Ly2, G.invLy2);
-                        double d2 = dx * dx + dy * dy;
-                        if (d2 < a.T_list) {
-                            if (c < a.cap) a.nbr32[(size_t)c * a.N + i] = jj;
-                            ++c;
-                        }
-                    }
+        for (int k = 0; k < kSeg; ++k) {
+            p[k + 1] = p[k] + (e[k] - s[k]);
+            off[k] = s[k] - p[k];
+        }
+    }
+    const int total = p[kSeg];
+    int c = 0;
+    for (int t0 = 0; t0 < total; t0 += kBatch) {
+        int jj[kBatch];
+        double2 q[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int t = t0 + u;
+            int o = off[0];
+#pragma unroll
+            for (int k = 1; k < kSeg; ++k) o = t >= p[k] ? off[k] : o;
+            jj[u] = t + o;
+            if (t < total) q[u] = *reinterpret_cast<const double2*>(&a.pos[jj[u]].x);
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            if (t0 + u < total && jj[u] != i) {
+                double dx = spp::min_image(me.x - q[u].x, G.Lx2, G.invLx2);
+                double dy = spp::min_image(me.y - q[u].y, G.Ly2, G.invLy2);
+                double d2 = dx * dx + dy * dy;
+                if (d2 < a.T_list) {
+                    if (c < a.cap) a.nbr32[(size_t)c * a.N + i] = jj[u];
+                    ++c;
                 }
             }
         }
