@@ -188,7 +188,8 @@ int spp_get_info(spp_engine* e, spp_info* info);
 /* Particle positions and headings, host -> device.  xw/yw are wrapped positions (reference r),
    xu/yu unwrapped (reference r_original), theta the heading.  Any of xu/yu may be NULL: they are
    then taken equal to xw/yw.  Also rebuilds cells and the neighbour list (Box::Load,
-   parallel/box.h:222-243, which ends in Full_Update_Cells).                                       */
+   parallel/box.h:222-243, which ends in Full_Update_Cells).  SPP_ERR_ARG, with the engine's state
+   unchanged, if a wrapped position lies outside its box [-L, L] or any value is NaN or infinite.  */
 int spp_set_state(spp_engine* e, const double* xw, const double* yw, const double* xu,
                   const double* yu, const double* theta, int32_t n);
 /* Device -> host in original order.  Any pointer may be NULL.                                     */
