@@ -9,11 +9,13 @@ are independent by construction.
 * polarisation of the continuous-time Vicsek model (particle.h:268-338) in its ordered phase;
   doubling D (0.3 -> 0.6) moves the mean from about 0.67 to 0.30, over 20 standard errors;
 * the pair distribution of the repulsive model with the harmonic law at noise D = 0.2 (its
-  first-shell peak is set by the repulsion).
+  first-shell peak is set by the repulsion);
+* the largest Lyapunov exponent of the repulsive model (RK4, Benettin renormalisation every 10
+  steps; parallel/lyapunovbox.h:128-194), engine tangent vectors against the oracle's restatement.
 
-Bounds: |t| < 5 for the polarisation (16 + 16 systems) and < 6 per bin of at least 20 counts for
-g(r) (8 + 8 systems, 13 bins); the same comparisons between two independent oracle ensembles
-give |t| of 0.5-2.
+Bounds: |t| < 5 for the polarisation (16 + 16 systems) and for lambda_1 (8 + 8 replicas), and
+< 6 per bin of at least 20 counts for g(r) (8 + 8 systems, 13 bins); the same comparisons
+between two independent oracle ensembles give |t| of 0.5-2.
 """
 import math
 
@@ -132,3 +134,58 @@ def test_pair_distribution_ensemble_matches_oracle(oracle_mod):
     assert np.all(np.abs(t[full]) < 6)
     # the repulsion empties the core: on average under one pair per snapshot closer than 0.4
     assert gpu.mean(axis=0)[:4].sum() < 1 and cpu.mean(axis=0)[:4].sum() < 1
+
+
+# ---------------------------------------------------------------------------------------------
+# Largest Lyapunov exponent: 8 replicas of 256 repulsive particles, RK4, renormalised every 10
+# steps (Benettin, amplitude 1e-6), over 2000 steps
+# ---------------------------------------------------------------------------------------------
+R_LY, N_LY, L_LY, D_LY, EPS_LY, STEPS_LY, EVERY_LY = 8, 256, 8.0, 0.05, 1e-6, 2000, 10
+
+
+def lyapunov_params(seed):
+    return S.default_params(S.MODEL_REPULSIVE).update(
+        n=N_LY, Lx=L_LY, Ly=L_LY, v0=1.0, g=1.0, D_rot=D_LY, seed=seed, dt=0.005, R_align=1.0,
+        repulsion=S.REP_HARMONIC, A_rep=10.0, sigma_rep=1.0, rc_rep=1.0, integrator=S.RK4, r_list=1.3,
+        rebuild=S.REBUILD_DISPLACEMENT)
+
+
+def test_largest_lyapunov_exponent_ensemble_matches_oracle(oracle_mod):
+    import oracle.lyapunov as OL
+
+    # engine: replicas x (base + one clone) in one batched engine
+    systems = [S.system(N_LY, L_LY, v0=1.0, g=1.0, D_rot=D_LY, seed=500 + r) for r in range(R_LY) for _ in range(2)]
+    xs, ys, ts = [], [], []
+    for r in range(R_LY):
+        x, y, t = oracle_mod.random_state(500 + r, N_LY, L_LY, L_LY)
+        xs += [x, x]
+        ys += [y, y]
+        ts += [t, t]
+    e = S.Engine(lyapunov_params(500), systems)
+    e.set_state(np.concatenate(xs), np.concatenate(ys), np.concatenate(ts))
+    e.lyapunov_init(R_LY, 1, EPS_LY, rescale=True, seed=4)
+    e.lyapunov_run(STEPS_LY, EVERY_LY)
+    sums, T = e.lyapunov_get()
+    gpu = sums[:, 0] / T
+    # oracle: other replicas, the clone started at a random deviation of length 1e-6
+    cpu = []
+    for r in range(R_LY):
+        p = lyapunov_params(900 + r)
+        x, y, th = oracle_mod.random_state(900 + r, N_LY, L_LY, L_LY)
+        base = oracle_mod.Oracle(p)
+        base.set_state(x, y, th)
+        clone = oracle_mod.Oracle(p)
+        d = np.random.default_rng(900 + r).standard_normal((3, N_LY))
+        s = OL.add_deviation(base.get_state(), d * (EPS_LY / np.sqrt((d * d).sum())), L_LY, L_LY)
+        clone.set_state(s["xw"], s["yw"], s["theta"], s["xu"], s["yu"])
+        lo = OL.LyapunovOracle([base, clone], L_LY, L_LY, EPS_LY, True)
+        for _ in range(STEPS_LY // EVERY_LY):
+            lo.multi_step(EVERY_LY)
+            lo.renormalize()
+        cpu.append(lo.logsum[0] / (STEPS_LY * p.dt))
+    t = welch_t(gpu, cpu)
+    print("lambda_1 engine %.3f (sd %.3f), oracle %.3f (sd %.3f), t %.2f" % (gpu.mean(), gpu.std(ddof=1), np.mean(cpu),
+                                                                          np.std(cpu, ddof=1), t))
+    assert abs(T - STEPS_LY * 0.005) < 1e-9
+    assert gpu.mean() > 0.3  # chaotic
+    assert abs(t) < 5
