@@ -146,7 +146,7 @@ __global__ __launch_bounds__(kBlock) void k_bin(int N, Grid G0, const P4* __rest
                                                 int* __restrict__ rank, const SysInfo* __restrict__ sys_tab,
                                                 const int* __restrict__ sys_of)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= N) return;
     Grid G = G0;
     int koff = 0;
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(int N, const int* __restrict
                                                     const int* __restrict__ start,
                                                     const int* __restrict__ rank, int* __restrict__ perm)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= N) return;
     perm[start[key[i]] + rank[i]] = i;
 }
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_rank_permute(PermuteArgs a, const in
                                                          const int* __restrict__ start, const int* __restrict__ perm,
                                                          int* __restrict__ perm2)
 {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = xcd_block() * blockDim.x + threadIdx.x;
     if (j >= a.N) return;
     const int me = perm[j], c = key[me], mine = a.in.id[me];
     const int s = start[c], e = start[c + 1];
@@ -908,7 +908,8 @@ __device__ __forceinline__ int nlist_particle(const NlistFlatArgs& a, int i, con
 __global__ __launch_bounds__(kBlock) void k_nlist_flat(NlistFlatArgs a)
 {
     __shared__ int red_len[kBlock / 64], red_nnz[kBlock / 64];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = xcd_block();
+    int i = b * blockDim.x + threadIdx.x;
     bool live = i < a.N;
     int c = 0;
     if (live) {
@@ -936,8 +937,8 @@ __global__ __launch_bounds__(kBlock) void k_nlist_flat(NlistFlatArgs a)
             mm = max(mm, red_len[k]);
             tt += red_nnz[k];
         }
-        a.blklen[blockIdx.x] = mm;
-        a.blknnz[blockIdx.x] = tt;
+        a.blklen[b] = mm;
+        a.blknnz[b] = tt;
     }
 }
 
