@@ -40,6 +40,9 @@ using spp::Phys;
 namespace {
 
 constexpr int kBlock = 256;
+// below this many particles (two wavefronts per SIMD of the device) the stage kernels keep
+// eight neighbour records in flight per lane instead of four
+constexpr int kSmallN = 131072;
 constexpr int kScanItems = 4;  // items per thread in the cell-count scan
 constexpr int kScanTile = kBlock * kScanItems;
 
@@ -942,7 +945,9 @@ __global__ __launch_bounds__(kBlock) void k_nlist_flat(NlistFlatArgs a)
     }
 }
 
-template <int MODEL, int REP, bool SPLIT>
+// GU: neighbour records in flight per lane (a block of the list loop).  Small systems, whose stage
+// time is one wavefront's chain of loads, take 8; large ones 4 (fewer VGPRs, same speed or better).
+template <int MODEL, int REP, bool SPLIT, int GU = 4>
 __device__ __forceinline__ void gather_forces_flat(const StageArgs& a, const Phys& P, int i, const P4& me,
                                                    double& fx, double& fy, double& tq)
 {
@@ -951,31 +956,31 @@ __device__ __forceinline__ void gather_forces_flat(const StageArgs& a, const Phy
     double s1 = 0, s2 = 0;
     fx = me.c * P.v0;  // Reset: f = v (speed v0), particle.h:559-564
     fy = me.s * P.v0;
-    for (int k = 0; k < cnt; k += 4) {
-        int j[4];
+    for (int k = 0; k < cnt; k += GU) {
+        int j[GU];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < GU; ++u)
             if (k + u < cnt) j[u] = nb[(size_t)(k + u) * a.N];
         if (!SPLIT) {
-            P4 q[4];
+            P4 q[GU];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < GU; ++u)
                 if (k + u < cnt) q[u] = a.gather[j[u]];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < GU; ++u)
                 if (k + u < cnt) spp::pair_accumulate<MODEL, REP>(P, me, q[u], fx, fy, s1, s2);
         } else {
-            P4 q[4];
+            P4 q[GU];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < GU; ++u)
                 if (k + u < cnt) {
                     double2 xy = *reinterpret_cast<const double2*>(&a.gather[j[u]].x);
                     q[u].x = xy.x;
                     q[u].y = xy.y;
                 }
-            bool in[4];
+            bool in[GU];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < GU; ++u) {
                 in[u] = false;
                 if (k + u < cnt) {
                     double dx = spp::min_image(me.x - q[u].x, P.Lx2, P.invLx2);
@@ -985,14 +990,14 @@ __device__ __forceinline__ void gather_forces_flat(const StageArgs& a, const Phy
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < GU; ++u)
                 if (in[u]) {
                     double2 cs = *reinterpret_cast<const double2*>(&a.gather[j[u]].c);
                     q[u].c = cs.x;
                     q[u].s = cs.y;
                 }
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < GU; ++u)
                 if (in[u]) spp::pair_accumulate<MODEL, REP>(P, me, q[u], fx, fy, s1, s2);
         }
     }
@@ -1002,7 +1007,7 @@ __device__ __forceinline__ void gather_forces_flat(const StageArgs& a, const Phy
 // One particle's fused stage (interaction at the stage input + the move), for slot i of a system
 // whose constants P already hold and whose particle indices start at pid_off.  Returns the wrapped
 // output position for the displacement check.
-template <int MODEL, int REP, int INTEG, int STAGE, bool SPLIT>
+template <int MODEL, int REP, int INTEG, int STAGE, bool SPLIT, int GU = 4>
 __device__ __forceinline__ void stage_particle(const StageArgs& a, const Phys& P, int i, int pid_off, double& x,
                                                double& y)
 {
@@ -1025,7 +1030,7 @@ __device__ __forceinline__ void stage_particle(const StageArgs& a, const Phys& P
         if (INTEG == SPP_RK4) kk = a.kacc[i];
     }
     double fx, fy, tq;
-    gather_forces_flat<MODEL, REP, SPLIT>(a, P, i, me, fx, fy, tq);
+    gather_forces_flat<MODEL, REP, SPLIT, GU>(a, P, i, me, fx, fy, tq);
     double th;
     if (STAGE <= 1) {
         if (INTEG == SPP_EULER) {
@@ -1071,7 +1076,7 @@ __device__ __forceinline__ void stage_particle(const StageArgs& a, const Phys& P
     a.th_out[i] = th;
 }
 
-template <int MODEL, int REP, int INTEG, int STAGE, bool SPLIT, bool BATCH>
+template <int MODEL, int REP, int INTEG, int STAGE, bool SPLIT, bool BATCH, int GU = 4>
 __global__ __launch_bounds__(kBlock) void k_stage_flat(StageArgs a)
 {
     int b = xcd_block();
@@ -1086,7 +1091,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_flat(StageArgs a)
     }
     constexpr bool final_stage = (INTEG == SPP_EULER) || (INTEG == SPP_RK2 && STAGE == 2) || (STAGE == 4);
     double x = 0, y = 0, dmax = 0;
-    if (live) stage_particle<MODEL, REP, INTEG, STAGE, SPLIT>(a, P, i, pid_off, x, y);
+    if (live) stage_particle<MODEL, REP, INTEG, STAGE, SPLIT, GU>(a, P, i, pid_off, x, y);
     check_disp(a, P, b, live, i, x, y, final_stage ? a.kstep + 1 : a.kstep, dmax);
     finish_disp(a, b, dmax);
 }
@@ -1382,25 +1387,27 @@ constexpr int kVariantBatch = -1;  // internal: slot-parallel kernels with per-s
 using InteractFn = void (*)(StageArgs, P4*);
 
 template <int MODEL, int REP, int I, int ST>
-StageFn variant_fn(int variant)
+StageFn variant_fn(int variant, bool small)
 {
     if (variant == SPP_KERNEL_STRIP_LDS) return k_stage<MODEL, REP, I, ST>;
     if (variant == SPP_KERNEL_SLOT_SPLIT) return k_stage_flat<MODEL, REP, I, ST, true, false>;
-    if (variant == kVariantBatch) return k_stage_flat<MODEL, REP, I, ST, false, true>;
-    return k_stage_flat<MODEL, REP, I, ST, false, false>;
+    if (variant == kVariantBatch)
+        return small ? k_stage_flat<MODEL, REP, I, ST, false, true, 8> : k_stage_flat<MODEL, REP, I, ST, false, true, 4>;
+    return small ? k_stage_flat<MODEL, REP, I, ST, false, false, 8> : k_stage_flat<MODEL, REP, I, ST, false, false, 4>;
 }
 
 template <int MODEL, int REP>
-StageFn stage_fn(int integ, int stage, int variant)
+StageFn stage_fn(int integ, int stage, int variant, bool small)
 {
-    if (integ == SPP_EULER) return variant_fn<MODEL, REP, SPP_EULER, 0>(variant);
+    if (integ == SPP_EULER) return variant_fn<MODEL, REP, SPP_EULER, 0>(variant, small);
     if (integ == SPP_RK2)
-        return stage == 1 ? variant_fn<MODEL, REP, SPP_RK2, 1>(variant) : variant_fn<MODEL, REP, SPP_RK2, 2>(variant);
+        return stage == 1 ? variant_fn<MODEL, REP, SPP_RK2, 1>(variant, small)
+                          : variant_fn<MODEL, REP, SPP_RK2, 2>(variant, small);
     switch (stage) {
-    case 1: return variant_fn<MODEL, REP, SPP_RK4, 1>(variant);
-    case 2: return variant_fn<MODEL, REP, SPP_RK4, 2>(variant);
-    case 3: return variant_fn<MODEL, REP, SPP_RK4, 3>(variant);
-    default: return variant_fn<MODEL, REP, SPP_RK4, 4>(variant);
+    case 1: return variant_fn<MODEL, REP, SPP_RK4, 1>(variant, small);
+    case 2: return variant_fn<MODEL, REP, SPP_RK4, 2>(variant, small);
+    case 3: return variant_fn<MODEL, REP, SPP_RK4, 3>(variant, small);
+    default: return variant_fn<MODEL, REP, SPP_RK4, 4>(variant, small);
     }
 }
 
@@ -1411,7 +1418,7 @@ StageFn abp_stage_fn(int integ, int stage)
     return stage <= 1 ? k_stage_abp<REP, SPP_RK2, 1> : k_stage_abp<REP, SPP_RK2, 2>;
 }
 
-StageFn pick_stage(int model, int rep, int integ, int stage, int variant)
+StageFn pick_stage(int model, int rep, int integ, int stage, int variant, bool small)
 {
     if (model == SPP_MODEL_ABP) {
         switch (rep) {
@@ -1423,14 +1430,14 @@ StageFn pick_stage(int model, int rep, int integ, int stage, int variant)
         default: return abp_stage_fn<SPP_REP_YUKAWA>(integ, stage);
         }
     }
-    if (model == SPP_MODEL_CONTINUOUS) return stage_fn<SPP_MODEL_CONTINUOUS, SPP_REP_NONE>(integ, stage, variant);
+    if (model == SPP_MODEL_CONTINUOUS) return stage_fn<SPP_MODEL_CONTINUOUS, SPP_REP_NONE>(integ, stage, variant, small);
     switch (rep) {
-    case SPP_REP_NONE: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_NONE>(integ, stage, variant);
-    case SPP_REP_R12_TRUNCATED: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_R12_TRUNCATED>(integ, stage, variant);
-    case SPP_REP_HARMONIC: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_HARMONIC>(integ, stage, variant);
-    case SPP_REP_R12: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_R12>(integ, stage, variant);
-    case SPP_REP_YUKAWA_TRUNCATED: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_YUKAWA_TRUNCATED>(integ, stage, variant);
-    default: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_YUKAWA>(integ, stage, variant);
+    case SPP_REP_NONE: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_NONE>(integ, stage, variant, small);
+    case SPP_REP_R12_TRUNCATED: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_R12_TRUNCATED>(integ, stage, variant, small);
+    case SPP_REP_HARMONIC: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_HARMONIC>(integ, stage, variant, small);
+    case SPP_REP_R12: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_R12>(integ, stage, variant, small);
+    case SPP_REP_YUKAWA_TRUNCATED: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_YUKAWA_TRUNCATED>(integ, stage, variant, small);
+    default: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_YUKAWA>(integ, stage, variant, small);
     }
 }
 
@@ -1931,7 +1938,7 @@ int launch_step(spp_engine* e, int S, int W, int kstep, int slot, bool record)
             a.out = o[s];
             a.th_out = ot[s];
         }
-        StageFn fn = pick_stage(e->p.model, e->p.repulsion, integ, stage, e->variant);
+        StageFn fn = pick_stage(e->p.model, e->p.repulsion, integ, stage, e->variant, e->N < kSmallN);
         if (e->variant == SPP_KERNEL_STRIP_LDS)
             TRY(launch_chunks(e, stage_name(integ, stage), fn, a));
         else
@@ -3431,7 +3438,7 @@ int spp_time_stage(spp_engine* e, int32_t stage, int32_t reps, double* avg_us)
     a.U_out = reinterpret_cast<double2*>(e->aux_alt[0]);
     a.dth = stage <= 1 ? e->dth_alt : e->dth;
     a.kacc = e->aux_alt[1];
-    StageFn fn = pick_stage(e->p.model, e->p.repulsion, integ, stage, e->variant);
+    StageFn fn = pick_stage(e->p.model, e->p.repulsion, integ, stage, e->variant, e->N < kSmallN);
     hipEvent_t t0, t1;
     HIPC(e, hipEventCreate(&t0));
     HIPC(e, hipEventCreate(&t1));
