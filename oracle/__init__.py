@@ -156,7 +156,14 @@ class Oracle:
             pass
 
     def set_state(self, xw, yw, theta, xu=None, yu=None):
-        f = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        def f(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            if a.shape != (self.n,):
+                raise ValueError(f"array of shape {a.shape} for {self.n} particles")
+            return a
+
         xw, yw, theta, xu, yu = f(xw), f(yw), f(theta), f(xu), f(yu)
         self._check(self._lib.orc_set_state(self._h, _p(xw), _p(yw), _p(xu), _p(yu), _p(theta), self.n))
 
@@ -191,6 +198,8 @@ class Oracle:
 
     def set_noise(self, dtheta):
         d = np.ascontiguousarray(dtheta, dtype=np.float64)
+        if d.shape != (self.n,):
+            raise ValueError(f"array of shape {d.shape} for {self.n} particles")
         self._check(self._lib.orc_set_noise(self._h, _p(d), self.n))
 
     def move_stage(self, stage):

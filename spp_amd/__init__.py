@@ -295,9 +295,17 @@ class Engine:
         self.close()
 
     # -- state ---------------------------------------------------------------------------------
+    def _host_array(self, a, name):
+        """A contiguous float64 copy of `a` holding exactly one value per particle (the C side reads
+        n values through the pointer)."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != (self.n,):
+            raise ValueError(f"{name} has shape {a.shape}; the engine holds {self.n} particles")
+        return a
+
     def set_state(self, xw, yw, theta, xu=None, yu=None):
-        f = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
-        xw, yw, theta, xu, yu = f(xw), f(yw), f(theta), f(xu), f(yu)
+        f = lambda a, name: None if a is None else self._host_array(a, name)
+        xw, yw, theta, xu, yu = f(xw, "xw"), f(yw, "yw"), f(theta, "theta"), f(xu, "xu"), f(yu, "yu")
         self._check(self._lib.spp_set_state(self._h, _ptr(xw), _ptr(yw), _ptr(xu), _ptr(yu), _ptr(theta), self.n))
 
     def get_state(self):
@@ -340,7 +348,7 @@ class Engine:
         return a
 
     def inject_noise(self, dtheta):
-        d = np.ascontiguousarray(dtheta, dtype=np.float64)
+        d = self._host_array(dtheta, "dtheta")
         self._check(self._lib.spp_inject_noise(self._h, _ptr(d), self.n))
 
     def move_stage(self, stage: int):

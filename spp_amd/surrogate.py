@@ -62,10 +62,14 @@ def lib():
     return _bound
 
 
-def _f(a, cols=None):
+def _f(a, shape=None):
+    """A contiguous float32 copy and its pointer; `shape` (None entries free) is checked, since
+    the C side reads as many values as the shape it is told implies."""
     if a is None:
         return None, None
     a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None and (a.ndim != len(shape) or any(w is not None and w != d for w, d in zip(shape, a.shape))):
+        raise ValueError(f"array of shape {a.shape}, expected {tuple('*' if w is None else w for w in shape)}")
     return a, a.ctypes.data_as(_F)
 
 
@@ -113,36 +117,36 @@ class Mlp:
         return p
 
     def set_params(self, p):
-        p, pp = _f(p)
+        p, pp = _f(p, (self.n_params,))
         self._check(self._lib.spp_mlp_set_params(self._h, pp, p.size))
 
     def forward(self, x):
-        x, xp = _f(x)
+        x, xp = _f(np.atleast_2d(x), (None, self.dims[0]))  # one point may come as a 1-D row
         y = np.empty((x.shape[0], self.dims[-1]), dtype=np.float32)
         self._check(self._lib.spp_mlp_forward(self._h, xp, x.shape[0], y.ctypes.data_as(_F)))
         return y
 
     def loss_grad(self, x, t, w=None):
-        x, xp = _f(x)
-        t, tp = _f(t)
-        w, wp = _f(w)
+        x, xp = _f(x, (None, self.dims[0]))
+        t, tp = _f(t, (x.shape[0], self.dims[-1]))
+        w, wp = _f(w, (self.dims[-1],))
         loss = C.c_float()
         g = np.empty(self.n_params, dtype=np.float32)
         self._check(self._lib.spp_mlp_loss_grad(self._h, xp, tp, wp, x.shape[0], C.byref(loss), g.ctypes.data_as(_F)))
         return loss.value, g
 
     def train_step(self, x, t, w=None):
-        x, xp = _f(x)
-        t, tp = _f(t)
-        w, wp = _f(w)
+        x, xp = _f(x, (None, self.dims[0]))
+        t, tp = _f(t, (x.shape[0], self.dims[-1]))
+        w, wp = _f(w, (self.dims[-1],))
         loss = C.c_float()
         self._check(self._lib.spp_mlp_train_step(self._h, xp, tp, wp, x.shape[0], C.byref(loss)))
         return loss.value
 
     def train(self, x, t, epochs, batch=256, w=None, seed=1):
-        x, xp = _f(x)
-        t, tp = _f(t)
-        w, wp = _f(w)
+        x, xp = _f(x, (None, self.dims[0]))
+        t, tp = _f(t, (x.shape[0], self.dims[-1]))
+        w, wp = _f(w, (self.dims[-1],))
         loss = C.c_float()
         self._check(self._lib.spp_mlp_train(self._h, xp, tp, wp, x.shape[0], epochs, batch, seed, C.byref(loss)))
         return loss.value

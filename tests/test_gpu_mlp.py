@@ -111,3 +111,21 @@ def test_training_fits_a_smooth_target():
     y = net.forward(x)
     assert np.sqrt(np.mean((y[:, 0] - t[:, 0]) ** 2)) < 0.05
     assert net.last_train_ms() > 0
+
+
+def test_binding_checks_shapes():
+    """The C side reads as many values as the batch and the layer widths imply, so the binding
+    refuses arrays of any other shape; a single point may come as one 1-D row."""
+    net = make(n_in=4, hidden=32, n_hidden=2, n_out=3, seed=2)
+    x = np.zeros((10, 4), dtype=np.float32)
+    t = np.zeros((10, 3), dtype=np.float32)
+    for bad in (np.zeros((10, 3)), np.zeros(40)):
+        with pytest.raises(ValueError):
+            net.forward(bad) if bad.ndim == 2 else net.loss_grad(bad, t)
+    with pytest.raises(ValueError):
+        net.loss_grad(x, t[:9])
+    with pytest.raises(ValueError):
+        net.train_step(x, t, w=np.ones(4))
+    with pytest.raises(ValueError):
+        net.set_params(np.zeros(net.n_params - 1))
+    assert net.forward(x[0]).shape == (1, 3)

@@ -375,6 +375,8 @@ def test_set_state_rejects_positions_outside_the_box_and_non_finite_values(oracl
         {"x": xb, "y": yb, "th": tb}[k][17] = bad
         with pytest.raises(S.SppError):
             e.set_state(xb, yb, tb)
+    with pytest.raises(ValueError):  # the binding checks lengths before the C side reads n values
+        e.set_state(x[:-1], y, th)
     edge = x.copy()
     edge[5] = 8.0  # the box edge itself is accepted
     e.set_state(edge, y, th)
