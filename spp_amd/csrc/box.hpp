@@ -50,6 +50,8 @@ public:
     // Random_Formation / Triangle_Lattice_Formation, shared/set-up.h); unwrapped = wrapped.
     void Init(const std::vector<double>& x, const std::vector<double>& y, const std::vector<double>& theta)
     {
+        if ((int)x.size() != N || (int)y.size() != N || (int)theta.size() != N)
+            throw Error(SPP_ERR_ARG, "Box::Init needs one x, y and theta per particle");
         check(spp_set_state(e_, x.data(), y.data(), nullptr, nullptr, theta.data(), N));
         set_time_origin(0.0);
     }
