@@ -64,26 +64,23 @@ public:
         std::ifstream is(name, std::ios::binary);
         if (!is) return false;
         auto frames = spp_traj::read_healthy_frames(is, writes_theta());
-        if (frames.empty()) return false;
-        const spp_traj::Frame& f = frames.back();
-        if (f.Ns() + f.Nm() != N) return false;
-        std::vector<double> xu(N), yu(N), xw(N), yw(N), th(N, 0.0);
-        for (int i = 0; i < f.Nm(); ++i) {
-            xu[i] = f.mx[i];
-            yu[i] = f.my[i];
-        }
-        for (int i = 0; i < f.Ns(); ++i) {
-            xu[f.Nm() + i] = f.x[i];
-            yu[f.Nm() + i] = f.y[i];
-            th[f.Nm() + i] = f.theta.empty() ? 0.0 : f.theta[i];
-        }
-        for (int i = 0; i < N; ++i) {  // r = r_original; r.Periodic_Transform() (:245-258)
-            xw[i] = xu[i] - 2 * params_.Lx * (double)(int)std::floor(xu[i] / (2 * params_.Lx) + 0.5);
-            yw[i] = yu[i] - 2 * params_.Ly * (double)(int)std::floor(yu[i] / (2 * params_.Ly) + 0.5);
-        }
-        check(spp_set_state(e_, xw.data(), yw.data(), xu.data(), yu.data(), th.data(), N));
-        set_time_origin(f.t);
+        if (frames.empty() || frames.back().Ns() + frames.back().Nm() != N) return false;
+        load_frame(frames.back());
         return true;
+    }
+
+    // operator>>(istream&, Box*), parallel/box.h:505-524: reads the next frame and loads its time,
+    // positions and headings.  A frame that cannot be read whole, or holds another particle count,
+    // sets the stream's failbit and leaves the box as it was.
+    friend std::istream& operator>>(std::istream& is, Box* box)
+    {
+        spp_traj::Frame f;
+        if (!spp_traj::read_frame(is, f, box->writes_theta()) || f.Ns() + f.Nm() != box->N) {
+            is.setstate(std::ios::failbit);
+            return is;
+        }
+        box->load_frame(f);
+        return is;
     }
 
     void Update_Cells() { check(spp_build_neighbors(e_)); }        // serial/box.h:107-134
@@ -212,6 +209,29 @@ private:
         spp_info inf;
         if (spp_get_info(e_, &inf) == SPP_OK) t = t0_ + (inf.t - base_);
     }
+    // A frame's positions (r_original for ring beads and, except for continuous particles, for
+    // swimmers; r for continuous particles) become both positions, wrapped as the reference's
+    // r = r_original; r.Periodic_Transform() (serial/box.h:245-258); headings default to 0.
+    void load_frame(const spp_traj::Frame& f)
+    {
+        std::vector<double> xu(N), yu(N), xw(N), yw(N), th(N, 0.0);
+        for (int i = 0; i < f.Nm(); ++i) {
+            xu[i] = f.mx[i];
+            yu[i] = f.my[i];
+        }
+        for (int i = 0; i < f.Ns(); ++i) {
+            xu[f.Nm() + i] = f.x[i];
+            yu[f.Nm() + i] = f.y[i];
+            th[f.Nm() + i] = f.theta.empty() ? 0.0 : f.theta[i];
+        }
+        for (int i = 0; i < N; ++i) {
+            xw[i] = xu[i] - 2 * params_.Lx * (double)(int)std::floor(xu[i] / (2 * params_.Lx) + 0.5);
+            yw[i] = yu[i] - 2 * params_.Ly * (double)(int)std::floor(yu[i] / (2 * params_.Ly) + 0.5);
+        }
+        check(spp_set_state(e_, xw.data(), yw.data(), xu.data(), yu.data(), th.data(), N));
+        set_time_origin(f.t);
+    }
+
     void check(int rc, spp_engine* e) const
     {
         if (rc != SPP_OK) throw Error(rc, spp_last_error(e));

@@ -1,9 +1,12 @@
 // box_demo.cpp -- the reference's data-gathering loop (parallel/repulsive-vicsek.cpp:52-105) on the
 // GPU engine through the C++ Box facade: Multi_Step(cell_update_period) chunks, the polarization
 // time series and a binary trajectory frame every saving_period chunks, with resume from an
-// existing trajectory (Positioning_Particles, :155-180).
+// existing trajectory (Positioning_Particles, :155-180); at the end the trajectory is read back
+// frame by frame (operator>>).
 //
 // usage: box_demo OUT.bin POLAR.dat N L steps period saving_period [resume]
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -59,6 +62,22 @@ int main(int argc, char** argv)
             if ((s / period + 1) % saving == 0) out << &box;
         }
         std::printf("box_demo: t=%.6f polarization=%.6f\n", box.t, box.polarization);
+        // read the whole trajectory back frame by frame (operator>>) into a second box; the last
+        // frame must reproduce the final state to float32 rounding
+        out.close();
+        std::ifstream in(out_name, std::ios::binary);
+        spp_gpu::Box back(p);
+        int frames = 0;
+        while (in >> &back) ++frames;
+        std::vector<double> xw, yw, xu, yu, th, bxw, byw, bxu, byu, bth;
+        box.Get_State(xw, yw, xu, yu, th);
+        back.Get_State(bxw, byw, bxu, byu, bth);
+        double err = 0;
+        for (int i = 0; i < N; ++i) {
+            err = std::max(err, std::fabs(bxu[i] - (double)(float)xu[i]));
+            err = std::max(err, std::fabs(bth[i] - (double)(float)th[i]));
+        }
+        std::printf("box_demo: read back %d frames, last t=%.6f, max error %.3g\n", frames, back.t, err);
     } catch (const spp_gpu::Error& err) {
         std::fprintf(stderr, "box_demo: error %d: %s\n", err.code, err.what());
         return 1;

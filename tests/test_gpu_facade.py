@@ -2,6 +2,7 @@
 data-gathering loop, its binary trajectory and resume from the last healthy frame."""
 import math
 import os
+import re
 import struct
 import subprocess
 
@@ -43,7 +44,10 @@ def test_box_demo_trajectory_and_resume(tmp_path):
     assert os.path.exists(DEMO), "build() compiles spp_amd/box_demo"
     n, L = 4096, 32.0
     traj, pol = str(tmp_path / "traj.bin"), str(tmp_path / "pol.dat")
-    subprocess.run([DEMO, traj, pol, str(n), str(L), "200", "20", "2"], check=True)
+    run = subprocess.run([DEMO, traj, pol, str(n), str(L), "200", "20", "2"], check=True, capture_output=True, text=True)
+    # operator>> reads every frame back; the last one holds the final state to float32 rounding
+    m = re.search(r"read back (\d+) frames, last t=([0-9.]+), max error (\S+)", run.stdout)
+    assert m and int(m.group(1)) == 6 and abs(float(m.group(2)) - 1.0) < 1e-9 and float(m.group(3)) < 1e-5, run.stdout
     fr = read_frames(traj)
     # initial frame + one every 2 chunks of 20 steps
     assert [round(f[0], 9) for f in fr] == [0.0, 0.2, 0.4, 0.6, 0.8, 1.0]
