@@ -142,28 +142,47 @@ __device__ __forceinline__ double wave_max(double v)
 // (Box::Update_Cells + Update_Neighbor_List, serial/box.h:107-179)
 // ------------------------------------------------------------------------------------------------
 
-// Cell key of every particle and its arrival rank among its cell mates (the atomic's old count),
-// so that the scatter needs no atomics.
+// Cell key of every particle and its arrival rank among its cell mates, so that the scatter needs
+// no atomics.  Slots arrive sorted by the previous cells and particles rarely change cell between
+// rebuilds, so equal keys mostly form runs of neighbouring lanes: the first lane of a run reserves
+// the whole run with one atomic and its lanes take consecutive ranks (equal keys in separate runs
+// reserve separately; any arrival order is fine, the id order inside cells is restored later).
 __global__ __launch_bounds__(kBlock) void k_bin(int N, Grid G0, const P4* __restrict__ pos,
                                                 int* __restrict__ key, int* __restrict__ count,
                                                 int* __restrict__ rank, const SysInfo* __restrict__ sys_tab,
                                                 const int* __restrict__ sys_of)
 {
-    int i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    Grid G = G0;
-    int koff = 0;
-    if (sys_of) {
-        const SysInfo& S = sys_tab[sys_of[i]];
-        G = S.G;
-        koff = S.key_off;
+    const int i = xcd_block() * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool live = i < N;  // every lane takes part in the wave's vote and shuffles
+    int k = -1;
+    if (live) {
+        Grid G = G0;
+        int koff = 0;
+        if (sys_of) {
+            const SysInfo& S = sys_tab[sys_of[i]];
+            G = S.G;
+            koff = S.key_off;
+        }
+        P4 p = pos[i];
+        int cx = cell_coord(p.x, G.Lx, G.Lx2, G.ncx, G.quirk);
+        int cy = cell_coord(p.y, G.Ly, G.Ly2, G.ncy, G.quirk);
+        k = koff + cy * G.ncx + cx;
     }
-    P4 p = pos[i];
-    int cx = cell_coord(p.x, G.Lx, G.Lx2, G.ncx, G.quirk);
-    int cy = cell_coord(p.y, G.Ly, G.Ly2, G.ncy, G.quirk);
-    int k = koff + cy * G.ncx + cx;
-    key[i] = k;
-    rank[i] = atomicAdd(&count[k], 1);
+    const int prev = __shfl_up(k, 1, 64);
+    const bool head = !live || lane == 0 || prev != k;  // dead lanes end runs and reserve nothing
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long above = lane == 63 ? 0ull : heads & (~0ull << (lane + 1));
+    const int next = above ? __builtin_ctzll(above) : 64;  // the next run's first lane
+    int base = 0;
+    if (live && head) base = atomicAdd(&count[k], next - lane);
+    const unsigned long long upto = lane == 63 ? heads : heads & ((2ull << lane) - 1);
+    const int first = 63 - __builtin_clzll(upto);  // lane 0 is always a head, so upto != 0
+    base = __shfl(base, first, 64);
+    if (live) {
+        key[i] = k;
+        rank[i] = base + (lane - first);
+    }
 }
 
 // Block-wide exclusive scan of kScanTile ints; returns the tile total.
