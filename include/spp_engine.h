@@ -90,9 +90,12 @@ enum {
     SPP_KERNEL_SLOT = 0,       /* one lane per particle, neighbour records gathered from L2         */
     SPP_KERNEL_SLOT_SPLIT = 1, /* as SLOT, heading half of a record fetched only inside the cutoff  */
     SPP_KERNEL_STRIP_LDS = 2,  /* one workgroup per cell strip, 3-row neighbourhood staged in LDS  */
-    SPP_KERNEL_PERSIST = 3     /* one workgroup per system (<= 4096 particles, Vicsek models, noise
+    SPP_KERNEL_PERSIST = 3,    /* one workgroup per system (<= 4096 particles, Vicsek models, noise
                                   stream) runs a whole spp_multi_step in one launch: rebuilds when a
                                   system's own displacement (or period) requires; single or batched */
+    SPP_KERNEL_SLOT_DEEP = 4   /* as SLOT with eight neighbour records in flight per lane at every
+                                  size (SLOT keeps eight below 131,072 particles, four above); same
+                                  results bit for bit; single or batched, Vicsek models            */
 };
 
 typedef struct spp_params {

@@ -11,7 +11,8 @@
                density U[0.25, 4], D U[0, 0.5], v0 = 0.5, RK2 dt = 0.01, run in calls of 100 steps;
 * lyapunov64 -- 64 Lyapunov replicas of a repulsive Vicsek box (N = 16,384, density 1, the harmonic
                repulsion of the N = 262,144 workload; triangular-lattice start, random headings), each
-               with 4 perturbed clones, RK4 dt = 0.005, tangent vectors renormalised every 10 steps.
+               with 4 perturbed clones, RK4 dt = 0.005, tangent vectors renormalised every 10 steps;
+               the autotuner first picks the slot kernel's depth (4 or 8 records in flight).
 
 Each line gives particle-steps/s over the timed steps (inputs resident on the device), the
 wall-clock per step, and the fused stage-1 kernel's duration (HIP events around back-to-back
@@ -34,6 +35,7 @@ if __package__ in (None, ""):
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 
 import spp_amd as S  # noqa: E402
+from spp_amd.autotune import autotune  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
@@ -60,12 +62,13 @@ def timed(e, n, steps, chunk, warm):
     return n * steps / el, el / steps * 1e3, i1.rebuilds - i0.rebuilds
 
 
-def report(name, workload, n, steps, rate, ms, rebuilds, e, rk4=False):
+def report(name, workload, n, steps, rate, ms, rebuilds, e, rk4=False, kernel_variant=S.KERNEL_SLOT):
     us = e.time_stage(1, 50)
     b = stage1_bytes(e.info().mean_neighbors, rk4) * n
     out = {"config": name, "workload": workload, "particles": n, "steps": steps, "particle_steps_per_s": rate,
                  "ms_per_step": ms, "rebuilds": rebuilds, "stage1_us": us, "mean_full_list": e.info().mean_neighbors,
-                 "stage1_bytes": b, "stage1_hbm_frac": b / (us * 1e-6) / 1e9 / HBM_PEAK_GBS}
+                 "stage1_bytes": b, "stage1_hbm_frac": b / (us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                 "kernel_variant": kernel_variant}
     print(json.dumps(out), flush=True)
 
 
@@ -134,8 +137,14 @@ def lyapunov64(scale):
                   xs.append(x)
                   ys.append(y)
                   ts.append(t)
-    e = S.Engine(p, systems)
-    e.set_state(np.concatenate(xs), np.concatenate(ys), np.concatenate(ts))
+    x, y, th = np.concatenate(xs), np.concatenate(ys), np.concatenate(ts)
+    # the autotuner picks four or eight neighbour records in flight per lane by timing plain steps
+    # of this batch (about a second); the winner is reported with the line
+    tuned = autotune(p, x, y, th, systems=systems, steps=100, warm=20, variants=[S.KERNEL_SLOT, S.KERNEL_SLOT_DEEP],
+                     list_factors=(1.3,), cell_scales=(1.0,))
+    p = tuned.params
+    e = S.Engine(p, tuned.systems)
+    e.set_state(x, y, th)
     e.lyapunov_init(R, K, 1e-7, rescale=True, seed=4)
     e.lyapunov_run(50, 10)
     e.synchronize()
@@ -146,7 +155,8 @@ def lyapunov64(scale):
     e.synchronize()
     el = time.perf_counter() - t0
     report("lyapunov64", "64 Lyapunov replicas x (1 + 4 clones) x 16384, RK4 dt 0.005, renormalised every 10 steps", N,
-                 steps, N * steps / el, el / steps * 1e3, e.info().rebuilds, e, rk4=True)
+                 steps, N * steps / el, el / steps * 1e3, e.info().rebuilds, e, rk4=True,
+           kernel_variant=p.kernel_variant)
 
 
 def main(argv=None):

@@ -41,7 +41,7 @@ namespace {
 
 constexpr int kBlock = 256;
 // below this many particles (two wavefronts per SIMD of the device) the stage kernels keep
-// eight neighbour records in flight per lane instead of four
+// eight neighbour records in flight per lane instead of four (SPP_KERNEL_SLOT_DEEP: at every size)
 constexpr int kSmallN = 131072;
 constexpr int kScanItems = 4;  // items per thread in the cell-count scan
 constexpr int kScanTile = kBlock * kScanItems;
@@ -1595,6 +1595,7 @@ struct spp_engine {
     int nchunks = 0;             // 256-slot chunks of the LDS variant
     int chunk_grid = 0;          // nchunks rounded up to a multiple of 8 (XCD remap)
     int variant = SPP_KERNEL_SLOT;
+    bool deep = false;           // SPP_KERNEL_SLOT_DEEP: eight records in flight at every size
     int nstat = 0;               // entries of the per-workgroup statistics arrays
     Chunk* chunks = nullptr;
     float2* build = nullptr;
@@ -1961,7 +1962,7 @@ int launch_step(spp_engine* e, int S, int W, int kstep, int slot, bool record)
             a.out = o[s];
             a.th_out = ot[s];
         }
-        StageFn fn = pick_stage(e->p.model, e->p.repulsion, integ, stage, e->variant, e->N < kSmallN);
+        StageFn fn = pick_stage(e->p.model, e->p.repulsion, integ, stage, e->variant, e->deep || e->N < kSmallN);
         if (e->variant == SPP_KERNEL_STRIP_LDS)
             TRY(launch_chunks(e, stage_name(integ, stage), fn, a));
         else
@@ -2540,10 +2541,11 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
         return fail(nullptr, SPP_ERR_ARG, "noise_mode %d", p->noise_mode);
     if (p->rebuild != SPP_REBUILD_DISPLACEMENT && p->rebuild != SPP_REBUILD_PERIOD)
         return fail(nullptr, SPP_ERR_ARG, "rebuild %d", p->rebuild);
-    if (p->kernel_variant < SPP_KERNEL_SLOT || p->kernel_variant > SPP_KERNEL_PERSIST)
+    if (p->kernel_variant < SPP_KERNEL_SLOT || p->kernel_variant > SPP_KERNEL_SLOT_DEEP)
         return fail(nullptr, SPP_ERR_ARG, "kernel_variant %d", p->kernel_variant);
-    if (systems && p->kernel_variant != SPP_KERNEL_SLOT && p->kernel_variant != SPP_KERNEL_PERSIST)
-        return fail(nullptr, SPP_ERR_UNSUPPORTED, "batched engines run the slot kernels (kernel_variant 0 or 3)");
+    if (systems && p->kernel_variant != SPP_KERNEL_SLOT && p->kernel_variant != SPP_KERNEL_PERSIST &&
+        p->kernel_variant != SPP_KERNEL_SLOT_DEEP)
+        return fail(nullptr, SPP_ERR_UNSUPPORTED, "batched engines run the slot kernels (kernel_variant 0, 3 or 4)");
     const bool persist = p->kernel_variant == SPP_KERNEL_PERSIST;
     if (persist && p->noise_mode != SPP_NOISE_STREAM)
         return fail(nullptr, SPP_ERR_UNSUPPORTED, "the persistent kernel draws its noise from the stream");
@@ -2612,7 +2614,9 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
     e->nblk = nblocks(e->N);
     e->nchunks = (e->N + kChunk - 1) / kChunk;
     e->chunk_grid = (e->nchunks + 7) / 8 * 8;
-    e->variant = systems ? kVariantBatch : (persist ? SPP_KERNEL_SLOT : p->kernel_variant);
+    const bool deep = p->kernel_variant == SPP_KERNEL_SLOT_DEEP;
+    e->variant = systems ? kVariantBatch : ((persist || deep) ? SPP_KERNEL_SLOT : p->kernel_variant);
+    e->deep = deep;
     e->persist = persist;
     e->nstat = std::max(e->chunk_grid, e->nblk);
     // the LDS stage kernels read list rows 0..kPfList-1 unconditionally
@@ -3461,7 +3465,7 @@ int spp_time_stage(spp_engine* e, int32_t stage, int32_t reps, double* avg_us)
     a.U_out = reinterpret_cast<double2*>(e->aux_alt[0]);
     a.dth = stage <= 1 ? e->dth_alt : e->dth;
     a.kacc = e->aux_alt[1];
-    StageFn fn = pick_stage(e->p.model, e->p.repulsion, integ, stage, e->variant, e->N < kSmallN);
+    StageFn fn = pick_stage(e->p.model, e->p.repulsion, integ, stage, e->variant, e->deep || e->N < kSmallN);
     hipEvent_t t0, t1;
     HIPC(e, hipEventCreate(&t0));
     HIPC(e, hipEventCreate(&t1));

@@ -58,6 +58,14 @@ def test_create_rejects_bad_parameters(engine_mod):
     p.abi_version = 99
     with pytest.raises(S.SppError):
         S.Engine(p)
+    p = engine_mod.default_params(S.MODEL_REPULSIVE, n=100, kernel_variant=S.KERNEL_SLOT_DEEP + 1)
+    with pytest.raises(S.SppError) as ei:
+        S.Engine(p)
+    assert ei.value.code == S.ERR_ARG
+    p = engine_mod.default_params(S.MODEL_ABP, n=100, kernel_variant=S.KERNEL_SLOT_DEEP)
+    with pytest.raises(S.SppError) as ei:
+        S.Engine(p)  # the active Brownian model runs on the slot kernel only
+    assert ei.value.code == S.ERR_UNSUPPORTED
 
 
 def test_host_noise_matches_oracle(engine_mod, oracle_mod):

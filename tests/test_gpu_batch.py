@@ -197,6 +197,26 @@ def lyap_engine(p, R, K, n, L, D, eps, rescale=True):
     return e, systems
 
 
+def test_lyapunov_deep_variant_bitwise_equals_slot():
+    """Ten 16,384-particle systems (2 replicas x 5) exceed 131,072 particles, where the slot
+    kernel keeps four neighbour records in flight and the deep variant eight; both add the pairs
+    in list order, so states and growth sums are bit-for-bit equal."""
+    R, K, n, L = 2, 4, 16384, 64.0
+    lx, ly = S.triangle_lattice(n, L, L)
+    out = []
+    for v in (S.KERNEL_SLOT, S.KERNEL_SLOT_DEEP):
+        systems = [S.system(n, L, v0=1.0, g=1.0, D_rot=0.05, seed=20 + r) for r in range(R) for _ in range(K + 1)]
+        ts = [S.random_state(50 + r, n, L, L)[2] for r in range(R) for _ in range(K + 1)]
+        e = S.Engine(lyap_params(kernel_variant=v), systems)
+        e.set_state(np.tile(lx, R * (K + 1)), np.tile(ly, R * (K + 1)), np.concatenate(ts))
+        e.lyapunov_init(R, K, 1e-7, rescale=True, seed=4)
+        e.lyapunov_run(40, 10)
+        out.append((e.get_state(), e.lyapunov_get()[0]))
+    for k in ("xw", "yw", "xu", "yu", "theta"):
+        assert np.array_equal(out[0][0][k], out[1][0][k]), k
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 def oracle_replicas(oracle_mod, e, p, systems, R, K, eps, rescale):
     import oracle.lyapunov as OL
 

@@ -305,7 +305,7 @@ def test_polarization_matches_oracle(oracle_mod):
 # Interaction-kernel variants: same physics, different gather strategies
 # ---------------------------------------------------------------------------------------------
 
-@pytest.mark.parametrize("variant", [S.KERNEL_SLOT, S.KERNEL_SLOT_SPLIT, S.KERNEL_STRIP_LDS])
+@pytest.mark.parametrize("variant", [S.KERNEL_SLOT, S.KERNEL_SLOT_SPLIT, S.KERNEL_STRIP_LDS, S.KERNEL_SLOT_DEEP])
 def test_kernel_variants_match_oracle(oracle_mod, variant):
     p = c2_params(n=16384, L=64.0, kernel_variant=variant)
     x, y, th = oracle_mod.random_state(21, p.n, p.Lx, p.Ly)
@@ -358,6 +358,24 @@ def test_lds_variant_bitwise_equals_slot(oracle_mod, n, L, squeeze, dt):
         e.multi_step(60)
         out.append(e.get_state())
         off, idx = e.get_neighbors()  # decodes and cross-checks the LDS-offset list
+    for k in ("xw", "yw", "xu", "yu", "theta"):
+        assert np.array_equal(out[0][k], out[1][k]), k
+
+
+@pytest.mark.parametrize("integ", [S.RK2, S.RK4])
+def test_deep_variant_bitwise_equals_slot(oracle_mod, integ):
+    """Above 131,072 particles the slot kernel keeps four neighbour records in flight per lane and
+    the deep variant eight; both add the pairs in list order, so the trajectories are bit-for-bit
+    equal."""
+    n, L = 147456, 192.0
+    th = oracle_mod.random_state(8, n, L, L)[2]
+    x, y = S.triangle_lattice(n, L, L)  # no deep overlaps among 147,456 particles
+    out = []
+    for v in (S.KERNEL_SLOT, S.KERNEL_SLOT_DEEP):
+        e = S.Engine(c2_params(n=n, L=L, kernel_variant=v, D=0.2).update(integrator=integ))
+        e.set_state(x, y, th)
+        e.multi_step(60)
+        out.append(e.get_state())
     for k in ("xw", "yw", "xu", "yu", "theta"):
         assert np.array_equal(out[0][k], out[1][k]), k
 
