@@ -43,12 +43,12 @@ constexpr int kBlock = 256;
 // below this many particles (two wavefronts per SIMD of the device) the stage kernels keep
 // eight neighbour records in flight per lane instead of four (SPP_KERNEL_SLOT_DEEP: at every size)
 constexpr int kSmallN = 131072;
-// up to this many particles the stage kernels write their outputs with streaming stores: a stage's
-// outputs (40-56 B per particle) then fit in the eight L2s, where ordinary stores would wait dirty
-// for the write-back at the end of the launch.  Above it the L2s evict them during the launch
-// anyway, and ordinary stores leave them in the memory-side cache for the next launch (at
-// 1,048,576 particles streaming made the step 17 % slower; at 262,144 4.6 % faster).
-constexpr int kStreamMaxN = 524288;
+// up to this many particles the stage kernels write their outputs with streaming stores: the
+// launch is short, and ordinary stores would leave its outputs dirty in the L2s until the
+// write-back at its end.  Larger engines keep ordinary stores, which leave the outputs in the
+// memory-side cache for the next launch: per step, streaming was 4.6 % faster at 262,144
+// particles, 0.8 % slower at 393,216, 9 % slower at 524,288 and 17 % slower at 1,048,576.
+constexpr int kStreamMaxN = 262144;
 constexpr int kScanItems = 4;  // items per thread in the cell-count scan
 constexpr int kScanTile = kBlock * kScanItems;
 
