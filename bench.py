@@ -34,32 +34,57 @@ def workload_params(S, n, L, D, variant=0):
 
 
 def stage_bytes(kbar):
-    """Algorithmic HBM bytes per particle for each fused RK2 stage (useful bytes only; neighbour
-    coordinates are counted once, as the particle's own record, since gathers hit cache).
-      stage 1: read x,y,theta (24) + list length (4) + list (4 kbar) + id (4) + build pos (8);
-               write mid x,y,theta (24) + noise (8)
-      stage 2: read mid (24) + length (4) + list (4 kbar) + theta_old (8) + x_u,y_u (16) +
-               noise (8) + build pos (8); write x,y,theta (24) + x_u,y_u (16)"""
-    s1 = 24 + 4 + 4 * kbar + 4 + 8 + 24 + 8
-    s2 = 24 + 4 + 4 * kbar + 8 + 16 + 8 + 8 + 24 + 16
-    return s1, s2
+    """Algorithmic bytes per particle of each fused RK2 stage: the minimum a stage must move, FP64
+    SoA, noise generated in registers, a neighbour's record counted once (as its owner's), kbar the
+    mean full-list length.
+      stage 1: read x,y,theta (24) + list (4 kbar); write midpoint x,y,theta (24)    = 48 + 4 kbar
+      stage 2: read midpoint (24) + x_u,y_u,theta_old (24) + list (4 kbar);
+               write x_w,y_w,x_u,y_u,theta (40)                                      = 88 + 4 kbar
+    A whole RK2 step is 136 + 8 kbar, plus about 100 B per rebuild."""
+    return 48 + 4 * kbar, 88 + 4 * kbar
 
 
-def measured_traffic(kernel, variant):
+def stage_impl_bytes(kbar):
+    """Bytes the engine's fused RK2 stages actually move per particle (see stage_bytes for the
+    algorithmic minimum):
+      stage 1: read record x,y,theta,- (32) + list length (4) + list (4 kbar) + id (4, noise
+               counter) + build position (8, displacement check); write midpoint record (32,
+               theta_old in its fourth field) + noise (8)                             = 88 + 4 kbar
+      stage 2: read midpoint record (32) + length (4) + list (4 kbar) + noise (8) + x_u,y_u (16)
+               + build position (8); write record (32) + theta (8) + x_u,y_u (16)    = 124 + 4 kbar"""
+    return 88 + 4 * kbar, 124 + 4 * kbar
+
+
+REBUILD_BYTES = 100.0  # algorithmic bytes per particle and rebuild (binning, sort, list)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_rk2_c2_n262144_traffic.json")
+
+
+def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary (rocprofv3 --pmc
     FETCH_SIZE and WRITE_SIZE passes, each scaled by the same counter on a copy kernel of known
-    size), or None when no summary for this kernel variant exists."""
-    path = os.path.join(ROOT, "profiles", "stage_rk2_c2_n262144_traffic.json")
+    size) and a label naming the summary and the kernel build it was measured on; (None, None)
+    when the summary does not cover this kernel."""
     try:
-        with open(path) as f:
+        with open(TRAFFIC_FILE) as f:
             d = json.load(f)
-        t = d["kernels"][str(variant)][kernel]
-        return t["hbm_bytes_per_launch"]
+        t = d["kernels"][kernel]
+        return t["hbm_bytes_per_launch"], "%s (%s)" % (os.path.relpath(TRAFFIC_FILE, ROOT), d["measured_on"])
     except (OSError, KeyError, ValueError):
-        return None
+        return None, None
 
 
-def cpu_baseline(p, x, y, th, budget_s, threads=1):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_baseline(p, x, y, th, budget_s, threads=1, library=None):
     """Particle-steps/s of `threads` concurrent copies of the C oracle on the same workload (the
     reference's replica pattern, serial/mpi-main.cpp:157-160), each stepped for about budget_s
     seconds after 2 warm-up steps.  ctypes releases the GIL inside the C calls, so the copies run
@@ -68,9 +93,7 @@ def cpu_baseline(p, x, y, th, budget_s, threads=1):
 
     import oracle
 
-    if not os.path.exists(oracle.LIB_PATH):
-        oracle.build()
-    copies = [oracle.Oracle(p) for _ in range(threads)]
+    copies = [oracle.Oracle(p, library=library) for _ in range(threads)]
     rates = [0.0] * threads
     steps = [0] * threads
 
@@ -150,6 +173,10 @@ def main():
     info2 = e.info()
     kbar = info2.mean_neighbors
     b1, b2 = stage_bytes(kbar)
+    i1, i2 = stage_impl_bytes(kbar)
+    # mean steps between rebuilds over the timed and the profiled runs together
+    nreb = (info1.rebuilds - info0.rebuilds) + (info2.rebuilds - info1.rebuilds)
+    interval = (args.steps + args.profile_steps) / max(1, nreb)
     # A plain copy of known size (the snapshot copy: 60 B per particle read and written) -- the
     # calibration kernel for hardware byte counters in profiling runs.
     e.snapshot(99)
@@ -173,7 +200,11 @@ def main():
     dom_ms = stage_us[dom] * 1e-3
     dom_bytes = float(np.mean([q[2][col] for q in samples])) * n
     achieved = sum(q[2][col] * n for q in samples) / (sum(q[col] for q in samples) * 1e-6) / 1e9
+    impl_bytes = (i1, i2)[col] * n
     kernel_ms = {k: round(v[0] * 1e3, 3) for k, v in prof.items()}  # microseconds
+    # whole step against the HBM roofline: 136 + 8 kbar per particle-step plus the rebuilds
+    step_bytes = b1 + b2 + REBUILD_BYTES / interval
+    traffic, traffic_src = measured_traffic(dom) if args.n == 262144 and args.variant == 0 else (None, None)
     out = {
         "metric": "particle-steps/s at N=262k RK2",
         "value": value,
@@ -198,6 +229,7 @@ def main():
             "rebuilds_in_timed": info1.rebuilds - info0.rebuilds,
             "host_syncs_in_timed": info1.host_syncs - info0.host_syncs,
             "rollbacks_in_timed": info1.rollbacks - info0.rollbacks,
+            "steps_per_rebuild": round(interval, 2),
         },
         "roofline": {
             "bound": "hbm",
@@ -206,35 +238,55 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(dom, args.variant) if args.n == 262144 else None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "bytes_per_launch": dom_bytes,
+            "bytes_rule": "48 + 4 kbar (stage 1) / 88 + 4 kbar (stage 2) per particle, kbar = mean full-list length",
+            "implementation_bytes_per_launch": impl_bytes,
+            "implementation_frac": impl_bytes / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "avg_launch_us": dom_ms * 1e3,
+        },
+        "step_roofline": {
+            "bytes_per_particle_step": step_bytes,
+            "bytes_rule": "136 + 8 kbar + 100 / steps_per_rebuild",
+            "achieved": step_bytes * value / 1e9,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": step_bytes * value / 1e9 / HBM_PEAK_GBS,
         },
         "stage_kernel_us": {k: round(v, 3) for k, v in stage_us.items()},
         "stage_kernel_us_at_end": {k: round(v, 3) for k, v in end_us.items()},
         "kernel_avg_us_event_bracketed": kernel_ms,
-        "step_algorithmic_bytes_per_particle": b1 + b2,
-        "step_achieved_GBs": (b1 + b2) * value / 1e9,
     }
 
     if not args.no_cpu_baseline:
         # Bounded samples of the same workload on the host: the C oracle (AoS, per-cell lists,
         # Newton-3 half Verlet list -- the reference's data structures), first on one core, then
         # as one copy per core of the host share, all at once.
+        # The oracle is compiled here for this host (-O3 -march=native, the flags the CPU-baseline
+        # plan names); the portable -O3 build of the tests is the fallback.
+        import oracle
+
+        try:
+            lib_path, flags = oracle.build_native(), " ".join(oracle.NATIVE_FLAGS)
+        except Exception:  # no compiler on this host
+            lib_path, flags = oracle.build(), "-O3 (portable build; native build unavailable)"
+        library = oracle.load_library(lib_path)
         cp = workload_params(S, n, L, args.noise)
-        one, _, steps1 = cpu_baseline(cp, x, y, th, args.cpu_seconds / 2, 1)
+        one, _, steps1 = cpu_baseline(cp, x, y, th, args.cpu_seconds / 2, 1, library)
         P = max(1, min(args.cpu_threads, os.cpu_count() or 1))
-        agg, rates, stepsP = cpu_baseline(cp, x, y, th, args.cpu_seconds / 2, P)
+        agg, rates, stepsP = cpu_baseline(cp, x, y, th, args.cpu_seconds / 2, P, library)
         out["cpu_baseline"] = {
             "value": agg,
             "unit": "particle-steps/s",
             "cores": P,
             "kind": "port",
-            "sample": "same workload (N=%d, RK2, displacement rebuilds) on %s: %d concurrent copies of the C "
-                      "oracle, %d steps in %.1f s each after 2 warm-up steps (aggregate); one copy alone: "
-                      "%.4g particle-steps/s over %d steps, x%d = %.4g" % (
-                          n, platform.processor() or platform.machine(), P, stepsP // P, args.cpu_seconds / 2,
-                          one, steps1, P, one * P),
+            "sample": "same workload (N=%d, RK2, displacement rebuilds, same start state) on %s (%d logical CPUs "
+                      "visible): %d concurrent copies of the C oracle (AoS particles, per-cell lists, Newton-3 half "
+                      "Verlet list), %s %s, %d steps in %.1f s each after 2 warm-up steps (aggregate); one copy "
+                      "alone: %.4g particle-steps/s over %d steps, x%d = %.4g" % (
+                          n, cpu_model(), os.cpu_count() or 0, P, oracle.compiler_version(), flags, stepsP // P,
+                          args.cpu_seconds / 2, one, steps1, P, one * P),
             "single_core": one,
             "gpu_over_cpu": value / agg,
             "gpu_over_ideal_cores": value / (one * P),

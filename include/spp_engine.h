@@ -79,6 +79,16 @@ enum {
                                      then Update_Cells, parallel/box.h:394-439)                   */
 };
 
+/* ---- how a rebuild sorts the particles into cells ---------------------------------------------- */
+enum {
+    SPP_REBUILD_SORT_AUTO = 0,   /* one-launch tile sort where it applies (single box, slot kernels,
+                                    no interval graphs): a particle moves less than a cell between
+                                    builds, so each tile of a cell row gathers its particles from the
+                                    previous order; the general sort takes over when that fails     */
+    SPP_REBUILD_SORT_GENERAL = 1 /* always the general counting sort (binning, scan, scatter,
+                                    in-cell order, permutation); same slots bit for bit            */
+};
+
 /* ---- angular noise source --------------------------------------------------------------------- */
 enum {
     SPP_NOISE_STREAM = 0,  /* Philox4x32-10 stream addressed by (seed, particle id, step), Box-Muller */
@@ -138,7 +148,8 @@ typedef struct spp_params {
     int32_t graph_intervals;/* 1: with SPP_REBUILD_PERIOD (RK2/RK4, no sampling), capture each full
                                interval (rebuild + rebuild_period steps) once as a HIP graph per
                                buffer layout and replay it; 0: launch every kernel                  */
-    int32_t reserved_i[5];
+    int32_t rebuild_sort;   /* SPP_REBUILD_SORT_*                                                   */
+    int32_t reserved_i[4];
     double reserved_d[11];
 } spp_params;
 
@@ -166,6 +177,8 @@ typedef struct spp_info {
     int32_t ell_capacity;
     int32_t max_neighbors;  /* longest full neighbour list at the last rebuild                     */
     double mean_neighbors;  /* mean full neighbour-list length at the last rebuild                 */
+    int64_t tile_rebuilds;  /* rebuilds sorted by the one-launch tile path (SPP_REBUILD_SORT_AUTO)  */
+    int64_t tile_fallbacks; /* tile sorts that failed and were redone by the general sort           */
 } spp_info;
 
 /* ---- lifetime ----------------------------------------------------------------------------------- */

@@ -24,6 +24,27 @@ def build() -> str:
     return LIB_PATH
 
 
+NATIVE_FLAGS = ["-O3", "-march=native", "-ffp-contract=off", "-std=c99", "-fPIC"]
+NATIVE_LIB_PATH = os.path.join(HERE, "build", "libspp_oracle_native.so")
+
+
+def build_native(cc: str = "gcc") -> str:
+    """Compile the oracle for the host it runs on (-O3 -march=native, contraction off so the
+    arithmetic stays that of the portable build): the timed CPU baseline, built where it is timed."""
+    os.makedirs(os.path.dirname(NATIVE_LIB_PATH), exist_ok=True)
+    tmp = NATIVE_LIB_PATH + ".tmp"
+    subprocess.run([cc, *NATIVE_FLAGS, "-shared", "-o", tmp, os.path.join(HERE, "spp_oracle.c"), "-lm"], check=True)
+    os.replace(tmp, NATIVE_LIB_PATH)
+    return NATIVE_LIB_PATH
+
+
+def compiler_version(cc: str = "gcc") -> str:
+    try:
+        return subprocess.run([cc, "--version"], capture_output=True, text=True, check=True).stdout.split("\n")[0]
+    except (OSError, subprocess.CalledProcessError):
+        return "unknown"
+
+
 _lib = None
 
 
@@ -32,44 +53,49 @@ def lib() -> C.CDLL:
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} missing: run `make -C oracle`")
-        L = C.CDLL(LIB_PATH)
-        L.orc_error.restype = C.c_char_p
-        L.orc_periodic.restype = C.c_double
-        L.orc_periodic.argtypes = [C.c_double, C.c_double]
-        L.orc_cell_quirk.argtypes = [C.c_double, C.c_double, C.c_int]
-        L.orc_cell_exact.argtypes = [C.c_double, C.c_double, C.c_int]
-        L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
-        L.orc_noise_normal.restype = C.c_double
-        L.orc_noise_normal.argtypes = [C.c_uint64, C.c_int32, C.c_int64]
-        L.orc_force_law.argtypes = [C.c_int] + [C.c_double] * 6 + [_D, _D]
-        L.orc_brute_pairs.restype = C.c_int64
-        L.orc_brute_pairs.argtypes = [_D, _D, C.c_int32, C.c_double, C.c_double, C.c_double, _I32, _I32, C.c_int64]
-        L.orc_random_state.argtypes = [C.c_uint64, C.c_int32, C.c_double, C.c_double, _D, _D, _D]
-        L.orc_create.restype = C.c_void_p
-        L.orc_create.argtypes = [C.c_void_p]
-        L.orc_destroy.argtypes = [C.c_void_p]
-        for nm in ("orc_set_state", "orc_get_state"):
-            getattr(L, nm).argtypes = [C.c_void_p, _D, _D, _D, _D, _D, C.c_int32]
-        L.orc_update_cells.argtypes = [C.c_void_p]
-        L.orc_get_cells.argtypes = [C.c_void_p, _I32, _I32, C.c_int32]
-        L.orc_half_list.restype = C.c_int64
-        L.orc_half_list.argtypes = [C.c_void_p, _I32, _I32, C.c_int64]
-        L.orc_interact.argtypes = [C.c_void_p]
-        L.orc_ring_energy.restype = C.c_double
-        L.orc_ring_energy.argtypes = [C.c_void_p]
-        L.orc_get_forces.argtypes = [C.c_void_p, _D, _D, _D, C.c_int32]
-        L.orc_set_noise.argtypes = [C.c_void_p, _D, C.c_int32]
-        L.orc_move_stage.argtypes = [C.c_void_p, C.c_int32]
-        L.orc_one_step.argtypes = [C.c_void_p]
-        L.orc_multi_step.argtypes = [C.c_void_p, C.c_int32]
-        L.orc_polarization.restype = C.c_double
-        L.orc_polarization.argtypes = [C.c_void_p, _D, _D]
-        L.orc_step_count.restype = C.c_int64
-        L.orc_step_count.argtypes = [C.c_void_p]
-        L.orc_rebuild_count.restype = C.c_int64
-        L.orc_rebuild_count.argtypes = [C.c_void_p]
-        _lib = L
+        _lib = load_library(LIB_PATH)
     return _lib
+
+
+def load_library(path: str) -> C.CDLL:
+    """Load an oracle build and declare its C signatures."""
+    L = C.CDLL(path)
+    L.orc_error.restype = C.c_char_p
+    L.orc_periodic.restype = C.c_double
+    L.orc_periodic.argtypes = [C.c_double, C.c_double]
+    L.orc_cell_quirk.argtypes = [C.c_double, C.c_double, C.c_int]
+    L.orc_cell_exact.argtypes = [C.c_double, C.c_double, C.c_int]
+    L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    L.orc_noise_normal.restype = C.c_double
+    L.orc_noise_normal.argtypes = [C.c_uint64, C.c_int32, C.c_int64]
+    L.orc_force_law.argtypes = [C.c_int] + [C.c_double] * 6 + [_D, _D]
+    L.orc_brute_pairs.restype = C.c_int64
+    L.orc_brute_pairs.argtypes = [_D, _D, C.c_int32, C.c_double, C.c_double, C.c_double, _I32, _I32, C.c_int64]
+    L.orc_random_state.argtypes = [C.c_uint64, C.c_int32, C.c_double, C.c_double, _D, _D, _D]
+    L.orc_create.restype = C.c_void_p
+    L.orc_create.argtypes = [C.c_void_p]
+    L.orc_destroy.argtypes = [C.c_void_p]
+    for nm in ("orc_set_state", "orc_get_state"):
+        getattr(L, nm).argtypes = [C.c_void_p, _D, _D, _D, _D, _D, C.c_int32]
+    L.orc_update_cells.argtypes = [C.c_void_p]
+    L.orc_get_cells.argtypes = [C.c_void_p, _I32, _I32, C.c_int32]
+    L.orc_half_list.restype = C.c_int64
+    L.orc_half_list.argtypes = [C.c_void_p, _I32, _I32, C.c_int64]
+    L.orc_interact.argtypes = [C.c_void_p]
+    L.orc_ring_energy.restype = C.c_double
+    L.orc_ring_energy.argtypes = [C.c_void_p]
+    L.orc_get_forces.argtypes = [C.c_void_p, _D, _D, _D, C.c_int32]
+    L.orc_set_noise.argtypes = [C.c_void_p, _D, C.c_int32]
+    L.orc_move_stage.argtypes = [C.c_void_p, C.c_int32]
+    L.orc_one_step.argtypes = [C.c_void_p]
+    L.orc_multi_step.argtypes = [C.c_void_p, C.c_int32]
+    L.orc_polarization.restype = C.c_double
+    L.orc_polarization.argtypes = [C.c_void_p, _D, _D]
+    L.orc_step_count.restype = C.c_int64
+    L.orc_step_count.argtypes = [C.c_void_p]
+    L.orc_rebuild_count.restype = C.c_int64
+    L.orc_rebuild_count.argtypes = [C.c_void_p]
+    return L
 
 
 def _p(a, t=_D):
@@ -130,10 +156,10 @@ class OracleError(RuntimeError):
 class Oracle:
     """The reference's serial Box, restated in C, driven by an spp_params struct."""
 
-    def __init__(self, params):
+    def __init__(self, params, library=None):
         import ctypes as _C
 
-        self._lib = lib()
+        self._lib = library if library is not None else lib()
         self._params = params
         self._h = self._lib.orc_create(_C.byref(params))
         if not self._h:

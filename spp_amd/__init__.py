@@ -25,6 +25,7 @@ REP_NONE, REP_R12_TRUNCATED, REP_HARMONIC, REP_R12, REP_YUKAWA_TRUNCATED, REP_YU
 BIN_EXACT, BIN_REF_QUIRK = 0, 1
 REBUILD_DISPLACEMENT, REBUILD_PERIOD = 0, 1
 NOISE_STREAM, NOISE_INJECTED = 0, 1
+REBUILD_SORT_AUTO, REBUILD_SORT_GENERAL = 0, 1
 KERNEL_SLOT, KERNEL_SLOT_SPLIT, KERNEL_STRIP_LDS, KERNEL_PERSIST, KERNEL_SLOT_DEEP = 0, 1, 2, 3, 4
 PERSIST_MAX_N = 4096  # particles per system of the persistent kernel (KERNEL_PERSIST)
 
@@ -70,7 +71,8 @@ class Params(C.Structure):
         ("ell_capacity", C.c_int32),
         ("kernel_variant", C.c_int32),
         ("graph_intervals", C.c_int32),
-        ("reserved_i", C.c_int32 * 5),
+        ("rebuild_sort", C.c_int32),
+        ("reserved_i", C.c_int32 * 4),
         ("reserved_d", C.c_double * 11),
     ]
 
@@ -101,6 +103,8 @@ class Info(C.Structure):
         ("ell_capacity", C.c_int32),
         ("max_neighbors", C.c_int32),
         ("mean_neighbors", C.c_double),
+        ("tile_rebuilds", C.c_int64),
+        ("tile_fallbacks", C.c_int64),
     ]
 
 

@@ -79,13 +79,14 @@ __device__ __forceinline__ void ring_forces(const StageArgs& a, const Phys& P, i
 // Forces and torque on one particle: Reset + pair repulsion + ring forces.
 template <int REP>
 __device__ __forceinline__ void abp_forces(const StageArgs& a, const Phys& P0, int i, int pid, const P4& me,
-                                           double& fx, double& fy)
+                                           double mc, double ms, double& fx, double& fy)
 {
     Phys P = P0;
-    P.v0 = pid < a.abp.n_membrane ? 0.0 : a.abp.F0;
+    const bool bead = (unsigned)pid < (unsigned)a.abp.n_membrane;
+    P.v0 = bead ? 0.0 : a.abp.F0;
     double tq;
-    gather_forces_flat<SPP_MODEL_REPULSIVE, REP, false>(a, P, i, me, fx, fy, tq);
-    if (pid < a.abp.n_membrane) ring_forces(a, P, pid, me, fx, fy);
+    gather_forces_flat<SPP_MODEL_REPULSIVE, REP, false>(a, P, i, me, mc, ms, fx, fy, tq);
+    if (bead) ring_forces(a, P, pid, me, fx, fy);
 }
 
 template <int REP, int INTEG, int STAGE>
@@ -100,6 +101,8 @@ __global__ __launch_bounds__(kBlock) void k_stage_abp(StageArgs a)
     double x = 0, y = 0, dmax = 0;
     if (live) {
         P4 me = a.gather[i];
+        double mc, ms;  // heading of the stage input (stage 1: the old one; stage 2: the midpoint)
+        spp::heading_unit(me.z, mc, ms);
         int pid = a.id[i];
         double dn, th_in = 0, thold = 0;
         double2 u = make_double2(0, 0);
@@ -114,7 +117,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_abp(StageArgs a)
             u = a.U_in[i];
         }
         double fx, fy;
-        abp_forces<REP>(a, P, i, pid, me, fx, fy);
+        abp_forces<REP>(a, P, i, pid, me, mc, ms, fx, fy);
         const double tq = A.torque0, dm = A.mob_par - A.mob_perp;
         double th, sn, cs;
         if (INTEG == SPP_EULER) {
@@ -136,27 +139,25 @@ __global__ __launch_bounds__(kBlock) void k_stage_abp(StageArgs a)
             th = spp::wrap_angle(th);
             double xx = me.x + fx * (P.half_dt * A.mob_perp);
             double yy = me.y + fy * (P.half_dt * A.mob_perp);
-            double fv = fx * me.c + fy * me.s;
-            xx += me.c * (fv * (P.half_dt * dm));
-            yy += me.s * (fv * (P.half_dt * dm));
+            double fv = fx * mc + fy * ms;
+            xx += mc * (fv * (P.half_dt * dm));
+            yy += ms * (fv * (P.half_dt * dm));
             x = spp::periodic_wrap(xx, P.Lx, P.Lx2);
             y = spp::periodic_wrap(yy, P.Ly, P.Ly2);
-            spp::sincos_heading(th, &sn, &cs);
         } else {
             th = thold + A.mob_rot * P.dt * (tq);
             th += dn;
             th = spp::wrap_angle(th);
             u.x = u.x + fx * (P.dt * A.mob_perp);
             u.y = u.y + fy * (P.dt * A.mob_perp);
-            double fv = fx * me.c + fy * me.s;
-            u.x += me.c * (fv * (P.dt * dm));
-            u.y += me.s * (fv * (P.dt * dm));
+            double fv = fx * mc + fy * ms;
+            u.x += mc * (fv * (P.dt * dm));
+            u.y += ms * (fv * (P.dt * dm));
             a.U_out[i] = u;
             x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
             y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
-            spp::sincos_heading(th, &sn, &cs);
         }
-        a.out[i] = P4{x, y, cs, sn};
+        a.out[i] = P4{x, y, th, 0.0};
         a.th_out[i] = th;
     }
     check_disp(a, P, b, live, i, x, y, final_stage ? a.kstep + 1 : a.kstep, dmax);
@@ -169,8 +170,9 @@ __global__ __launch_bounds__(kBlock) void k_interact_abp(StageArgs a, P4* force)
     int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.N) return;
     P4 me = a.gather[i];
-    double fx, fy;
-    abp_forces<REP>(a, a.P, i, a.id[i], me, fx, fy);
+    double mc, ms, fx, fy;
+    spp::heading_unit(me.z, mc, ms);
+    abp_forces<REP>(a, a.P, i, a.id[i], me, mc, ms, fx, fy);
     force[i] = P4{fx, fy, a.abp.torque0, 0.0};
 }
 
@@ -182,6 +184,8 @@ __global__ __launch_bounds__(kBlock) void k_move_abp(MoveArgs a)
     const Phys& P = a.P;
     const AbpConst& A = a.abp;
     P4 me = a.cur.pos[i];
+    double mc, ms;
+    spp::heading_unit(me.z, mc, ms);
     double th_in = a.cur.th[i];
     P4 f = a.force[i];
     double2 u = a.cur.U[i];
@@ -191,7 +195,7 @@ __global__ __launch_bounds__(kBlock) void k_move_abp(MoveArgs a)
         double dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)a.cur.id[i], a.step);
         a.dth[i] = dn;
         if (a.integ == SPP_EULER) {
-            th = th_in + A.mob_rot * P.dt * (f.c);
+            th = th_in + A.mob_rot * P.dt * (f.z);
             th += dn;
             spp::sincos_heading(th, &sn, &cs);
             u.x += f.x * (P.dt * A.mob_perp);
@@ -204,33 +208,31 @@ __global__ __launch_bounds__(kBlock) void k_move_abp(MoveArgs a)
             y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
         } else {
             a.old[i] = P4{u.x, u.y, th_in, 0.0};
-            th = th_in + A.mob_rot * P.half_dt * (f.c);
+            th = th_in + A.mob_rot * P.half_dt * (f.z);
             th += dn / 2;
             th = spp::wrap_angle(th);
             double xx = me.x + f.x * (P.half_dt * A.mob_perp);
             double yy = me.y + f.y * (P.half_dt * A.mob_perp);
-            double fv = f.x * me.c + f.y * me.s;
-            xx += me.c * (fv * (P.half_dt * dm));
-            yy += me.s * (fv * (P.half_dt * dm));
+            double fv = f.x * mc + f.y * ms;
+            xx += mc * (fv * (P.half_dt * dm));
+            yy += ms * (fv * (P.half_dt * dm));
             x = spp::periodic_wrap(xx, P.Lx, P.Lx2);
             y = spp::periodic_wrap(yy, P.Ly, P.Ly2);
-            spp::sincos_heading(th, &sn, &cs);
         }
     } else {
         P4 o = a.old[i];
-        th = o.c + A.mob_rot * P.dt * (f.c);
+        th = o.z + A.mob_rot * P.dt * (f.z);
         th += a.dth[i];
         th = spp::wrap_angle(th);
         u.x = o.x + f.x * (P.dt * A.mob_perp);
         u.y = o.y + f.y * (P.dt * A.mob_perp);
-        double fv = f.x * me.c + f.y * me.s;
-        u.x += me.c * (fv * (P.dt * dm));
-        u.y += me.s * (fv * (P.dt * dm));
+        double fv = f.x * mc + f.y * ms;
+        u.x += mc * (fv * (P.dt * dm));
+        u.y += ms * (fv * (P.dt * dm));
         a.cur.U[i] = u;
         x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
         y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
-        spp::sincos_heading(th, &sn, &cs);
     }
-    a.cur.pos[i] = P4{x, y, cs, sn};
+    a.cur.pos[i] = P4{x, y, th, 0.0};
     a.cur.th[i] = th;
 }

@@ -2,9 +2,12 @@
 // declared in include/spp_engine.h.
 //
 // Device layout (all per-particle arrays in CELL-SORTED slot order; id[slot] = caller's index):
-//   state set (x3)  pos P4[N] (x, y, cos th, sin th), th double[N], U double2[N] (unwrapped
-//                   positions, reference r_original), id int[N]
-//   pos1/th1, pos2/th2  RK stage states (pos1 = RK2 midpoint)
+//   state set (x3)  pos P4[N] (x, y, th, 0): wrapped position and heading angle, the record
+//                   neighbours gather; th double[N] (the same angles, read by the host-facing
+//                   and Lyapunov code); U double2[N] (unwrapped positions, reference
+//                   r_original); id int[N]
+//   pos1/th1, pos2/th2  RK stage states (pos1 = RK2 midpoint); their records carry the step's
+//                   starting angle theta_old in the fourth field
 //   dth    double[N]   this step's angular noise
 //   kacc   P4[N]       RK4 accumulator k1 + 2 k2 + 2 k3 (fx, fy, tau)
 //   nbr    int[cap*N]  full Verlet list, ELL, slot-major (nbr[k*N + i]): a wavefront's k-th
@@ -50,6 +53,7 @@ constexpr int kSmallN = 131072;
 // particles, 0.8 % slower at 393,216, 9 % slower at 524,288 and 17 % slower at 1,048,576.
 constexpr int kStreamMaxN = 262144;
 constexpr int kScanItems = 4;  // items per thread in the cell-count scan
+constexpr int kSets = 4;       // particle state sets (see spp_engine::set)
 constexpr int kScanTile = kBlock * kScanItems;
 
 // One interval's bookkeeping, reduced on the device from per-block partials (no same-address
@@ -59,6 +63,8 @@ struct Report {
     int max_len;      // longest full neighbour list of the interval's rebuild
     long long nnz;    // total entries of that list
     double maxdisp2;  // largest squared displacement at the interval's end
+    int sort_fail;    // the interval's rebuild took the tile path and could not sort (redo it)
+    int pad;
 };
 
 struct SlotBufs {
@@ -130,6 +136,13 @@ __global__ void k_fill(T* a, T v, int n)
 }
 
 __device__ __forceinline__ int wave_sum(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ __forceinline__ long long wave_sum_ll(long long v)
 {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -321,12 +334,231 @@ __global__ __launch_bounds__(kBlock) void k_rank_permute(PermuteArgs a, const in
     a.out.th[k] = a.in.th[me];
     a.out.U[k] = a.in.U[me];
     a.out.id[k] = mine;
-    a.inv[mine] = k;
+    if ((unsigned)mine < (unsigned)a.N) a.inv[mine] = k;
     if (a.d_in) {
         a.d_out[k] = a.d_in[me];
 #pragma unroll
         for (int q = 0; q < 3; ++q) a.a_out[q][k] = a.a_in[q][me];
     }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Rebuild, tile path: binning, counting sort, canonical order inside cells and the permutation in
+// ONE launch (the general path above takes five).
+//
+// Between two builds of the displacement policy no particle moves farther than the half skin,
+// which is less than a cell, so a particle now in cell (cx, cy) sat at the last build in one of
+// the cells (cx + a, cy + b), |a|, |b| <= 1, whose slots that build's cell starts still describe
+// (the slot order has not changed since).  A tile -- cells [c0, c1) of row y -- therefore collects
+// its particles from at most six runs of the current order (rows y-1..y+1, columns c0-1..c1,
+// periodic), sorts them in LDS by (cell, caller index) -- the general path's canonical order, so
+// both paths give the same slots bit for bit -- and learns its first slot from the tiles before it:
+// each tile publishes its count at once, then its 256 threads read the counts of all earlier tiles
+// together and sum them (a few hundred tiles, so one parallel read replaces a chained look-back,
+// whose memory-side loads would line up one after another).  Tile b is workgroup b, in row-major
+// order: workgroups are dispatched in index order and all of them fit on the device at once (a few
+// hundred of 256 threads and 33 KB of LDS), so a tile waits only for running ones.  The counts
+// carry the launch's epoch, so stale entries of earlier launches read as unpublished and nothing
+// has to be cleared between launches -- and no counter is shared by every workgroup (same-address
+// atomics from all XCDs serialise).
+//
+// A particle that moved farther than one cell is collected by no tile, so the total falls short of
+// N; that, a tile overflowing its LDS or a look-back that does not settle sets *fail, and the host
+// rolls the interval back and redoes the rebuild on the general path.  Every index a failed launch
+// computes is clamped into its array, so even a garbage input order stays in bounds.
+// ------------------------------------------------------------------------------------------------
+constexpr int kSortCap = 2048;     // particles one tile can hold (LDS)
+constexpr int kSortMaxW = 1024;    // cells per tile (LDS histogram; kBlock * kScanItems)
+constexpr int kSortTarget = 384;   // particles per tile the host aims for
+constexpr int kSpinLimit = 1 << 20;
+
+struct SortArgs {
+    int N;
+    Grid G;
+    int seg, segw, ntiles;         // tiles per row, cells per tile (the row's last may be narrower)
+    unsigned epoch;                // this launch's tag in the tile states (1 .. 2^30 - 1)
+    SetPtrs in, out;
+    int* inv;
+    const int* cs_old;             // cell starts of the build the input order follows
+    int* cs_new;
+    unsigned long long* tstate;    // per tile: (epoch << 2 | 1) << 32 | particle count
+    int* fail;
+};
+
+__device__ __forceinline__ unsigned long long tile_load(const unsigned long long* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void tile_store(unsigned long long* p, unsigned long long v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int clamp_slot(int v, int N) { return v < 0 ? 0 : (v > N ? N : v); }
+
+__global__ __launch_bounds__(kBlock) void k_sort_tiles(SortArgs a)
+{
+    __shared__ int e_old[kSortCap], e_id[kSortCap];
+    __shared__ uint16_t e_col[kSortCap], order[kSortCap];
+    __shared__ int hist[kSortMaxW + 1], fillc[kSortMaxW];
+    __shared__ int lds_wave[kBlock / 64];
+    __shared__ long long red[kBlock / 64];
+    __shared__ int s_n, s_base, s_bad;
+    const int t = threadIdx.x, N = a.N;
+    const int W = a.G.ncx, H = a.G.ncy;
+    const int tile = blockIdx.x;
+    const unsigned tag = a.epoch << 2;
+    if (t == 0) {
+        s_n = 0;
+        s_bad = 0;
+    }
+    for (int k = t; k <= a.segw; k += kBlock) hist[k] = 0;
+    for (int k = t; k < a.segw; k += kBlock) fillc[k] = 0;
+    __syncthreads();
+    const int y = tile / a.seg, c0 = (tile - y * a.seg) * a.segw;
+    const int c1 = min(W, c0 + a.segw), w = c1 - c0;
+
+    // The runs of the current order that hold every candidate: columns [c0-1, c1+1) of rows
+    // y-1..y+1, with the periodic column on the far side when the tile touches a row end.  Run k
+    // covers flattened candidates [p[k], p[k+1]); candidate q sits at slot q + off[k].
+    int p[7], off[6];
+    {
+        const int lo = c0 - 1, hi = c1 + 1;
+        int ra[6], rb[6];  // column range of each run (empty when ra == rb)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            int r = y + d - 1;
+            r = r < 0 ? r + H : (r >= H ? r - H : r);
+            const int ca = max(lo, 0), cb = min(hi, W);
+            ra[2 * d] = r * W + ca;
+            rb[2 * d] = r * W + cb;
+            // the periodic column beyond a row end, unless the main run already holds it
+            const bool wl = lo < 0 && cb < W, wr = hi > W && ca > 0;
+            ra[2 * d + 1] = wl ? r * W + W - 1 : (wr ? r * W : 0);
+            rb[2 * d + 1] = wl ? r * W + W : (wr ? r * W + 1 : 0);
+        }
+        p[0] = 0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int s0 = clamp_slot(a.cs_old[ra[k]], N), s1 = clamp_slot(a.cs_old[rb[k]], N);
+            const int len = ra[k] < rb[k] && s1 > s0 ? s1 - s0 : 0;
+            p[k + 1] = p[k] + len;
+            off[k] = s0 - p[k];
+        }
+    }
+    const int total = p[6];
+
+    // collect the particles whose new cell lies in the tile
+    for (int q = t; q < total; q += kBlock) {
+        int o = off[0];
+#pragma unroll
+        for (int k = 1; k < 6; ++k) o = q >= p[k] ? off[k] : o;
+        const int s = q + o;
+        const double2 xy = *reinterpret_cast<const double2*>(&a.in.pos[s].x);
+        const int cy = cell_coord(xy.y, a.G.Ly, a.G.Ly2, H, a.G.quirk);
+        if (cy != y) continue;
+        const int cx = cell_coord(xy.x, a.G.Lx, a.G.Lx2, W, a.G.quirk);
+        if (cx < c0 || cx >= c1) continue;
+        const int e = atomicAdd(&s_n, 1);
+        if (e < kSortCap) {
+            e_old[e] = s;
+            e_col[e] = (uint16_t)(cx - c0);
+            e_id[e] = a.in.id[s];
+            atomicAdd(&hist[cx - c0], 1);
+        }
+    }
+    __syncthreads();
+    const int n = min(s_n, kSortCap);
+    if (t == 0) {
+        if (s_n > kSortCap) s_bad = 1;
+        // publish the count at once, so later tiles can look past this one
+        tile_store(&a.tstate[tile], ((unsigned long long)(tag | 1u) << 32) | (unsigned)n);
+    }
+
+    // column starts: exclusive scan of the column counts (w <= kSortMaxW = kBlock * kScanItems)
+    {
+        int v[kScanItems];
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const int c = t * kScanItems + k;
+            v[k] = c < w ? hist[c] : 0;
+        }
+        __syncthreads();
+        block_exclusive_scan(v, lds_wave);
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const int c = t * kScanItems + k;
+            if (c < w) hist[c] = v[k];
+        }
+        if (t == 0) hist[w] = n;
+    }
+    __syncthreads();
+    for (int e = t; e < n; e += kBlock) {
+        const int c = e_col[e];
+        order[hist[c] + atomicAdd(&fillc[c], 1)] = (uint16_t)e;
+    }
+
+    // The tile's first slot: the sum of the counts of all earlier tiles, read by every thread at
+    // once (a tile whose count is not published yet is read again, a bounded number of times).
+    {
+        long long part = 0;
+        bool bad = false;
+        for (int j = t; j < tile && !bad; j += kBlock) {
+            for (int spins = 0;; ++spins) {
+                const unsigned long long st = tile_load(&a.tstate[j]);
+                if ((unsigned)(st >> 32) == (tag | 1u)) {
+                    part += (unsigned)st;
+                    break;
+                }
+                if (spins >= kSpinLimit) {
+                    bad = true;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        if (bad) s_bad = 1;
+        part = wave_sum_ll(part);
+        if ((t & 63) == 0) red[t >> 6] = part;
+    }
+    __syncthreads();
+    if (t == 0) {
+        long long prefix = 0;
+        for (int k = 0; k < kBlock / 64; ++k) prefix += red[k];
+        if (prefix > N) {
+            s_bad = 1;
+            prefix = N;
+        }
+        s_base = (int)prefix;
+        if (tile == a.ntiles - 1 && prefix + n != N) s_bad = 1;  // a particle went missing
+    }
+    __syncthreads();
+    const int base = s_base;
+
+    // cell starts of the tile's cells, then every particle to its slot: cell start + the number of
+    // cell mates with a smaller caller index (the general path's canonical order)
+    for (int c = t; c < w; c += kBlock) a.cs_new[y * W + c0 + c] = clamp_slot(base + hist[c], N);
+    if (tile == a.ntiles - 1 && t == 0) a.cs_new[H * W] = clamp_slot(base + n, N);
+    for (int p = t; p < n; p += kBlock) {
+        const int e = order[p];
+        const int c = e_col[e], c_s = hist[c], c_e = hist[c + 1], mine = e_id[e];
+        int r = 0;
+        for (int q = c_s; q < c_e; ++q) r += e_id[order[q]] < mine;
+        const int slot = base + c_s + r;
+        if (slot >= N) {
+            s_bad = 1;
+            continue;
+        }
+        const int src = e_old[e];
+        a.out.pos[slot] = a.in.pos[src];
+        a.out.th[slot] = a.in.th[src];
+        a.out.U[slot] = a.in.U[src];
+        a.out.id[slot] = mine;
+        if ((unsigned)mine < (unsigned)N) a.inv[mine] = slot;
+    }
+    __syncthreads();
+    if (t == 0 && s_bad) atomicOr(a.fail, 1);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -637,13 +869,13 @@ __device__ __forceinline__ P4 chunk_record(const StageArgs& a, int i, int k, int
 }
 
 template <int MODEL, int REP>
-__device__ __forceinline__ void gather_forces_lds(const StageArgs& a, int i, const P4& me, const ListHead& h,
-                                                  double& fx, double& fy, double& tq)
+__device__ __forceinline__ void gather_forces_lds(const StageArgs& a, int i, const P4& me, double mc, double ms,
+                                                  const ListHead& h, double& fx, double& fy, double& tq)
 {
     const int cnt = h.cnt;
     double s1 = 0, s2 = 0;
-    fx = me.c * a.P.v0;  // Reset: f = v (speed v0), particle.h:559-564
-    fy = me.s * a.P.v0;
+    fx = mc * a.P.v0;  // Reset: f = v (speed v0), particle.h:559-564
+    fy = ms * a.P.v0;
 #pragma unroll
     for (int k0 = 0; k0 < kPfList; k0 += 4) {
         if (k0 >= cnt) break;
@@ -652,7 +884,7 @@ __device__ __forceinline__ void gather_forces_lds(const StageArgs& a, int i, con
         for (int u = 0; u < 4; ++u) q[u] = chunk_record(a, i, k0 + u, k0 + u < cnt ? h.l[k0 + u] : 0);
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (k0 + u < cnt) spp::pair_accumulate<MODEL, REP>(a.P, me, q[u], fx, fy, s1, s2);
+            if (k0 + u < cnt) spp::pair_accumulate<MODEL, REP>(a.P, me, mc, ms, q[u], fx, fy, s1, s2);
     }
     for (int k = kPfList; k < cnt; k += 4) {
         int l[4];
@@ -663,7 +895,7 @@ __device__ __forceinline__ void gather_forces_lds(const StageArgs& a, int i, con
         for (int u = 0; u < 4; ++u) q[u] = chunk_record(a, i, k + u, l[u]);
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (k + u < cnt) spp::pair_accumulate<MODEL, REP>(a.P, me, q[u], fx, fy, s1, s2);
+            if (k + u < cnt) spp::pair_accumulate<MODEL, REP>(a.P, me, mc, ms, q[u], fx, fy, s1, s2);
     }
     tq = spp::drift_torque<MODEL>(a.P, s1, s2);
 }
@@ -671,13 +903,13 @@ __device__ __forceinline__ void gather_forces_lds(const StageArgs& a, int i, con
 // Pair sums of slot i over its list: LDS offsets (global slot indices where marked) for a staged
 // chunk, global slot indices otherwise (FB).
 template <int MODEL, int REP, bool FB>
-__device__ __forceinline__ void gather_forces(const StageArgs& a, int i, const P4& me, double& fx,
-                                              double& fy, double& tq)
+__device__ __forceinline__ void gather_forces(const StageArgs& a, int i, const P4& me, double mc, double ms,
+                                              double& fx, double& fy, double& tq)
 {
     const int cnt = a.ncount[i];
     double s1 = 0, s2 = 0;
-    fx = me.c * a.P.v0;  // Reset: f = v (speed v0), particle.h:559-564
-    fy = me.s * a.P.v0;
+    fx = mc * a.P.v0;  // Reset: f = v (speed v0), particle.h:559-564
+    fy = ms * a.P.v0;
     for (int k = 0; k < cnt; k += 4) {
         P4 q[4];
         if (!FB) {
@@ -700,7 +932,7 @@ __device__ __forceinline__ void gather_forces(const StageArgs& a, int i, const P
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (k + u < cnt) spp::pair_accumulate<MODEL, REP>(a.P, me, q[u], fx, fy, s1, s2);
+            if (k + u < cnt) spp::pair_accumulate<MODEL, REP>(a.P, me, mc, ms, q[u], fx, fy, s1, s2);
     }
     tq = spp::drift_torque<MODEL>(a.P, s1, s2);
 }
@@ -750,26 +982,28 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int s, const Chun
         double x = 0, y = 0;
         if (live) {
             P4 me = a.gather[i];
+            double mc, ms;
+            spp::heading_unit(me.z, mc, ms);
             double dn = 0, th_in = 0, thold = 0;
             double2 u = make_double2(0, 0);
             P4 kk{0, 0, 0, 0};
             if (STAGE <= 1) {
                 int pid = a.id[i];
                 if (a.id_out) a.id_out[i] = pid;
-                th_in = a.th_in[i];
+                th_in = me.z;
                 if (INTEG == SPP_EULER) u = a.U_in[i];
                 dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)pid, noise_step(a));
             } else {
-                thold = a.th0[i];
+                thold = me.w;  // carried by the middle stages' records
                 dn = a.dth[i];
                 u = a.U_in[i];
                 if (INTEG == SPP_RK4) kk = a.kacc[i];
             }
             double fx, fy, tq;
             if (FB)
-                gather_forces<MODEL, REP, true>(a, i, me, fx, fy, tq);
+                gather_forces<MODEL, REP, true>(a, i, me, mc, ms, fx, fy, tq);
             else
-                gather_forces_lds<MODEL, REP>(a, i, me, h, fx, fy, tq);
+                gather_forces_lds<MODEL, REP>(a, i, me, mc, ms, h, fx, fy, tq);
             double th;
             if (STAGE <= 1) {
                 if (INTEG == SPP_EULER) {
@@ -793,7 +1027,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int s, const Chun
             } else {
                 double h = (INTEG == SPP_RK4 && STAGE == 2) ? P.half_dt : P.dt;
                 if (INTEG == SPP_RK4 && STAGE == 4) {
-                    th = thold + (kk.c + tq) * P.dt6;
+                    th = thold + (kk.z + tq) * P.dt6;
                     th += dn;
                     u.x = u.x + (kk.x + fx) * P.dt6;
                     u.y = u.y + (kk.y + fy) * P.dt6;
@@ -802,16 +1036,15 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int s, const Chun
                     th += (INTEG == SPP_RK4 && STAGE == 2) ? dn / 2 : dn;
                     u.x = u.x + fx * h;
                     u.y = u.y + fy * h;
-                    if (INTEG == SPP_RK4) a.kacc[i] = P4{kk.x + fx * 2, kk.y + fy * 2, kk.c + tq * 2, 0.0};
+                    if (INTEG == SPP_RK4) a.kacc[i] = P4{kk.x + fx * 2, kk.y + fy * 2, kk.z + tq * 2, 0.0};
                 }
                 th = spp::wrap_angle(th);
                 if (INTEG == SPP_RK2 || STAGE == 4) a.U_out[i] = u;
                 x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
                 y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
             }
-            double sn, cs;
-            spp::sincos_heading(th, &sn, &cs);
-            a.out[i] = P4{x, y, cs, sn};
+            // middle stages carry the step's starting heading for the later stages
+            a.out[i] = P4{x, y, th, final_stage ? 0.0 : (STAGE <= 1 ? th_in : thold)};
             a.th_out[i] = th;
         }
         check_disp(a, P, s, live, i, x, y, final_stage ? a.kstep + 1 : a.kstep, dmax);
@@ -840,8 +1073,8 @@ __global__ __launch_bounds__(kChunk) void k_stage(StageArgs a)
 // ------------------------------------------------------------------------------------------------
 // Slot-parallel variant: one lane per particle over the whole slot range, neighbour records
 // gathered from L2 through a 32-bit slot-index list.  Loads of lanes past their list length are
-// masked off (they would cost texture-address cycles for nothing); with SPLIT the heading half of
-// a record is requested only for neighbours inside the interaction cutoff.
+// masked off (they would cost texture-address cycles for nothing); with SPLIT the heading of a
+// record is requested only for neighbours inside the interaction cutoff.
 // ------------------------------------------------------------------------------------------------
 
 struct NlistFlatArgs {
@@ -975,13 +1208,13 @@ __global__ __launch_bounds__(kBlock) void k_nlist_flat(NlistFlatArgs a)
 // time is one wavefront's chain of loads, take 8; large ones 4 (fewer VGPRs, same speed or better).
 template <int MODEL, int REP, bool SPLIT, int GU = 4>
 __device__ __forceinline__ void gather_forces_flat(const StageArgs& a, const Phys& P, int i, const P4& me,
-                                                   double& fx, double& fy, double& tq)
+                                                   double mc, double ms, double& fx, double& fy, double& tq)
 {
     const int cnt = a.ncount[i];
     const int* nb = a.nbr32 + i;
     double s1 = 0, s2 = 0;
-    fx = me.c * P.v0;  // Reset: f = v (speed v0), particle.h:559-564
-    fy = me.s * P.v0;
+    fx = mc * P.v0;  // Reset: f = v (speed v0), particle.h:559-564
+    fy = ms * P.v0;
     for (int k = 0; k < cnt; k += GU) {
         int j[GU];
 #pragma unroll
@@ -994,7 +1227,7 @@ __device__ __forceinline__ void gather_forces_flat(const StageArgs& a, const Phy
                 if (k + u < cnt) q[u] = a.gather[j[u]];
 #pragma unroll
             for (int u = 0; u < GU; ++u)
-                if (k + u < cnt) spp::pair_accumulate<MODEL, REP>(P, me, q[u], fx, fy, s1, s2);
+                if (k + u < cnt) spp::pair_accumulate<MODEL, REP>(P, me, mc, ms, q[u], fx, fy, s1, s2);
         } else {
             P4 q[GU];
 #pragma unroll
@@ -1017,14 +1250,10 @@ __device__ __forceinline__ void gather_forces_flat(const StageArgs& a, const Phy
             }
 #pragma unroll
             for (int u = 0; u < GU; ++u)
-                if (in[u]) {
-                    double2 cs = *reinterpret_cast<const double2*>(&a.gather[j[u]].c);
-                    q[u].c = cs.x;
-                    q[u].s = cs.y;
-                }
+                if (in[u]) q[u].z = a.gather[j[u]].z;
 #pragma unroll
             for (int u = 0; u < GU; ++u)
-                if (in[u]) spp::pair_accumulate<MODEL, REP>(P, me, q[u], fx, fy, s1, s2);
+                if (in[u]) spp::pair_accumulate<MODEL, REP>(P, me, mc, ms, q[u], fx, fy, s1, s2);
         }
     }
     tq = spp::drift_torque<MODEL>(P, s1, s2);
@@ -1038,25 +1267,27 @@ __device__ __forceinline__ void stage_particle(const StageArgs& a, const Phys& P
                                                double& y)
 {
     P4 me = a.gather[i];
+    double mc, ms;  // v = (cos, sin) of the stage input's heading
+    spp::heading_unit(me.z, mc, ms);
     double dn = 0, th_in = 0, thold = 0;
     double2 u = make_double2(0, 0);
     P4 kk{0, 0, 0, 0};
     if (STAGE <= 1) {
         int pid = a.id[i];
         if (a.id_out) a.id_out[i] = pid;
-        th_in = a.th_in[i];
+        th_in = me.z;
         if (INTEG == SPP_EULER) u = a.U_in[i];
         // the stream is addressed by the particle's index within its system, so systems with
         // equal seeds (Lyapunov clones) draw identical noise
         dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)(pid - pid_off), noise_step(a));
     } else {
-        thold = a.th0[i];
+        thold = me.w;  // theta_old, carried by the middle stages' records
         dn = a.dth[i];
         u = a.U_in[i];
         if (INTEG == SPP_RK4) kk = a.kacc[i];
     }
     double fx, fy, tq;
-    gather_forces_flat<MODEL, REP, SPLIT, GU>(a, P, i, me, fx, fy, tq);
+    gather_forces_flat<MODEL, REP, SPLIT, GU>(a, P, i, me, mc, ms, fx, fy, tq);
     double th;
     if (STAGE <= 1) {
         if (INTEG == SPP_EULER) {
@@ -1080,7 +1311,7 @@ __device__ __forceinline__ void stage_particle(const StageArgs& a, const Phys& P
     } else {
         double h = (INTEG == SPP_RK4 && STAGE == 2) ? P.half_dt : P.dt;
         if (INTEG == SPP_RK4 && STAGE == 4) {
-            th = thold + (kk.c + tq) * P.dt6;
+            th = thold + (kk.z + tq) * P.dt6;
             th += dn;
             u.x = u.x + (kk.x + fx) * P.dt6;
             u.y = u.y + (kk.y + fy) * P.dt6;
@@ -1089,20 +1320,18 @@ __device__ __forceinline__ void stage_particle(const StageArgs& a, const Phys& P
             th += (INTEG == SPP_RK4 && STAGE == 2) ? dn / 2 : dn;
             u.x = u.x + fx * h;
             u.y = u.y + fy * h;
-            if (INTEG == SPP_RK4) spp::store_out(a.stream, &a.kacc[i], kk.x + fx * 2, kk.y + fy * 2, kk.c + tq * 2, 0.0);
+            if (INTEG == SPP_RK4) spp::store_out(a.stream, &a.kacc[i], kk.x + fx * 2, kk.y + fy * 2, kk.z + tq * 2, 0.0);
         }
         th = spp::wrap_angle(th);
         if (INTEG == SPP_RK2 || STAGE == 4) spp::store_out(a.stream, &a.U_out[i], u);
         x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
         y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
     }
-    double sn, cs;
-    spp::sincos_heading(th, &sn, &cs);
-    spp::store_out(a.stream, &a.out[i], x, y, cs, sn);
-    // the angle of a middle RK stage is never read: every later stage restarts from th0 and takes
-    // the stage heading from the (cos, sin) of the record above, so only a step's final angle is
-    // stored (8 B per particle less in RK2 stage 1 and RK4 stages 1-3)
+    // The record holds the stage's heading angle; a middle RK stage's record also carries the
+    // step's starting heading (theta_old) for the later stages.  The angle array of the state set
+    // is written by a step's final stage only (the middle stages' angles live in their records).
     constexpr bool kFinal = (INTEG == SPP_EULER) || (INTEG == SPP_RK2 && STAGE == 2) || (STAGE == 4);
+    spp::store_out(a.stream, &a.out[i], x, y, th, kFinal ? 0.0 : (STAGE <= 1 ? th_in : thold));
     if (kFinal) spp::store_out(a.stream, &a.th_out[i], th);
 }
 
@@ -1134,8 +1363,9 @@ __global__ __launch_bounds__(kBlock) void k_interact_flat(StageArgs a, P4* force
     Phys P = a.P;
     if (BATCH) apply_sys(P, a.sys_tab[a.sys_of[i]]);
     P4 me = a.gather[i];
-    double fx, fy, tq;
-    gather_forces_flat<MODEL, REP, false>(a, P, i, me, fx, fy, tq);
+    double mc, ms, fx, fy, tq;
+    spp::heading_unit(me.z, mc, ms);
+    gather_forces_flat<MODEL, REP, false>(a, P, i, me, mc, ms, fx, fy, tq);
     force[i] = P4{fx, fy, tq, 0.0};
 }
 
@@ -1181,7 +1411,7 @@ __global__ __launch_bounds__(kBlock) void k_move(MoveArgs a)
         double dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)(a.cur.id[i] - pid_off), a.step);
         a.dth[i] = dn;
         if (a.integ == SPP_EULER) {
-            th = th_in + P.dt * (f.c);
+            th = th_in + P.dt * (f.z);
             th += dn;
             u.x += f.x * P.dt;
             u.y += f.y * P.dt;
@@ -1191,7 +1421,7 @@ __global__ __launch_bounds__(kBlock) void k_move(MoveArgs a)
         } else {
             a.old[i] = P4{u.x, u.y, th_in, 0.0};
             th = th_in;
-            th += P.half_dt * f.c;
+            th += P.half_dt * f.z;
             th += dn / 2;
             th = spp::wrap_angle(th);
             if (a.integ == SPP_RK4) a.kacc[i] = f;
@@ -1203,19 +1433,19 @@ __global__ __launch_bounds__(kBlock) void k_move(MoveArgs a)
         double dn = a.dth[i];
         if (a.integ == SPP_RK4 && a.stage == 4) {
             P4 k = a.kacc[i];
-            th = o.c + (k.c + f.c) * P.dt6;
+            th = o.z + (k.z + f.z) * P.dt6;
             th += dn;
             u.x = o.x + (k.x + f.x) * P.dt6;
             u.y = o.y + (k.y + f.y) * P.dt6;
         } else {
             double h = (a.integ == SPP_RK4 && a.stage == 2) ? P.half_dt : P.dt;
-            th = o.c + h * f.c;
+            th = o.z + h * f.z;
             th += (a.integ == SPP_RK4 && a.stage == 2) ? dn / 2 : dn;
             u.x = o.x + f.x * h;
             u.y = o.y + f.y * h;
             if (a.integ == SPP_RK4) {
                 P4 k = a.kacc[i];
-                a.kacc[i] = P4{k.x + f.x * 2, k.y + f.y * 2, k.c + f.c * 2, 0.0};
+                a.kacc[i] = P4{k.x + f.x * 2, k.y + f.y * 2, k.z + f.z * 2, 0.0};
             }
         }
         th = spp::wrap_angle(th);
@@ -1223,37 +1453,21 @@ __global__ __launch_bounds__(kBlock) void k_move(MoveArgs a)
         x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
         y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
     }
-    double sn, cs;
-    spp::sincos_heading(th, &sn, &cs);
-    a.cur.pos[i] = P4{x, y, cs, sn};
+    a.cur.pos[i] = P4{x, y, th, 0.0};
     a.cur.th[i] = th;
 }
 
 #include "abp_kernels.hpp"
-
-// Headings -> (cos, sin) after a host upload, computed by the same routine as the stages (the
-// library's for uploaded angles beyond its range).
-__global__ __launch_bounds__(kBlock) void k_init_heading(int N, P4* __restrict__ pos,
-                                                         const double* __restrict__ th)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    double sn, cs;
-    const double t = th[i];
-    if (fabs(t) <= 524288.0)
-        spp::sincos_heading(t, &sn, &cs);
-    else
-        sincos(t, &sn, &cs);
-    pos[i].c = cs;
-    pos[i].s = sn;
-}
 
 // Scatter caller-order values into slot order: dst[slot] = src[id[slot]].
 __global__ void k_gather_by_id(int N, const int* __restrict__ id, const double* __restrict__ src,
                                double* __restrict__ dst)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N) dst[i] = src[id[i]];
+    if (i < N) {
+        const int j = id[i];
+        dst[i] = (unsigned)j < (unsigned)N ? src[j] : 0.0;
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_copy_set(int N, SetPtrs src, SetPtrs dst)
@@ -1273,10 +1487,7 @@ __global__ __launch_bounds__(kBlock) void k_polar(int N, const P4* __restrict__ 
     __shared__ double sx[kBlock / 64], sy[kBlock / 64];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double vx = 0, vy = 0;
-    if (i < N) {
-        vx = pos[i].c;
-        vy = pos[i].s;
-    }
+    if (i < N) spp::heading_unit(pos[i].z, vx, vy);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         vx += __shfl_xor(vx, o);
@@ -1305,8 +1516,10 @@ __global__ __launch_bounds__(kBlock) void k_polar_sys(const SysInfo* __restrict_
     const SysInfo& S = sys[blockIdx.x];
     double vx = 0, vy = 0;
     for (int i = S.pid_off + threadIdx.x; i < S.pid_off + S.n; i += kBlock) {
-        vx += pos[i].c;
-        vy += pos[i].s;
+        double c, s;
+        spp::heading_unit(pos[i].z, c, s);
+        vx += c;
+        vy += s;
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -1356,7 +1569,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_hist(int N, const P4* __restric
 
 // Reduces one interval's per-block partials into the host-mapped report (one block) and clears
 // the per-block bad steps for the slot's next interval.
-__global__ __launch_bounds__(kBlock) void k_report(int nblk, SlotBufs b, int with_list, Report* out)
+__global__ __launch_bounds__(kBlock) void k_report(int nblk, SlotBufs b, int with_list, int* sort_fail, Report* out)
 {
     __shared__ int r_bad[kBlock / 64], r_len[kBlock / 64];
     __shared__ long long r_nnz[kBlock / 64];
@@ -1404,6 +1617,8 @@ __global__ __launch_bounds__(kBlock) void k_report(int nblk, SlotBufs b, int wit
         out->max_len = len;
         out->nnz = nnz;
         out->maxdisp2 = mx;
+        out->sort_fail = *sort_fail;
+        *sort_fail = 0;  // the next interval's rebuild starts clear
         __threadfence_system();
     }
 }
@@ -1556,7 +1771,9 @@ struct ProfRec {
 
 // One enqueued interval awaiting verification.
 struct Interval {
-    int start_set;   // set holding the state at the interval's start (kept untouched)
+    int anchor_set;  // set a rollback returns to, kept untouched until the interval is verified
+    int anchor_since;// steps since the last build there
+    int start_set;   // set holding the state the interval's first step reads (after its rebuild)
     int m;           // steps
     int slot;        // flag slot
     int64_t step0;
@@ -1589,7 +1806,12 @@ struct spp_engine {
     hipStream_t stream = nullptr;
     int device = 0;
 
-    SetPtrs set[3] = {};
+    // State sets.  A rollback returns to an interval's anchor: the state its first step reads (three
+    // sets), or, for engines with the tile sort, the state before its rebuild -- a failed tile sort
+    // leaves no usable rebuilt state.  Two intervals may be in flight, each keeping its anchor
+    // untouched until the host has verified it, so the tile engines hold a fourth set.
+    SetPtrs set[kSets] = {};
+    int nsets = 3;
     int cur = 0;
     P4 *pos1 = nullptr, *pos2 = nullptr;
     double *th1 = nullptr, *th2 = nullptr;
@@ -1610,6 +1832,18 @@ struct spp_engine {
         *block_sums = nullptr;
     int* inv = nullptr;          // caller index -> slot of the current set
     int* perm2 = nullptr;        // the scatter's order, canonicalised within cells
+    // tile path of the rebuild (k_sort_tiles): eligible engines, the ping-pong cell starts (the
+    // launch reads the previous build's while it writes the new ones), the look-back states and
+    // their epoch
+    bool sort_ok = false;        // single box, slot kernels, >= 3 cells per side, no graphs
+    bool order_valid = false;    // the current set's slot order is the last build's (cell_start)
+    bool force_general = false;  // the next rebuild takes the general path (after a tile failure)
+    int sort_seg = 0, sort_segw = 0, sort_ntiles = 0;
+    int* cell_start_alt = nullptr;
+    unsigned long long* tile_state = nullptr;
+    unsigned sort_epoch = 0;
+    int* sort_fail = nullptr;
+    int64_t tile_rebuilds = 0, tile_fallbacks = 0;
     AbpConst abp{};
     SlotBufs slotb[2] = {};      // per-block partials of the (at most two) intervals in flight
     Report* hrep = nullptr;      // host-mapped reports, 2 slots
@@ -1824,12 +2058,44 @@ int launch_nlist(spp_engine* e, int slot)
 
 int launch_report(spp_engine* e, int slot, bool with_list)
 {
-    return launch(e, "report", k_report, 1, e->nstat, e->slotb[slot], with_list ? 1 : 0, e->drep + slot);
+    return launch(e, "report", k_report, 1, e->nstat, e->slotb[slot], with_list ? 1 : 0, e->sort_fail, e->drep + slot);
 }
 
-int launch_rebuild(spp_engine* e, int dst, int slot, bool aux)
+// tile: the caller reads the rebuild's failure flag from the interval report (only the interval
+// scheduler does); every other caller takes the general sort, which needs no premise on the order.
+int launch_rebuild(spp_engine* e, int dst, int slot, bool aux, bool tile)
 {
     int N = e->N, nk = e->nkeys, src = e->cur;
+    if (tile && !aux && e->sort_ok && e->order_valid && !e->force_general) {
+        // tile path: one launch from the current order to the new sorted set
+        SortArgs sa{};
+        sa.N = N;
+        sa.G = e->G;
+        sa.seg = e->sort_seg;
+        sa.segw = e->sort_segw;
+        sa.ntiles = e->sort_ntiles;
+        sa.in = e->set[src];
+        sa.out = e->set[dst];
+        sa.inv = e->inv;
+        sa.cs_old = e->cell_start;
+        sa.cs_new = e->cell_start_alt;
+        sa.tstate = e->tile_state;
+        sa.fail = e->sort_fail;
+        e->sort_epoch = e->sort_epoch % ((1u << 30) - 1) + 1;  // 1 .. 2^30 - 1, never the cleared 0
+        sa.epoch = e->sort_epoch;
+        TRY(launch(e, "sort_tiles", k_sort_tiles, e->sort_ntiles, sa));
+        std::swap(e->cell_start, e->cell_start_alt);
+        e->tile_rebuilds++;
+        e->cur = dst;
+        TRY(launch_nlist(e, slot));
+        e->persist_valid = false;
+        e->rebuilds++;
+        e->steps_since_build = 0;
+        e->list_valid = true;
+        return SPP_OK;
+    }
+    e->force_general = false;
+    e->order_valid = true;  // the general path sorts any order
     // the arrival ranks live in perm2 until the per-cell ordering overwrites it
     TRY(launch(e, "bin", k_bin, nblocks(N), N, e->G, (const P4*)e->set[src].pos, e->key,
                e->cell_count, e->perm2, (const SysInfo*)e->dsys, (const int*)e->sys_of));
@@ -1875,18 +2141,26 @@ int launch_rebuild(spp_engine* e, int dst, int slot, bool aux)
     return SPP_OK;
 }
 
-int free_set_other_than(int a, int b)
+// Host-side guard: device ids index caller arrays in the getters, so they are checked first.
+int check_ids(spp_engine* e, const std::vector<int>& id)
 {
-    for (int k = 0; k < 3; ++k)
-        if (k != a && k != b) return k;
+    for (int v : id)
+        if ((unsigned)v >= (unsigned)e->N) return fail(e, SPP_ERR_STATE, "device particle id %d out of range", v);
+    return SPP_OK;
+}
+
+int free_set_other_than(const spp_engine* e, int a, int b, int c = -1)
+{
+    for (int k = 0; k < e->nsets; ++k)
+        if (k != a && k != b && k != c) return k;
     return -1;
 }
 
 // Synchronous rebuild with a capacity check (used outside the interval scheduler).
 int rebuild_sync(spp_engine* e, bool aux)
 {
-    int dst = free_set_other_than(e->cur, -1);
-    TRY(launch_rebuild(e, dst, 0, aux));
+    int dst = free_set_other_than(e, e->cur, -1);
+    TRY(launch_rebuild(e, dst, 0, aux, false));
     for (;;) {
         TRY(launch(e, "fill", k_fill<double>, nblocks(e->nstat), e->slotb[0].blkmax, 0.0, e->nstat));
         TRY(launch_report(e, 0, true));
@@ -2015,16 +2289,23 @@ int check_ready(spp_engine* e)
 int interval_body(spp_engine* e, int m, bool rebuild, int protect, int slot, Interval& iv)
 {
     // slot's per-block bad steps: INT_MAX from creation or from the report that closed its last use
+    // The anchor: the state before the rebuild on tile engines (a failed tile sort is redone from
+    // there), the state after it otherwise (a rollback then rebuilds that already sorted state).
+    const int origin = e->cur;
+    const int64_t origin_since = e->steps_since_build;
     if (rebuild) {
-        int dst = free_set_other_than(e->cur, protect);
-        TRY(launch_rebuild(e, dst, slot, false));
+        int dst = free_set_other_than(e, e->cur, protect);
+        TRY(launch_rebuild(e, dst, slot, false, true));
     }
     iv.start_set = e->cur;
     iv.step0 = e->step;
     iv.since0 = e->steps_since_build;
     iv.t0 = e->t;
+    iv.anchor_set = e->sort_ok ? origin : e->cur;
+    iv.anchor_since = (int)std::min<int64_t>(e->sort_ok ? origin_since : e->steps_since_build, INT_MAX);
     e->graph_base = e->step;
-    int W = free_set_other_than(e->cur, protect);
+    // the output set spares the anchor and the front interval's anchor
+    int W = free_set_other_than(e, e->cur, protect, iv.anchor_set);
     for (int k = 0; k < m; ++k) {
         TRY(launch_step(e, e->cur, k == 0 ? W : e->cur, k, slot, k == m - 1));
         if (k == 0) e->inj_pending = false;
@@ -2097,7 +2378,22 @@ int enqueue_interval(spp_engine* e, int m, bool rebuild, int protect, Interval& 
     return SPP_OK;
 }
 
-int predict_m(spp_engine* e, int remaining)
+// Whether the next interval starts with a rebuild.  Displacement policy: always when the list does
+// not describe the current state; otherwise only once the steps since the last build reach the
+// predicted safe length (0.85 of the estimated crossing).  An interval that continues on the old
+// list is still checked step by step against the build positions, and a crossing rolls it back to
+// a rebuild (run_intervals), so continuing is safe; it spares a rebuild at the start of every call
+// of a short-call loop (Box::One_Step).
+bool rebuild_due(const spp_engine* e)
+{
+    if (!e->list_valid) return true;
+    if (e->p.rebuild == SPP_REBUILD_PERIOD) return e->steps_since_build >= e->p.rebuild_period;
+    if (e->steps_since_build == 0) return false;
+    if (e->T_cross <= 0) return true;  // no displacement history yet
+    return 0.85 * e->T_cross - (double)e->steps_since_build < 1.0;
+}
+
+int predict_m(spp_engine* e, int remaining, bool rebuild)
 {
     int m;
     if (e->force_m > 0) {
@@ -2105,12 +2401,14 @@ int predict_m(spp_engine* e, int remaining)
         e->force_m = 0;
     } else if (e->p.rebuild == SPP_REBUILD_PERIOD) {
         // an interval that starts with a rebuild runs a whole period
-        const bool due = !e->list_valid || e->steps_since_build >= e->p.rebuild_period;
-        m = due ? e->p.rebuild_period : std::max(1, e->p.rebuild_period - (int)e->steps_since_build);
+        m = rebuild ? e->p.rebuild_period : std::max(1, e->p.rebuild_period - (int)e->steps_since_build);
     } else if (e->T_cross <= 0) {
         m = 4;  // no history yet: a short interval to measure the displacement rate
-    } else {
+    } else if (rebuild) {
         m = std::max(1, (int)(0.85 * e->T_cross));
+    } else {
+        // the rest of the current list's predicted safe length
+        m = std::max(1, (int)(0.85 * e->T_cross - (double)e->steps_since_build));
     }
     return std::min(std::min(m, remaining), 4096);
 }
@@ -2123,10 +2421,10 @@ int run_intervals(spp_engine* e, int steps)
     bool period = e->p.rebuild == SPP_REBUILD_PERIOD;
     while (to_enqueue > 0 || !q.empty()) {
         if (to_enqueue > 0 && q.size() < 2) {
-            int m = predict_m(e, to_enqueue);
-            bool rebuild = !e->list_valid || e->steps_since_build > 0;
-            if (period) rebuild = !e->list_valid || e->steps_since_build >= e->p.rebuild_period;
-            int protect = q.empty() ? -1 : q.front().start_set;
+            // a replay after a rollback (force_m) restarts from the rebuild it rolled back to
+            bool rebuild = rebuild_due(e);
+            int m = predict_m(e, to_enqueue, rebuild);
+            int protect = q.empty() ? -1 : q.front().anchor_set;
             Interval iv{};
             TRY(enqueue_interval(e, m, rebuild, protect, iv));
             q.push_back(iv);
@@ -2137,13 +2435,14 @@ int run_intervals(spp_engine* e, int steps)
         HIPC(e, hipEventSynchronize(iv.done));
         e->host_syncs++;
         const Report f = e->hrep[iv.slot];
-        bool overflow = iv.rebuilt && f.max_len > e->cap;
+        const bool sort_fail = iv.rebuilt && f.sort_fail != 0;
+        bool overflow = !sort_fail && iv.rebuilt && f.max_len > e->cap;
         int bad = period ? INT_MAX : f.bad_step;
-        if (iv.rebuilt) {
+        if (iv.rebuilt && !sort_fail) {
             e->max_nbrs = f.max_len;
             e->mean_nbrs = (double)f.nnz / e->N;
         }
-        if (!overflow && bad >= iv.m) {
+        if (!sort_fail && !overflow && bad >= iv.m) {
             q.pop_front();
             if (!period && iv.rebuilt) {
                 // linear extrapolation of the largest displacement to the trigger
@@ -2164,14 +2463,24 @@ int run_intervals(spp_engine* e, int steps)
         int redo = 0;
         for (auto& x : q) redo += x.m;
         q.clear();
-        e->cur = iv.start_set;
+        // back to the interval's anchor; the redo rebuilds from there
+        e->cur = iv.anchor_set;
         e->step = iv.step0;
         e->t = iv.t0;
-        e->steps_since_build = iv.since0;
+        e->steps_since_build = iv.anchor_since;
         e->rollbacks++;
         if (e->p.noise_mode == SPP_NOISE_INJECTED) e->inj_pending = true;  // replay uses it again
         to_enqueue += redo;
         e->list_valid = false;  // the device list may belong to a later interval
+        e->order_valid = false; // and the cell starts to a later build
+        if (sort_fail) {
+            // the tile path could not sort (a particle moved farther than a cell, or a tile
+            // overflowed): same rebuild point on the general path; after three, general for good
+            e->force_general = true;
+            if (++e->tile_fallbacks >= 3) e->sort_ok = false;
+            e->force_m = iv.m;
+            continue;
+        }
         if (overflow) {
             // same rebuild point (the interval started with one), larger capacity
             e->cap = ((f.max_len + 7) / 8) * 8 + 8;
@@ -2233,7 +2542,7 @@ int run_persist(spp_engine* e, int steps)
         e->persist_rebuild_base = e->rebuilds;
     }
     const long long target = e->step + steps;
-    const int spare = free_set_other_than(e->cur, -1);
+    const int spare = free_set_other_than(e, e->cur, -1);
     PersistFn fn = pick_persist(e->p.model, e->p.repulsion, e->p.integrator);
     for (;;) {
         persist::Args a{};
@@ -2559,6 +2868,8 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
         return fail(nullptr, SPP_ERR_UNSUPPORTED, "the persistent kernel draws its noise from the stream");
     if (p->rebuild == SPP_REBUILD_PERIOD && p->rebuild_period <= 0)
         return fail(nullptr, SPP_ERR_ARG, "rebuild_period must be positive");
+    if (p->rebuild_sort != SPP_REBUILD_SORT_AUTO && p->rebuild_sort != SPP_REBUILD_SORT_GENERAL)
+        return fail(nullptr, SPP_ERR_ARG, "rebuild_sort %d", p->rebuild_sort);
     double rcut = max_cutoff(*p);
     if (p->r_list < rcut) return fail(nullptr, SPP_ERR_ARG, "r_list %g below cutoff %g", p->r_list, rcut);
 
@@ -2627,6 +2938,21 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
     e->deep = deep;
     e->persist = persist;
     e->nstat = std::max(e->chunk_grid, e->nblk);
+    // Tile path of the rebuild: single boxes on the slot kernels (the LDS variant needs the general
+    // path's chunk table; captured graphs would freeze the ping-pong of the cell starts).  A tile is
+    // about kSortTarget particles of one cell row, at most kSortMaxW cells.
+    e->sort_ok = !systems && !persist && p->rebuild_sort == SPP_REBUILD_SORT_AUTO &&
+                 e->variant != SPP_KERNEL_STRIP_LDS && !p->graph_intervals;
+    if (e->sort_ok) {
+        const int W = e->G.ncx;
+        const double per_cell = (double)e->N / ((double)W * e->G.ncy);
+        int segw = (int)std::lround(kSortTarget / std::max(per_cell, 1e-9));
+        segw = std::max(1, std::min(std::min(segw, W), kSortMaxW));
+        static_assert(kSortCap <= 65536 && kSortMaxW <= 65536, "16-bit tile indices");
+        e->sort_segw = segw;
+        e->sort_seg = (W + segw - 1) / segw;
+        e->sort_ntiles = e->sort_seg * e->G.ncy;
+    }
     // the LDS stage kernels read list rows 0..kPfList-1 unconditionally
     e->cap = std::max(kPfList, p->ell_capacity > 0 ? p->ell_capacity : 32);
     {
@@ -2695,7 +3021,8 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
     };
     size_t N = e->N;
     A(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) == hipSuccess ? SPP_OK : SPP_ERR_HIP);
-    for (int k = 0; k < 3; ++k) A(alloc_set(e, e->set[k]));
+    e->nsets = e->sort_ok ? 4 : 3;
+    for (int k = 0; k < e->nsets; ++k) A(alloc_set(e, e->set[k]));
     A(dalloc(e, &e->pos1, N));
     A(dalloc(e, &e->pos2, N));
     A(dalloc(e, &e->th1, N));
@@ -2717,6 +3044,11 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
     A(dalloc(e, &e->inv, N));
     A(dalloc(e, &e->cell_count, (size_t)e->nkeys));
     A(dalloc(e, &e->cell_start, (size_t)e->nkeys + 1));
+    if (e->sort_ok) {
+        A(dalloc(e, &e->cell_start_alt, (size_t)e->nkeys + 1));
+        A(dalloc(e, &e->tile_state, (size_t)e->sort_ntiles));
+    }
+    A(dalloc(e, &e->sort_fail, 1));
     A(dalloc(e, &e->block_sums, (size_t)e->nscan_blocks));
     for (int sl = 0; sl < 2; ++sl) {
         A(dalloc(e, &e->slotb[sl].blkbad, (size_t)e->nstat));
@@ -2762,8 +3094,14 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
             rc = SPP_ERR_HIP;
     }
     if (rc == SPP_OK) {
-        // cell counts start at zero and every rebuild's scatter returns them to zero
-        if (launch(e, "fill", k_fill<int>, nblocks(e->nkeys), e->cell_count, 0, e->nkeys) != SPP_OK)
+        // cell counts start at zero and every rebuild's scatter returns them to zero; the tile
+        // path's ticket counters and look-back states start at zero and every launch clears them
+        if (launch(e, "fill", k_fill<int>, nblocks(e->nkeys), e->cell_count, 0, e->nkeys) != SPP_OK ||
+            launch(e, "fill", k_fill<int>, 1, e->sort_fail, 0, 1) != SPP_OK)
+            rc = SPP_ERR_HIP;
+        if (rc == SPP_OK && e->sort_ok &&
+            launch(e, "fill", k_fill<unsigned long long>, (e->sort_ntiles + 255) / 256, e->tile_state, 0ull,
+                   e->sort_ntiles) != SPP_OK)
             rc = SPP_ERR_HIP;
         for (int sl = 0; sl < 2 && rc == SPP_OK; ++sl) {
             const SlotBufs& b = e->slotb[sl];
@@ -2804,12 +3142,12 @@ void spp_destroy(spp_engine* e)
 {
     if (!e) return;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
-    for (int k = 0; k < 3; ++k) free_set(e->set[k]);
+    for (int k = 0; k < kSets; ++k) free_set(e->set[k]);
     void* bufs[] = {e->pos1, e->pos2, e->th1, e->th2, e->dth, e->dth_alt, e->inj_host_order,
                     e->inj, e->kacc, e->force, e->old, e->aux_alt[0], e->aux_alt[1], e->aux_alt[2],
                     e->nbr16, e->nbr32, e->chunks, e->ncount, e->cellxy, e->build, e->key, e->perm, e->perm2, e->inv, e->cell_count,
                     e->cell_start, e->block_sums, e->dpolar, e->dsys, e->sys_of, e->dpolar_sys, e->dsys1, e->dsamples,
-                    e->pstatus};
+                    e->pstatus, e->cell_start_alt, e->tile_state, e->sort_fail};
     for (void* b : bufs) dfree(b);
     for (auto& b : e->slotb) {
         dfree(b.blkbad);
@@ -2851,6 +3189,8 @@ int spp_get_info(spp_engine* e, spp_info* info)
     info->ell_capacity = e->cap;
     info->max_neighbors = e->max_nbrs;
     info->mean_neighbors = e->mean_nbrs;
+    info->tile_rebuilds = e->tile_rebuilds;
+    info->tile_fallbacks = e->tile_fallbacks;
     return SPP_OK;
 }
 
@@ -2877,7 +3217,7 @@ int spp_set_state(spp_engine* e, const double* xw, const double* yw, const doubl
     std::vector<double2> U(n);
     std::vector<int> id(n);
     for (int i = 0; i < n; ++i) {
-        pos[i] = P4{xw[i], yw[i], 0.0, 0.0};
+        pos[i] = P4{xw[i], yw[i], theta[i], 0.0};
         U[i] = make_double2(xu ? xu[i] : xw[i], yu ? yu[i] : yw[i]);
         id[i] = i;
     }
@@ -2886,7 +3226,6 @@ int spp_set_state(spp_engine* e, const double* xw, const double* yw, const doubl
     HIPC(e, hipMemcpyAsync(s.th, theta, n * sizeof(double), hipMemcpyHostToDevice, e->stream));
     HIPC(e, hipMemcpyAsync(s.U, U.data(), n * sizeof(double2), hipMemcpyHostToDevice, e->stream));
     HIPC(e, hipMemcpyAsync(s.id, id.data(), n * sizeof(int), hipMemcpyHostToDevice, e->stream));
-    TRY(launch(e, "init_heading", k_init_heading, nblocks(n), n, s.pos, (const double*)s.th));
     e->have_state = true;
     e->unfused_stage = 0;
     e->inj_pending = false;
@@ -2908,6 +3247,7 @@ int spp_get_state(spp_engine* e, double* xw, double* yw, double* xu, double* yu,
     HIPC(e, hipMemcpyAsync(U.data(), s.U, n * sizeof(double2), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipMemcpyAsync(id.data(), s.id, n * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipStreamSynchronize(e->stream));
+    TRY(check_ids(e, id));
     for (int k = 0; k < n; ++k) {
         int i = id[k];
         if (xw) xw[i] = pos[k].x;
@@ -2942,6 +3282,7 @@ int spp_get_neighbors(spp_engine* e, int32_t* offsets, int32_t* idx, int64_t cap
     HIPC(e, hipMemcpyAsync(cnt.data(), e->ncount, N * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipMemcpyAsync(id.data(), e->set[e->cur].id, N * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipStreamSynchronize(e->stream));
+    TRY(check_ids(e, id));
     std::vector<int64_t> rowlen(N + 1, 0);
     for (int k = 0; k < N; ++k) rowlen[id[k] + 1] = cnt[k];
     for (int i = 0; i < N; ++i) rowlen[i + 1] += rowlen[i];
@@ -2981,6 +3322,7 @@ int spp_get_cells(spp_engine* e, int32_t* cell_x, int32_t* cell_y, int32_t n)
     HIPC(e, hipMemcpyAsync(cxy.data(), e->cellxy, n * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipMemcpyAsync(id.data(), e->set[e->cur].id, n * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipStreamSynchronize(e->stream));
+    TRY(check_ids(e, id));
     for (int k = 0; k < n; ++k) {
         if (cell_x) cell_x[id[k]] = cxy[k] & 0xffff;
         if (cell_y) cell_y[id[k]] = cxy[k] >> 16;
@@ -3019,11 +3361,12 @@ int spp_get_forces(spp_engine* e, double* fx, double* fy, double* tau, int32_t n
     HIPC(e, hipMemcpyAsync(f.data(), e->force, n * sizeof(P4), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipMemcpyAsync(id.data(), e->set[e->cur].id, n * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipStreamSynchronize(e->stream));
+    TRY(check_ids(e, id));
     for (int k = 0; k < n; ++k) {
         int i = id[k];
         if (fx) fx[i] = f[k].x;
         if (fy) fy[i] = f[k].y;
-        if (tau) tau[i] = f[k].c;
+        if (tau) tau[i] = f[k].z;
     }
     return SPP_OK;
 }

@@ -59,7 +59,7 @@ __device__ __forceinline__ double ref_wrap(double x, double L2)
 __global__ __launch_bounds__(256) void k_inverse(int N, const int* __restrict__ id, int* __restrict__ inv)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < N) inv[id[k]] = k;
+    if (k < N && (unsigned)id[k] < (unsigned)N) inv[id[k]] = k;
 }
 
 // Deviations of every clone from its base, thread per (replica, particle):
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(256) void k_apply(Args a)
         double dx = sc * q[0][c], dy = sc * q[1][c], dt = sc * q[2][c];
         int s = a.inv[(r * (K + 1) + c + 1) * a.n + j];
         double th = tb + dt;
-        a.pos[s] = P4{ref_wrap(b.x + dx, G.Lx2), ref_wrap(b.y + dy, G.Ly2), cos(th), sin(th)};
+        a.pos[s] = P4{ref_wrap(b.x + dx, G.Lx2), ref_wrap(b.y + dy, G.Ly2), th, 0.0};
         a.th[s] = th;
         a.U[s] = make_double2(ub.x + dx, ub.y + dy);
     }

@@ -311,8 +311,10 @@ __global__ __launch_bounds__(kBlock1024) void k_persist(Args a)
             if (k >= 0 && k < a.sample_cap && threadIdx.x < kBlock) {
                 double vx = 0, vy = 0;
                 for (int i = S.pid_off + threadIdx.x; i < S.pid_off + S.n; i += kBlock) {
-                    vx += cur.pos[i].c;
-                    vy += cur.pos[i].s;
+                    double c, sn;
+                    spp::heading_unit(cur.pos[i].z, c, sn);
+                    vx += c;
+                    vy += sn;
                 }
 #pragma unroll
                 for (int o = 32; o > 0; o >>= 1) {

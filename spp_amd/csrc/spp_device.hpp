@@ -16,11 +16,15 @@ namespace spp {
 constexpr double kPi = 3.14159265358979323846;  // the reference's PI = M_PI
 constexpr double kTwoPi = 6.283185307179586;    // 2*M_PI as the reference evaluates it
 
-// One particle as its neighbours read it: wrapped position and heading unit vector
-// (reference r and v = (cos theta, sin theta)), 32 B so a gather is one 128-B line access.
-// The heading angle itself lives in a separate array that only its owner reads.
+// One particle as its neighbours read it: wrapped position (reference r) and heading angle
+// (reference theta, wrapped into [-pi, pi)), 32 B so a gather is one aligned quarter of a 128-B
+// line.  Neighbours need theta itself: the alignment torque is sin(theta_j - theta_i) evaluated
+// from the angles (sin_diff), which keeps its relative accuracy for nearly aligned headings.  The
+// fourth field carries theta_old (the step's starting heading) in the outputs of the middle RK
+// stages, so the later stages of the step read it with the record; it is 0 in step-final records.
+// The same 32-B type also holds generic 4-vectors (forces and torque, RK4 accumulators).
 struct alignas(32) P4 {
-    double x, y, c, s;
+    double x, y, z, w;
 };
 
 // Streaming (nontemporal) stores for a stage's per-particle outputs.  Nothing in the same launch
@@ -32,11 +36,11 @@ struct alignas(32) P4 {
 // permutation, the slot-major list) became 1.4-1.8x slower with the same hint, so they keep
 // ordinary stores.
 typedef double stream_d2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ void store_stream(P4* p, double x, double y, double c, double s)
+__device__ __forceinline__ void store_stream(P4* p, double x, double y, double z, double w)
 {
     stream_d2* q = reinterpret_cast<stream_d2*>(p);
     __builtin_nontemporal_store(stream_d2{x, y}, q);
-    __builtin_nontemporal_store(stream_d2{c, s}, q + 1);
+    __builtin_nontemporal_store(stream_d2{z, w}, q + 1);
 }
 __device__ __forceinline__ void store_stream(double2* p, const double2& v)
 {
@@ -44,12 +48,12 @@ __device__ __forceinline__ void store_stream(double2* p, const double2& v)
 }
 __device__ __forceinline__ void store_stream(double* p, double v) { __builtin_nontemporal_store(v, p); }
 // a streaming or an ordinary store, chosen per launch (a uniform branch)
-__device__ __forceinline__ void store_out(bool stream, P4* p, double x, double y, double c, double s)
+__device__ __forceinline__ void store_out(bool stream, P4* p, double x, double y, double z, double w)
 {
     if (stream)
-        store_stream(p, x, y, c, s);
+        store_stream(p, x, y, z, w);
     else
-        *p = P4{x, y, c, s};
+        *p = P4{x, y, z, w};
 }
 template <class T>
 __device__ __forceinline__ void store_out(bool stream, T* p, const T& v)
@@ -118,6 +122,17 @@ __device__ __forceinline__ double wrap_angle(double th)
     return th - k * 2 * kPi;
 }
 
+// Heading unit vector v = (cos theta, sin theta) of a record's angle (the reference's v, set from
+// theta at every Move).  Angles a step wrote are in [-pi, pi); angles uploaded by a caller may be
+// anything, and those beyond the heading routine's range take the library's reduction.
+__device__ __forceinline__ void heading_unit(double th, double& c, double& s)
+{
+    if (fabs(th) <= 524288.0)
+        sincos_heading(th, &s, &c);
+    else
+        sincos(th, &s, &c);
+}
+
 // ------------------------------------------------------------------------------------------------
 // Philox4x32-10 (Salmon et al., SC'11) and the angular-noise stream: counter
 // (id, step_lo, step_hi, 'SPPn'), key (seed_lo, seed_hi), Box-Muller on two 53-bit uniforms.
@@ -158,9 +173,10 @@ __host__ __device__ __forceinline__ double noise_normal(uint64_t seed, uint32_t 
 // of every pair, which equals what the reference's Newton-3 half-list update adds to it).
 //   REPULSIVE  (particle.h:728-757): F = law(dr, d) for d < rc; s1 += sin(th_j - th_i) for d < Ra
 //   CONTINUOUS (particle.h:296-338): for d2 < Ra^2, s1 += sin(th_j - th_i), s2 += (dr x v_i)/d
-// sin(th_j - th_i) = s_j c_i - c_j s_i from the stored headings (no transcendental per pair).
+// me holds i's record (x, y, theta); (mc, ms) = (cos, sin) of theta_i, needed by the CONTINUOUS
+// self-term only.  sin(th_j - th_i) is taken from the angles (sin_diff), like particle.h:750.
 template <int MODEL, int REP>
-__device__ __forceinline__ void pair_accumulate(const Phys& P, const P4& me, const P4& pj,
+__device__ __forceinline__ void pair_accumulate(const Phys& P, const P4& me, double mc, double ms, const P4& pj,
                                                 double& fx, double& fy, double& s1, double& s2)
 {
     double dx = min_image(me.x - pj.x, P.Lx2, P.invLx2);
@@ -168,8 +184,8 @@ __device__ __forceinline__ void pair_accumulate(const Phys& P, const P4& me, con
     double d2 = dx * dx + dy * dy;
     if (MODEL == SPP_MODEL_CONTINUOUS) {
         if (d2 < P.Ra2) {
-            s1 += pj.s * me.c - pj.c * me.s;
-            if (P.c_self != 0.0) s2 += (dx * me.s - dy * me.c) * rsqrt(d2);
+            s1 += sin_diff(pj.z, me.z);
+            if (P.c_self != 0.0) s2 += (dx * ms - dy * mc) * rsqrt(d2);
         }
         return;
     } else {
@@ -191,7 +207,7 @@ __device__ __forceinline__ void pair_accumulate(const Phys& P, const P4& me, con
             fx += s * dx;
             fy += s * dy;
         }
-        if (d2 < P.T_align) s1 += pj.s * me.c - pj.c * me.s;
+        if (d2 < P.T_align) s1 += sin_diff(pj.z, me.z);
     }
 }
 

@@ -62,4 +62,42 @@ SPP_HD inline void sincos_heading(double x, double* sn, double* cs)
     *cs = ((k + 1) & 2) ? -b : b;
 }
 
+// sin(a - b) for two headings a, b in [-pi, pi], as the alignment torque of the reference
+// evaluates it (sin(p.theta - theta), shared/particle.h:750 and :305-306): the difference is
+// rounded once, and the sine of that double is taken to about one ulp *relative*, also where
+// a - b is close to 0, +-pi or +-2 pi (aligned and anti-aligned neighbours).  The product form
+// sin a cos b - cos a sin b of stored unit vectors cannot do this: its terms are ~1 and their
+// roundings (~1e-16 absolute) swamp a small difference.
+//   * reduction d = k pi + r, k = rint(d / pi) in {-2, ..., 2}, pi = PA + PB + PC with PA, PB of
+//     33 significant bits (k PA, k PB exact); r is exact when it is small (Sterbenz), so the
+//     result keeps the relative accuracy of fl(a - b) itself;
+//   * sin r on |r| <= pi/2 by its Taylor series to r^21 (truncation below 1.3e-18), coefficients
+//     the doubles nearest (-1)^m/(2m+1)!, Estrin's scheme with explicit fma;
+//   * sin(k pi + r) = (-1)^k sin r.
+SPP_HD inline double sin_diff(double a, double b)
+{
+    constexpr double kInvPi = 0.3183098861837907;
+    constexpr double PA = 3.1415926534682512;      // leading 33 bits of pi
+    constexpr double PB = 1.2154201012607932e-10;  // next 33 bits
+    constexpr double PC = 4.044532497591901e-21;   // pi - PA - PB, rounded
+    const double d = a - b;
+    const double kf = rint(d * kInvPi);
+    double r = fma(-kf, PA, d);
+    r = fma(-kf, PB, r);
+    r = fma(-kf, PC, r);
+    const double z = r * r;
+    const double z2 = z * z;
+    const double z4 = z2 * z2;
+    const double p01 = fma(0.008333333333333333, z, -0.16666666666666666);
+    const double p23 = fma(2.7557319223985893e-06, z, -0.0001984126984126984);
+    const double p45 = fma(1.6059043836821613e-10, z, -2.505210838544172e-08);
+    const double p67 = fma(2.8114572543455206e-15, z, -7.647163731819816e-13);
+    const double p89 = fma(1.9572941063391263e-20, z, -8.22063524662433e-18);
+    const double q0 = fma(p23, z2, p01);
+    const double q1 = fma(p67, z2, p45);
+    const double p = fma(fma(p89, z4, q1), z4, q0);
+    const double s = fma(r * z, p, r);
+    return ((int)kf & 1) ? -s : s;
+}
+
 }  // namespace spp

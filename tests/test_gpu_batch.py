@@ -141,6 +141,77 @@ def test_batch_system_equals_single_engine_bitwise():
             assert np.array_equal(got[k][key], w[key]), (k, key)
 
 
+def c4a_ensemble(seed=4):
+    """The ensemble workload: 256 Vicsek boxes, N log-uniform in [1024, 65536], density U[0.25, 4],
+    D U[0, 0.5], v0 0.5 (about 3.9 million particles), uniform random starts."""
+    rng = np.random.default_rng(seed)
+    systems, xs = [], []
+    for k in range(256):
+        n = int(round(math.exp(rng.uniform(math.log(1024), math.log(65536)))))
+        rho = rng.uniform(0.25, 4.0)
+        L = 0.5 * math.sqrt(n / rho)
+        systems.append(S.system(n, L, v0=0.5, g=1.0, D_rot=rng.uniform(0, 0.5), seed=100 + k))
+        xs.append(S.random_state(200 + k, n, L, L))
+    return systems, [np.concatenate([v[i] for v in xs]) for i in range(3)]
+
+
+def test_c4a_ensemble_256_ragged_boxes(oracle_mod):
+    """The full 256-box ensemble in one engine, run as two 100-step calls (RK2, dt 0.01,
+    displacement rebuilds): every polarisation lies in [0, 1], each box's torques sum to zero
+    (Newton-3), and the five smallest boxes follow independent oracle runs within 1e-9."""
+    p = vicsek_params()
+    systems, (x, y, th) = c4a_ensemble()
+    e = S.Engine(p, systems)
+    e.set_state(x, y, th)
+    assert e.info().ell_capacity >= e.info().max_neighbors
+    for _ in range(2):
+        e.multi_step(100)
+    pol = e.batch_polarization()
+    assert pol.shape == (256,) and np.all(pol >= 0) and np.all(pol <= 1 + 1e-12)
+    got = split(e, e.get_state())
+    e.interact()
+    tau = e.get_forces()[2]
+    for k in range(256):
+        a, b = e.offsets[k], e.offsets[k + 1]
+        assert abs(tau[a:b].sum()) < 1e-9 * (b - a), k
+    smallest = np.argsort([s.n for s in systems])[:5]
+    for k in smallest:
+        s = systems[k]
+        a, b = e.offsets[k], e.offsets[k + 1]
+        o = oracle_mod.Oracle(system_params(p, s))
+        o.set_state(x[a:b], y[a:b], th[a:b])
+        o.multi_step(200)
+        w = o.get_state()
+        err = max(np.abs(got[k]["xu"] - w["xu"]).max(), np.abs(got[k]["yu"] - w["yu"]).max(),
+                  angle_diff(got[k]["theta"], w["theta"]).max())
+        assert err < TRAJ_TOL, (k, s.n, err)
+        assert abs(pol[k] - o.polarization()[0]) < 1e-10
+
+
+def test_c4a_ensemble_systems_equal_single_engines_bitwise():
+    """With fixed-period rebuilds every box of the 256-box ensemble has its rebuilds at the same
+    steps as a single-box engine, so selected boxes (smallest, largest, densest, noisiest) are bit
+    for bit what they compute alone."""
+    p = vicsek_params(rebuild=S.REBUILD_PERIOD, rebuild_period=10)
+    systems, (x, y, th) = c4a_ensemble()
+    e = S.Engine(p, systems)
+    e.set_state(x, y, th)
+    e.multi_step(100)
+    got = split(e, e.get_state())
+    sizes = np.array([s.n for s in systems])
+    dens = np.array([s.n / (4 * s.Lx * s.Ly) for s in systems])
+    noise = np.array([s.D_rot for s in systems])
+    for k in sorted({int(np.argmin(sizes)), int(np.argmax(sizes)), int(np.argmax(dens)), int(np.argmax(noise))}):
+        s = systems[k]
+        a, b = e.offsets[k], e.offsets[k + 1]
+        one = S.Engine(system_params(p, s))
+        one.set_state(x[a:b], y[a:b], th[a:b])
+        one.multi_step(100)
+        w = one.get_state()
+        for key in ("xw", "yw", "xu", "yu", "theta"):
+            assert np.array_equal(got[k][key], w[key]), (k, key)
+
+
 def test_batch_neighbors_are_per_system():
     """No list entry crosses systems, and each system's list is its own brute-force pair set."""
     import oracle

@@ -122,6 +122,72 @@ def test_single_interact_full_size_c2(oracle_mod):
     assert abs(gf[2].sum()) < 1e-9
 
 
+def _wrap(t):
+    return t - 2 * np.pi * np.floor((t + np.pi) / (2 * np.pi))
+
+
+# Ordered flocks (low noise) hold nearly equal headings: the torque's sin(theta_j - theta_i) is then
+# tiny and must keep its *relative* accuracy (particle.h:750 takes the sine of the angle difference).
+ALIGNED_HEADINGS = {
+    "spread_1e-4": lambda rng, n: 0.3 + rng.uniform(-1e-4, 1e-4, n),
+    "spread_1e-6": lambda rng, n: 0.3 + rng.uniform(-1e-6, 1e-6, n),
+    "spread_1e-8": lambda rng, n: 0.3 + rng.uniform(-1e-8, 1e-8, n),
+    # the same heading seen as +pi and -pi: differences near +-2 pi
+    "across_wrap_1e-6": lambda rng, n: _wrap(np.pi + rng.uniform(-1e-6, 1e-6, n)),
+    # two counter-moving lanes: differences near 0 and near +-pi
+    "anti_aligned_1e-6": lambda rng, n: _wrap(0.3 + np.pi * rng.integers(0, 2, n) + rng.uniform(-1e-6, 1e-6, n)),
+}
+
+
+@pytest.mark.parametrize("model", ["c2_harmonic", "c1_continuous"])
+@pytest.mark.parametrize("case", sorted(ALIGNED_HEADINGS))
+def test_interact_near_aligned_headings(oracle_mod, case, model):
+    p = c2_params(n=20000, L=64.0) if model == "c2_harmonic" else c1_params(n=20000, Lx=64.0, Ly=64.0)
+    x, y, _ = oracle_mod.random_state(23, p.n, p.Lx, p.Ly)
+    th = ALIGNED_HEADINGS[case](np.random.default_rng(7), p.n)
+    e, o = _pair(p, x, y, th, oracle_mod)
+    e.interact()
+    o.interact()
+    gf, of = e.get_forces(), o.get_forces()
+    assert np.linalg.norm(of[2]) > 0
+    err = rel_err(gf[2], of[2])
+    assert err < FORCE_TOL, err
+    assert vec_rel_err(gf[0], gf[1], of[0], of[1]) < FORCE_TOL
+
+
+def test_interact_fully_aligned_box_is_torque_free(oracle_mod):
+    """Equal headings (a D = 0 flock): every torque is exactly zero on both sides, and 100 noiseless
+    steps keep every heading bit-for-bit while the positions follow the oracle."""
+    p = c2_params(n=20000, L=64.0, D=0.0)
+    x, y, _ = oracle_mod.random_state(29, p.n, p.Lx, p.Ly)
+    th = np.full(p.n, 0.3)
+    e, o = _pair(p, x, y, th, oracle_mod)
+    e.interact()
+    o.interact()
+    assert np.all(e.get_forces()[2] == 0.0) and np.all(o.get_forces()[2] == 0.0)
+    e.multi_step(100)
+    o.multi_step(100)
+    gs, os_ = e.get_state(), o.get_state()
+    assert np.all(gs["theta"] == 0.3)
+    assert np.abs(gs["xu"] - os_["xu"]).max() < TRAJ_TOL and np.abs(gs["yu"] - os_["yu"]).max() < TRAJ_TOL
+
+
+@pytest.mark.parametrize("dth", [1e-3, 1e-9, 1e-14, np.pi - 1e-9])
+def test_two_particle_torque_closed_form(oracle_mod, dth):
+    """Two particles inside the alignment radius: tau_1 = g sin(th_2 - th_1) / (pi R^2) exactly as
+    the reference evaluates it (the difference rounded once), to a few ulps relative."""
+    p = c2_params(n=2, L=8.0, repulsion=S.REP_NONE, g=1.0, R_align=1.0)
+    th1 = 0.3
+    th = np.array([th1, _wrap(th1 + dth)])
+    e = S.Engine(p)
+    e.set_state(np.array([0.1, 0.6]), np.array([0.2, 0.2]), th)
+    e.interact()
+    tau = e.get_forces()[2]
+    want = math.sin(th[1] - th[0]) / math.pi
+    assert abs(tau[0] - want) <= 4e-16 * abs(want)
+    assert tau[1] == -tau[0]
+
+
 # ---------------------------------------------------------------------------------------------
 # Stages and steps with injected noise
 # ---------------------------------------------------------------------------------------------
@@ -180,6 +246,39 @@ TRAJ_CASES = {
     "c2_small_rk4": lambda: c2_params(n=16384, L=64.0, integrator=S.RK4),
     "c2_period_rebuild": lambda: c2_params(n=16384, L=64.0, rebuild=S.REBUILD_PERIOD, rebuild_period=10),
 }
+
+
+@pytest.mark.parametrize("D", [0.05, 0.2])
+def test_trajectory_100_steps_full_size_c2(oracle_mod, D):
+    """The headline workload itself: N = 262,144 in a box of side 512, both noise values, 100 steps
+    from a uniform start against the oracle under the identical noise stream."""
+    p = c2_params(D=D)
+    x, y, th = oracle_mod.random_state(1, p.n, p.Lx, p.Ly)
+    e, o = _pair(p, x, y, th, oracle_mod)
+    e.multi_step(100)
+    o.multi_step(100)
+    gs, os_ = e.get_state(), o.get_state()
+    err = max(np.abs(gs["xu"] - os_["xu"]).max(), np.abs(gs["yu"] - os_["yu"]).max(),
+              angle_diff(gs["theta"], os_["theta"]).max())
+    assert err < TRAJ_TOL, err
+    assert abs(e.polarization()[0] - o.polarization()[0]) < 1e-10
+    assert e.info().tile_rebuilds > 0  # the headline path rebuilds through the tile sort
+
+
+def test_trajectory_above_streaming_store_limit(oracle_mod):
+    """300,000 particles (above the size up to which stage outputs use streaming stores) on a
+    triangular lattice: 20 fused steps against the oracle."""
+    n, L = 300000, 0.5 * math.sqrt(300000)
+    p = c2_params(n=n, L=L, D=0.2)
+    th = oracle_mod.random_state(3, n, L, L)[2]
+    x, y = S.triangle_lattice(n, L, L)
+    e, o = _pair(p, x, y, th, oracle_mod)
+    e.multi_step(20)
+    o.multi_step(20)
+    gs, os_ = e.get_state(), o.get_state()
+    err = max(np.abs(gs["xu"] - os_["xu"]).max(), np.abs(gs["yu"] - os_["yu"]).max(),
+              angle_diff(gs["theta"], os_["theta"]).max())
+    assert err < TRAJ_TOL, err
 
 
 @pytest.mark.parametrize("case", sorted(TRAJ_CASES))
@@ -246,6 +345,61 @@ def test_rebuild_schedules_agree(oracle_mod):
     ia = a.info()
     assert 2 <= ia.rebuilds < 200
     assert ia.host_syncs < 200
+
+
+def test_one_step_loop_rebuilds_only_when_due(oracle_mod):
+    """A Box::One_Step loop (one spp_multi_step call per step) continues on the current list while
+    the displacement allows it: 50 calls rebuild about as often as one 50-step call, far fewer than
+    50 times, and follow the same trajectory (the oracle's within the trajectory bar)."""
+    n, L = 16384, 64.0
+    p = c2_params(n=n, L=L)
+    th = oracle_mod.random_state(13, n, L, L)[2]
+    x, y = S.triangle_lattice(n, L, L)  # no overlaps: displacements grow smoothly
+    a = S.Engine(p)
+    a.set_state(x, y, th)
+    r0 = a.info().rebuilds
+    a.multi_step(50)
+    b = S.Engine(p)
+    b.set_state(x, y, th)
+    for _ in range(50):
+        b.one_step()
+    o = oracle_mod.Oracle(p)
+    o.set_state(x, y, th)
+    o.multi_step(50)
+    ra, rb = a.info().rebuilds - r0, b.info().rebuilds - r0
+    assert rb <= ra + 3 and rb < 12, (ra, rb)
+    sa, sb, so = a.get_state(), b.get_state(), o.get_state()
+    assert np.abs(sa["xu"] - sb["xu"]).max() < 1e-11
+    assert angle_diff(sa["theta"], sb["theta"]).max() < 1e-11
+    assert np.abs(sb["xu"] - so["xu"]).max() < TRAJ_TOL and angle_diff(sb["theta"], so["theta"]).max() < TRAJ_TOL
+    assert b.info().steps == 50
+
+
+def test_split_call_loop_with_caller_list_updates(oracle_mod):
+    """Box::Interact + Move_Runge_Kutta2_1/2 for 60 steps, the caller refreshing the cells and the
+    list every 10 steps (Update_Cells at the reference's cell_update_period cadence): Interact uses
+    the list of the last build, as serial/box.h:322-371 does, so the trajectory stays on the
+    oracle's driven the same way."""
+    n, L = 8192, 48.0
+    p = c2_params(n=n, L=L, noise_mode=S.NOISE_INJECTED)
+    x, y, th = oracle_mod.random_state(19, n, L, L)
+    e, o = _pair(p, x, y, th, oracle_mod)
+    rng = np.random.default_rng(2)
+    for step in range(60):
+        if step and step % 10 == 0:
+            e.build_neighbors()
+            o.update_cells()
+        dn = rng.normal(0, 0.02, n)
+        e.inject_noise(dn)
+        o.set_noise(dn)
+        for st in (1, 2):
+            e.interact()
+            o.interact()
+            e.move_stage(st)
+            o.move_stage(st)
+    gs, os_ = e.get_state(), o.get_state()
+    assert np.abs(gs["xu"] - os_["xu"]).max() < TRAJ_TOL
+    assert angle_diff(gs["theta"], os_["theta"]).max() < TRAJ_TOL
 
 
 def test_snapshot_restore_replays_bitwise(oracle_mod):

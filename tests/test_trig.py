@@ -34,3 +34,26 @@ def test_exact_points(out):
     assert vals[math.pi][1] == -1.0 and vals[-math.pi][1] == -1.0
     for x in (0.5, -2.5):
         assert vals[x] == pytest.approx((math.sin(x), math.cos(x)), rel=3e-16, abs=1e-17)
+
+
+@pytest.fixture(scope="module")
+def sin_diff_out(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("trig") / "sin_diff_accuracy")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off",
+                    os.path.join(ROOT, "tests", "cpp", "sin_diff_accuracy.cpp"), "-o", exe], check=True)
+    return subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")
+
+
+def test_sin_diff_relative_error(sin_diff_out):
+    """sin(a - b) of the alignment torque stays within a few ulps *relative* for uniform, nearly
+    aligned (|a - b| down to 1e-14), nearly anti-aligned and wrap-aligned heading pairs -- the
+    product form s_j c_i - c_j s_i is ~1e-16 absolute, i.e. 1e-2 relative at |a - b| = 1e-14."""
+    uni, aligned, anti, wrapped = (float(v) for v in sin_diff_out[0].split())
+    assert uni <= 2.5
+    assert aligned <= 1.5 and anti <= 1.5 and wrapped <= 1.5
+
+
+def test_sin_diff_exact_values(sin_diff_out):
+    z, t, u, one = (float(v) for v in sin_diff_out[1].split())
+    assert z == 0.0 and t == 1e-12 and u == 1e-300
+    assert abs(one - 1.0) <= 2.3e-16
