@@ -259,6 +259,16 @@ int spp_batch_polarization(spp_engine* e, double* p, int32_t nsys);
    (The reference has a body-frame 2-D pair distribution, analyze/analyze.h:365-423, and a radial
    density about the origin, :332-362; the isotropic g(r) is new.)                              */
 int spp_pair_distribution(spp_engine* e, double r_max, int32_t nbins, int64_t* counts, double* g);
+/* spp_pair_distribution without any allocation.  No counterpart in the reference: it is for callers
+   that cannot allocate device (or host) memory -- code under HIP graph capture, tight serving
+   loops, harnesses that forbid hipMalloc.  The histogram lives in a static device array of 8192
+   bins (64 KB per GPU, held until the process ends): nbins above 8192 is SPP_ERR_ARG.  The list
+   must be current (SPP_ERR_STATE otherwise: a rebuild could grow it; call spp_build_neighbors
+   first), and profiling must be off.  counts and g as in spp_pair_distribution, bit for bit; the
+   counts pass through the caller's counts array (or, if it is NULL, through g).  All engines on one
+   GPU share the array: calls that overlap on one GPU (engines on different streams or threads)
+   race.                                                                                          */
+int spp_pair_distribution_noalloc(spp_engine* e, double r_max, int32_t nbins, int64_t* counts, double* g);
 
 /* ---- Lyapunov exponents by finite-difference clones ----------------------------------------------
    The reference evolves perturbed copies of the state with identical noise and re-orthonormalises

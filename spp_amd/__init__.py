@@ -151,6 +151,7 @@ SIGNATURES = {
     "spp_batch_polarization": (C.c_int, [C.c_void_p, _D, C.c_int32]),
     "spp_multi_step_sampled": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _D, C.c_int64]),
     "spp_pair_distribution": (C.c_int, [C.c_void_p, C.c_double, C.c_int32, _I64, _D]),
+    "spp_pair_distribution_noalloc": (C.c_int, [C.c_void_p, C.c_double, C.c_int32, _I64, _D]),
     "spp_lyapunov_init": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_uint64]),
     "spp_lyapunov_renormalize": (C.c_int, [C.c_void_p, _D]),
     "spp_lyapunov_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
@@ -389,6 +390,15 @@ class Engine:
         self._check(self._lib.spp_pair_distribution(self._h, r_max, nbins, _ptr(counts, _I64), _ptr(g)))
         dr = r_max / nbins
         return (np.arange(nbins) + 0.5) * dr, g, counts
+
+    def pair_distribution_noalloc(self, r_max: float, nbins: int, counts=None, g=None):
+        """spp_pair_distribution_noalloc into caller-provided arrays (either may be None)."""
+        for a in (counts, g):
+            if a is not None and (a.shape != (nbins,) or not a.flags.c_contiguous):
+                raise ValueError("output arrays need shape (nbins,), contiguous")
+        if counts is not None and counts.dtype != np.int64 or g is not None and g.dtype != np.float64:
+            raise ValueError("counts must be int64 and g float64")
+        self._check(self._lib.spp_pair_distribution_noalloc(self._h, r_max, nbins, _ptr(counts, _I64), _ptr(g)))
 
     # -- Lyapunov clones (batched engine: replicas x (base + vectors) systems) -------------------
     def lyapunov_init(self, replicas: int, vectors: int, epsilon: float, rescale: bool = True, seed: int = 1):

@@ -32,6 +32,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1727,6 +1728,30 @@ const char* stage_name(int integ, int stage)
 
 thread_local std::string g_create_error;
 
+// Device scratch of the allocation-free entry point (spp_pair_distribution_noalloc): a fixed
+// array in the library's code object, one copy per GPU, held until the process ends.  Host code
+// uses its per-device address from hipGetSymbolAddress (resolved once per device when the first
+// engine on that device is created), never the array's name.
+constexpr int kScratchBins = 8192;
+static __device__ __align__(256) unsigned long long scratch[kScratchBins];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned long long* g_scratch_ptr[kMaxDevices] = {};
+
+// The calling thread's current device must be `dev`.
+int resolve_scratch(int dev, unsigned long long** out)
+{
+    if (dev < 0 || dev >= kMaxDevices) return SPP_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return SPP_ERR_HIP;
+        g_scratch_ptr[dev] = static_cast<unsigned long long*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return SPP_OK;
+}
+
 // Smallest double T with fl(sqrt(T)) >= R: then sqrt(d2) < R  <=>  d2 < T for every d2 >= 0
 // (correctly rounded sqrt is monotone), so the reference's predicates need no sqrt.
 double sqrt_threshold(double R)
@@ -1894,6 +1919,8 @@ struct spp_engine {
     int64_t graph_base = 0;
     long long* dclock = nullptr; // the replayed interval's first step
     int64_t graph_replays = 0, graph_captures = 0;
+
+    unsigned long long* scratch = nullptr;  // this device's static scratch (spp_pair_distribution_noalloc)
 
     bool profiling = false;
     std::vector<ProfRec> prof;
@@ -2914,6 +2941,10 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
     e->p = *p;
     e->N = (int)ntot;
     (void)hipGetDevice(&e->device);
+    if (resolve_scratch(e->device, &e->scratch) != SPP_OK) {
+        delete e;
+        return fail(nullptr, SPP_ERR_HIP, "device scratch of the allocation-free entry points not resolvable");
+    }
     auto grid_of = [&](const SysDesc& d) {
         Grid G{};
         G.Lx = d.Lx;
@@ -3553,6 +3584,41 @@ int spp_pair_distribution(spp_engine* e, double r_max, int32_t nbins, int64_t* c
         if (counts) counts[k] = (int64_t)h[k];
         const double a = spp::kPi * ((k + 1) * dr * (k + 1) * dr - k * dr * k * dr);
         if (g) g[k] = (double)h[k] / (e->N * rho * a);
+    }
+    return SPP_OK;
+}
+
+int spp_pair_distribution_noalloc(spp_engine* e, double r_max, int32_t nbins, int64_t* counts, double* g)
+{
+    TRY(check_ready(e));
+    if (e->dsys) return fail(e, SPP_ERR_UNSUPPORTED, "pair distribution of a batched engine");
+    if (nbins <= 0 || nbins > kScratchBins || !(r_max > 0))
+        return fail(e, SPP_ERR_ARG, "need 0 < nbins <= %d (the static scratch), r_max > 0", kScratchBins);
+    if (r_max > e->p.r_list) return fail(e, SPP_ERR_ARG, "r_max %g beyond the list radius %g", r_max, e->p.r_list);
+    // a rebuild may have to grow the list (an allocation): the caller builds it beforehand
+    if (!e->list_valid) return fail(e, SPP_ERR_STATE, "no current neighbour list: call spp_build_neighbors first");
+    if (!counts && !g) return SPP_OK;
+    if (e->profiling) return fail(e, SPP_ERR_STATE, "profiling records events (allocations): disable it first");
+    // the scratch keeps the previous call's contents: clear the bins this call uses (a fill
+    // kernel, so the call may also be captured into a graph)
+    unsigned long long* dh = e->scratch;
+    hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(nblocks(nbins)), dim3(kBlock), 0, e->stream, dh, 0ull, nbins);
+    const double dr = r_max / nbins;
+    hipLaunchKernelGGL(k_pair_hist, dim3(e->nblk), dim3(kBlock), nbins * sizeof(unsigned int), e->stream, e->N,
+                       (const P4*)e->set[e->cur].pos, (const int*)e->nbr32, (const int*)e->ncount, e->G, 1.0 / dr,
+                       nbins, dh);
+    HIPC(e, hipGetLastError());
+    // the counts land in the caller's array (or, without one, in g, converted in place): 8 B per bin
+    // either way, so no host buffer is needed
+    void* dst = counts ? static_cast<void*>(counts) : static_cast<void*>(g);
+    HIPC(e, hipMemcpyAsync(dst, dh, nbins * sizeof(unsigned long long), hipMemcpyDeviceToHost, e->stream));
+    HIPC(e, hipStreamSynchronize(e->stream));
+    const double rho = e->N / (4.0 * e->p.Lx * e->p.Ly);
+    for (int k = 0; k < nbins; ++k) {
+        unsigned long long c;
+        std::memcpy(&c, static_cast<const char*>(dst) + 8 * (size_t)k, 8);
+        const double a = spp::kPi * ((k + 1) * dr * (k + 1) * dr - k * dr * k * dr);
+        if (g) g[k] = (double)c / (e->N * rho * a);
     }
     return SPP_OK;
 }
