@@ -149,8 +149,20 @@ typedef struct spp_params {
                                interval (rebuild + rebuild_period steps) once as a HIP graph per
                                buffer layout and replay it; 0: launch every kernel                  */
     int32_t rebuild_sort;   /* SPP_REBUILD_SORT_*                                                   */
-    int32_t reserved_i[4];
-    double reserved_d[11];
+    /* SPP_MODEL_ABP multi-bead swimmers (ActiveBrownianChain with nb > 1, shared/particle.h:792-1057).
+       nb_swimmer <= 1: single-bead swimmers, every pair feels the repulsion law above (the reference's
+       nb = 1 branch).  nb_swimmer >= 2: swimmer i carries nb_swimmer beads at s_k = v_i bead_sep
+       ((1 - nb)/2 + k); pairs of two single beads (ring beads) keep the law above, every other pair
+       sums R12_Repulsive_Truncated(sigma_bead, rc_bead, A_bead) over bead pairs, with torques s_k x F
+       (particle.h:983-1011); swimmers move with the mobilities of Set_nb(nb_swimmer) (:856-876), ring
+       beads with mob_par / mob_perp / mob_rot.                                                    */
+    int32_t nb_swimmer;
+    int32_t reserved_i[3];
+    double bead_sep;        /* bead separation (reference ActiveBrownianChain::separation)          */
+    double sigma_bead;      /* bead repulsion sigma (sigma_p)                                        */
+    double rc_bead;         /* bead repulsion cutoff (repulsion_radius)                              */
+    double A_bead;          /* bead repulsion amplitude (A_p)                                        */
+    double reserved_d[7];
 } spp_params;
 
 typedef struct spp_engine spp_engine;

@@ -95,6 +95,7 @@ def load_library(path: str) -> C.CDLL:
     L.orc_step_count.argtypes = [C.c_void_p]
     L.orc_rebuild_count.restype = C.c_int64
     L.orc_rebuild_count.argtypes = [C.c_void_p]
+    L.orc_chain_mobilities.argtypes = [C.c_int32, C.c_double, C.c_double, _D, _D, _D]
     return L
 
 
@@ -141,6 +142,13 @@ def brute_pairs(x, y, Lx, Ly, r):
     pj = np.empty(max(m, 1), dtype=np.int32)
     lib().orc_brute_pairs(_p(x), _p(y), n, Lx, Ly, r, _p(pi, _I32), _p(pj, _I32), m)
     return pi[:m], pj[:m]
+
+
+def chain_mobilities(nb: int, sep: float, sigma: float):
+    """(m_parallel, m_perpendicular, k_perpendicular) of an nb-bead chain (Set_nb)."""
+    a, b, c = C.c_double(), C.c_double(), C.c_double()
+    lib().orc_chain_mobilities(nb, sep, sigma, C.byref(a), C.byref(b), C.byref(c))
+    return a.value, b.value, c.value
 
 
 def random_state(seed: int, n: int, Lx: float, Ly: float):

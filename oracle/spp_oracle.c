@@ -271,7 +271,16 @@ static void push(int32_t** a, int32_t* n, int32_t* cap, int32_t v)
 
 static double max_cutoff(const spp_params* p)
 {
-    if (p->model == SPP_MODEL_ABP) return p->repulsion != SPP_REP_NONE ? p->rc_rep : 0.0;
+    if (p->model == SPP_MODEL_ABP) {
+        double c = p->repulsion != SPP_REP_NONE ? p->rc_rep : 0.0;
+        /* two chains interact while bead pairs, up to bead_sep (nb - 1) / 2 from either centre, are
+           closer than rc_bead */
+        if (p->nb_swimmer >= 2) {
+            double cb = p->rc_bead + p->bead_sep * (p->nb_swimmer - 1);
+            if (cb > c) c = cb;
+        }
+        return c;
+    }
     double c = p->R_align;
     if (p->repulsion != SPP_REP_NONE && p->rc_rep > c) c = p->rc_rep;
     return c;
@@ -291,6 +300,11 @@ orc_sys* orc_create(const spp_params* p)
         (p->integrator == SPP_RK4 || p->n_membrane < 0 || p->n_membrane > p->n ||
          (p->n_membrane > 0 && p->n_membrane < 3))) {
         set_err("orc_create: active Brownian model needs Euler/RK2 and n_membrane 0 or in [3, n]");
+        return NULL;
+    }
+    if (p->model == SPP_MODEL_ABP && p->nb_swimmer >= 2 &&
+        (p->nb_swimmer > 16 || !(p->bead_sep > 0) || !(p->sigma_bead > 0) || !(p->rc_bead > 0))) {
+        set_err("orc_create: chains need 2 <= nb_swimmer <= 16 and positive bead_sep, sigma_bead, rc_bead");
         return NULL;
     }
     orc_sys* s = (orc_sys*)calloc(1, sizeof(orc_sys));
@@ -488,6 +502,35 @@ int64_t orc_half_list(orc_sys* s, int32_t* owner, int32_t* other, int64_t cap)
     return m;
 }
 
+/* Beads of particle i: ring beads (i < n_membrane) are single; swimmers carry nb_swimmer beads. */
+static int nbeads(const orc_sys* s, int32_t i)
+{
+    return (s->p.nb_swimmer >= 2 && i >= s->p.n_membrane) ? s->p.nb_swimmer : 1;
+}
+
+/* ActiveBrownianChain::Set_nb (particle.h:856-876): mobilities of a chain of nb beads. */
+static void chain_mobilities(int nb, double sep, double sigma, double* mpar, double* mperp, double* kperp)
+{
+    if (nb == 1) {
+        *mpar = 0.5;
+        *mperp = 0.5;
+        *kperp = 1;
+        return;
+    }
+    double aspect_ratio = 1 / ((sep * (nb - 1) / sigma) + 1);
+    double e2 = 1 - aspect_ratio * aspect_ratio;
+    double e = sqrt(e2);
+    double L = log((1 + e) / (1 - e));
+    *mpar = 1.0;
+    *mperp = 0.5 * (2 * e + (3 * e2 - 1) * L) / ((1 + e2) * L - 2 * e);
+    *kperp = 6 / (1 + aspect_ratio * aspect_ratio) / sigma / sigma;
+}
+
+void orc_chain_mobilities(int32_t nb, double sep, double sigma, double* mpar, double* mperp, double* kperp)
+{
+    chain_mobilities(nb, sep, sigma, mpar, mperp, kperp);
+}
+
 /* Pair interaction.  SPP_MODEL_REPULSIVE follows RepulsiveParticle::Interact
  * (particle.h:728-757); SPP_MODEL_CONTINUOUS follows ContinuousParticle::Interact
  * (particle.h:296-338).  Newton-3: both particles are updated.                                    */
@@ -499,16 +542,40 @@ static void pair(orc_sys* s, particle* a, particle* b)
     dr.y = orc_periodic(dr.y, s->Ly2);
     double d2 = dr.x * dr.x + dr.y * dr.y;
     if (p->model == SPP_MODEL_ABP) {
-        /* ActiveBrownianChain::Interact, nb == 1 on both sides (particle.h:968-981) */
-        double d = sqrt(d2);
-        if (p->repulsion != SPP_REP_NONE && d < p->rc_rep) {
-            double fx, fy;
-            orc_force_law(p->repulsion, dr.x, dr.y, d, p->rc_rep, p->sigma_rep, p->A_rep, &fx, &fy);
-            a->f.x += fx;
-            a->f.y += fy;
-            b->f.x -= fx;
-            b->f.y -= fy;
+        const int na = nbeads(s, (int32_t)(a - s->P)), nbb = nbeads(s, (int32_t)(b - s->P));
+        if (na == 1 && nbb == 1) {
+            /* ActiveBrownianChain::Interact, nb == 1 on both sides (particle.h:968-981) */
+            double d = sqrt(d2);
+            if (p->repulsion != SPP_REP_NONE && d < p->rc_rep) {
+                double fx, fy;
+                orc_force_law(p->repulsion, dr.x, dr.y, d, p->rc_rep, p->sigma_rep, p->A_rep, &fx, &fy);
+                a->f.x += fx;
+                a->f.y += fy;
+                b->f.x -= fx;
+                b->f.y -= fy;
+            }
+            return;
         }
+        /* bead pairs (particle.h:983-1011): s = v separation ((1 - nb)/2 + i), dr = dr0 + s_this -
+           s_that, R12_Repulsive_Truncated within repulsion_radius, torques s x F */
+        const double sep = p->bead_sep;
+        for (int i = 0; i < na; i++)
+            for (int j = 0; j < nbb; j++) {
+                vec2 st = {a->v.x * (sep * ((1 - na) / 2.0 + i)), a->v.y * (sep * ((1 - na) / 2.0 + i))};
+                vec2 su = {b->v.x * (sep * ((1 - nbb) / 2.0 + j)), b->v.y * (sep * ((1 - nbb) / 2.0 + j))};
+                vec2 db = {dr.x + st.x - su.x, dr.y + st.y - su.y};
+                double d = sqrt(db.x * db.x + db.y * db.y);
+                if (d < p->rc_bead) {
+                    double fx, fy;
+                    orc_force_law(SPP_REP_R12_TRUNCATED, db.x, db.y, d, p->rc_bead, p->sigma_bead, p->A_bead, &fx, &fy);
+                    a->f.x += fx;
+                    a->f.y += fy;
+                    b->f.x -= fx;
+                    b->f.y -= fy;
+                    a->torque += st.x * fy - st.y * fx;
+                    b->torque += -su.x * fy + su.y * fx;
+                }
+            }
         return;
     }
     if (p->model == SPP_MODEL_REPULSIVE) {
@@ -686,9 +753,16 @@ static void set_v(particle* q)
 static void abp_move(orc_sys* s, int32_t stage)
 {
     const double dt = s->p.dt, half_dt = s->half_dt;
-    const double kp = s->p.mob_rot, mpar = s->p.mob_par, mperp = s->p.mob_perp;
+    /* ring beads (and single-bead swimmers) move with the configured mobilities, chains of
+       nb_swimmer beads with Set_nb's */
+    double c_par, c_perp, c_rot;
+    chain_mobilities(s->p.nb_swimmer >= 2 ? s->p.nb_swimmer : 1, s->p.bead_sep, s->p.sigma_bead, &c_par, &c_perp,
+                     &c_rot);
     for (int32_t i = 0; i < s->N; ++i) {
         particle* q = &s->P[i];
+        const int chain = nbeads(s, i) > 1;
+        const double kp = chain ? c_rot : s->p.mob_rot, mpar = chain ? c_par : s->p.mob_par,
+                     mperp = chain ? c_perp : s->p.mob_perp;
         double tq = q->torque;
         if (stage == 0) {
             q->theta += kp * dt * (tq);

@@ -57,6 +57,8 @@ double orc_polarization(orc_sys* s, double* sx, double* sy);
 int64_t orc_step_count(orc_sys* s);
 int64_t orc_rebuild_count(orc_sys* s);
 double orc_ring_energy(orc_sys* s);
+/* ActiveBrownianChain::Set_nb mobilities (m_parallel, m_perpendicular, k_perpendicular) */
+void orc_chain_mobilities(int32_t nb, double sep, double sigma, double* mpar, double* mperp, double* kperp);
 
 #ifdef __cplusplus
 }

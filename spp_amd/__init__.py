@@ -72,8 +72,13 @@ class Params(C.Structure):
         ("kernel_variant", C.c_int32),
         ("graph_intervals", C.c_int32),
         ("rebuild_sort", C.c_int32),
-        ("reserved_i", C.c_int32 * 4),
-        ("reserved_d", C.c_double * 11),
+        ("nb_swimmer", C.c_int32),
+        ("reserved_i", C.c_int32 * 3),
+        ("bead_sep", C.c_double),
+        ("sigma_bead", C.c_double),
+        ("rc_bead", C.c_double),
+        ("A_bead", C.c_double),
+        ("reserved_d", C.c_double * 7),
     ]
 
     def update(self, **kw) -> "Params":

@@ -76,17 +76,86 @@ __device__ __forceinline__ void ring_forces(const StageArgs& a, const Phys& P, i
     }
 }
 
-// Forces and torque on one particle: Reset + pair repulsion + ring forces.
+// Bead pairs of two chains (ActiveBrownianChain::Interact, particle.h:983-1011), this particle's
+// side: beads at s = v sep ((1 - nb)/2 + k), dr = (dr0 + s_this) - s_that, R12_Repulsive_Truncated
+// within rc_b (force-fields.h:14-39); adds F to f and s_this x F to the torque.
+__device__ __forceinline__ void bead_pairs(const AbpConst& A, double dx0, double dy0, double mc, double ms, int na,
+                                           double cj, double sj, int nbj, double& fx, double& fy, double& tq)
+{
+    for (int k = 0; k < na; ++k) {
+        const double ok = A.sep * ((1 - na) / 2.0 + k);
+        const double stx = mc * ok, sty = ms * ok;
+        for (int l = 0; l < nbj; ++l) {
+            const double ol = A.sep * ((1 - nbj) / 2.0 + l);
+            const double dx = dx0 + stx - cj * ol, dy = dy0 + sty - sj * ol;
+            const double d2 = dx * dx + dy * dy;
+            if (d2 < A.T_b) {
+                const double d = sqrt(d2);
+                const double q = A.sigma_b / d, q2 = q * q, q4 = q2 * q2, q8 = q4 * q4;
+                const double st = A.A_b * (q8 * q4 * q - A.rc13_b);
+                const double Fx = st * dx / d, Fy = st * dy / d;
+                fx += Fx;
+                fy += Fy;
+                tq += stx * Fy - sty * Fx;
+            }
+        }
+    }
+}
+
+// Pairs of a particle with nb beads (1 for ring beads) when the swimmers are chains: two single
+// beads feel the configured law (the reference's nb = 1 branch), every other pair the bead law.
+// The neighbour's class comes from its caller index, its heading from its record.
+template <int REP>
+__device__ __forceinline__ void chain_gather(const StageArgs& a, const Phys& P, int i, const P4& me, double mc,
+                                             double ms, int na, double& fx, double& fy, double& tq)
+{
+    const AbpConst& A = a.abp;
+    const int cnt = a.ncount[i];
+    double s1 = 0, s2 = 0;
+    for (int k = 0; k < cnt; ++k) {
+        const int j = a.nbr32[(size_t)k * a.N + i];
+        const P4 q = a.gather[j];
+        const int nbj = (unsigned)a.id[j] < (unsigned)A.n_membrane ? 1 : A.nb;
+        if (na == 1 && nbj == 1) {
+            spp::pair_accumulate<SPP_MODEL_REPULSIVE, REP>(P, me, mc, ms, q, fx, fy, s1, s2);
+        } else {
+            const double dx0 = spp::min_image(me.x - q.x, P.Lx2, P.invLx2);
+            const double dy0 = spp::min_image(me.y - q.y, P.Ly2, P.invLy2);
+            double cj, sj;
+            spp::heading_unit(q.z, cj, sj);
+            bead_pairs(A, dx0, dy0, mc, ms, na, cj, sj, nbj, fx, fy, tq);
+        }
+    }
+}
+
+// Forces and torque on one particle: Reset (f = F0 v, torque = torque0) + pair repulsion + ring
+// forces.
 template <int REP>
 __device__ __forceinline__ void abp_forces(const StageArgs& a, const Phys& P0, int i, int pid, const P4& me,
-                                           double mc, double ms, double& fx, double& fy)
+                                           double mc, double ms, double& fx, double& fy, double& tq)
 {
     Phys P = P0;
     const bool bead = (unsigned)pid < (unsigned)a.abp.n_membrane;
     P.v0 = bead ? 0.0 : a.abp.F0;
-    double tq;
-    gather_forces_flat<SPP_MODEL_REPULSIVE, REP, false>(a, P, i, me, mc, ms, fx, fy, tq);
+    tq = a.abp.torque0;
+    if (a.abp.nb <= 1) {
+        double t_unused;
+        gather_forces_flat<SPP_MODEL_REPULSIVE, REP, false>(a, P, i, me, mc, ms, fx, fy, t_unused);
+    } else {
+        fx = mc * P.v0;
+        fy = ms * P.v0;
+        chain_gather<REP>(a, P, i, me, mc, ms, bead ? 1 : a.abp.nb, fx, fy, tq);
+    }
     if (bead) ring_forces(a, P, pid, me, fx, fy);
+}
+
+// Mobilities of a particle: (parallel, perpendicular, rotational).
+__device__ __forceinline__ void abp_mobility(const AbpConst& A, int pid, double& mpar, double& mperp, double& krot)
+{
+    const bool chain = A.nb > 1 && !((unsigned)pid < (unsigned)A.n_membrane);
+    mpar = chain ? A.c_par : A.mob_par;
+    mperp = chain ? A.c_perp : A.mob_perp;
+    krot = chain ? A.c_rot : A.mob_rot;
 }
 
 template <int REP, int INTEG, int STAGE>
@@ -116,16 +185,18 @@ __global__ __launch_bounds__(kBlock) void k_stage_abp(StageArgs a)
             dn = a.dth[i];
             u = a.U_in[i];
         }
-        double fx, fy;
-        abp_forces<REP>(a, P, i, pid, me, mc, ms, fx, fy);
-        const double tq = A.torque0, dm = A.mob_par - A.mob_perp;
+        double fx, fy, tq;
+        abp_forces<REP>(a, P, i, pid, me, mc, ms, fx, fy, tq);
+        double mpar, mperp, krot;
+        abp_mobility(A, pid, mpar, mperp, krot);
+        const double dm = mpar - mperp;
         double th, sn, cs;
         if (INTEG == SPP_EULER) {
-            th = th_in + A.mob_rot * P.dt * (tq);
+            th = th_in + krot * P.dt * (tq);
             th += dn;
             spp::sincos_heading(th, &sn, &cs);
-            u.x += fx * (P.dt * A.mob_perp);
-            u.y += fy * (P.dt * A.mob_perp);
+            u.x += fx * (P.dt * mperp);
+            u.y += fy * (P.dt * mperp);
             double fv = fx * cs + fy * sn;
             u.x += cs * (fv * (P.dt * dm));
             u.y += sn * (fv * (P.dt * dm));
@@ -134,22 +205,22 @@ __global__ __launch_bounds__(kBlock) void k_stage_abp(StageArgs a)
             y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
         } else if (STAGE == 1) {
             a.dth[i] = dn;
-            th = th_in + A.mob_rot * P.half_dt * (tq);
+            th = th_in + krot * P.half_dt * (tq);
             th += dn / 2;
             th = spp::wrap_angle(th);
-            double xx = me.x + fx * (P.half_dt * A.mob_perp);
-            double yy = me.y + fy * (P.half_dt * A.mob_perp);
+            double xx = me.x + fx * (P.half_dt * mperp);
+            double yy = me.y + fy * (P.half_dt * mperp);
             double fv = fx * mc + fy * ms;
             xx += mc * (fv * (P.half_dt * dm));
             yy += ms * (fv * (P.half_dt * dm));
             x = spp::periodic_wrap(xx, P.Lx, P.Lx2);
             y = spp::periodic_wrap(yy, P.Ly, P.Ly2);
         } else {
-            th = thold + A.mob_rot * P.dt * (tq);
+            th = thold + krot * P.dt * (tq);
             th += dn;
             th = spp::wrap_angle(th);
-            u.x = u.x + fx * (P.dt * A.mob_perp);
-            u.y = u.y + fy * (P.dt * A.mob_perp);
+            u.x = u.x + fx * (P.dt * mperp);
+            u.y = u.y + fy * (P.dt * mperp);
             double fv = fx * mc + fy * ms;
             u.x += mc * (fv * (P.dt * dm));
             u.y += ms * (fv * (P.dt * dm));
@@ -170,10 +241,10 @@ __global__ __launch_bounds__(kBlock) void k_interact_abp(StageArgs a, P4* force)
     int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.N) return;
     P4 me = a.gather[i];
-    double mc, ms, fx, fy;
+    double mc, ms, fx, fy, tq;
     spp::heading_unit(me.z, mc, ms);
-    abp_forces<REP>(a, a.P, i, a.id[i], me, mc, ms, fx, fy);
-    force[i] = P4{fx, fy, a.abp.torque0, 0.0};
+    abp_forces<REP>(a, a.P, i, a.id[i], me, mc, ms, fx, fy, tq);
+    force[i] = P4{fx, fy, tq, 0.0};
 }
 
 // Split-call move for the single-bead model (forces from k_interact_abp).
@@ -189,17 +260,19 @@ __global__ __launch_bounds__(kBlock) void k_move_abp(MoveArgs a)
     double th_in = a.cur.th[i];
     P4 f = a.force[i];
     double2 u = a.cur.U[i];
-    const double dm = A.mob_par - A.mob_perp;
+    double mpar, mperp, krot;
+    abp_mobility(A, a.cur.id[i], mpar, mperp, krot);
+    const double dm = mpar - mperp;
     double th, x, y, sn, cs;
     if (a.stage <= 1) {
         double dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)a.cur.id[i], a.step);
         a.dth[i] = dn;
         if (a.integ == SPP_EULER) {
-            th = th_in + A.mob_rot * P.dt * (f.z);
+            th = th_in + krot * P.dt * (f.z);
             th += dn;
             spp::sincos_heading(th, &sn, &cs);
-            u.x += f.x * (P.dt * A.mob_perp);
-            u.y += f.y * (P.dt * A.mob_perp);
+            u.x += f.x * (P.dt * mperp);
+            u.y += f.y * (P.dt * mperp);
             double fv = f.x * cs + f.y * sn;
             u.x += cs * (fv * (P.dt * dm));
             u.y += sn * (fv * (P.dt * dm));
@@ -208,11 +281,11 @@ __global__ __launch_bounds__(kBlock) void k_move_abp(MoveArgs a)
             y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
         } else {
             a.old[i] = P4{u.x, u.y, th_in, 0.0};
-            th = th_in + A.mob_rot * P.half_dt * (f.z);
+            th = th_in + krot * P.half_dt * (f.z);
             th += dn / 2;
             th = spp::wrap_angle(th);
-            double xx = me.x + f.x * (P.half_dt * A.mob_perp);
-            double yy = me.y + f.y * (P.half_dt * A.mob_perp);
+            double xx = me.x + f.x * (P.half_dt * mperp);
+            double yy = me.y + f.y * (P.half_dt * mperp);
             double fv = f.x * mc + f.y * ms;
             xx += mc * (fv * (P.half_dt * dm));
             yy += ms * (fv * (P.half_dt * dm));
@@ -221,11 +294,11 @@ __global__ __launch_bounds__(kBlock) void k_move_abp(MoveArgs a)
         }
     } else {
         P4 o = a.old[i];
-        th = o.z + A.mob_rot * P.dt * (f.z);
+        th = o.z + krot * P.dt * (f.z);
         th += a.dth[i];
         th = spp::wrap_angle(th);
-        u.x = o.x + f.x * (P.dt * A.mob_perp);
-        u.y = o.y + f.y * (P.dt * A.mob_perp);
+        u.x = o.x + f.x * (P.dt * mperp);
+        u.y = o.y + f.y * (P.dt * mperp);
         double fv = f.x * mc + f.y * ms;
         u.x += mc * (fv * (P.dt * dm));
         u.y += ms * (fv * (P.dt * dm));

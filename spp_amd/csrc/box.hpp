@@ -172,7 +172,8 @@ public:
         f.t = box->t;
         f.Lx = box->params_.Lx;
         f.Ly = box->params_.Ly;
-        f.nb = 1;
+        // the first swimmer's bead count (parallel/beadbox.h:463)
+        f.nb = box->params_.model == SPP_MODEL_ABP && box->params_.nb_swimmer >= 2 ? box->params_.nb_swimmer : 1;
         // continuous particles write the wrapped r (BasicDynamicParticle::Write), the others
         // r_original (RepulsiveParticle::Write, ActiveBrownianChain::Write); swimmers without
         // self-propulsion write no heading (particle.h:1038-1045)

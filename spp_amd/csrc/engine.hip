@@ -370,7 +370,7 @@ __global__ __launch_bounds__(kBlock) void k_rank_permute(PermuteArgs a, const in
 // ------------------------------------------------------------------------------------------------
 constexpr int kSortCap = 2048;     // particles one tile can hold (LDS)
 constexpr int kSortMaxW = 1024;    // cells per tile (LDS histogram; kBlock * kScanItems)
-constexpr int kSortTarget = 384;   // particles per tile the host aims for
+constexpr int kSortTarget = 768;   // particles per tile the host aims for
 constexpr int kSpinLimit = 1 << 20;
 
 struct SortArgs {
@@ -397,6 +397,36 @@ __device__ __forceinline__ void tile_store(unsigned long long* p, unsigned long 
 }
 
 __device__ __forceinline__ int clamp_slot(int v, int N) { return v < 0 ? 0 : (v > N ? N : v); }
+
+// Slot of tile entry q inside the tile: its cell's start + the number of cell mates with a smaller
+// caller index (the general path's canonical order); src = its slot in the input order.
+__device__ __forceinline__ void tile_rank(int q, const uint16_t* order, const uint16_t* e_col, const int* e_id,
+                                          const int* e_old, const int* hist, int& lslot, int& mine, int& src)
+{
+    const int e = order[q];
+    const int c = e_col[e], c_s = hist[c], c_e = hist[c + 1];
+    mine = e_id[e];
+    int r = 0;
+    for (int k = c_s; k < c_e; ++k) r += e_id[order[k]] < mine;
+    lslot = c_s + r;
+    src = e_old[e];
+}
+
+// Moves one particle's state to its slot; returns 1 when the slot falls outside the engine (a
+// failed sort).
+__device__ __forceinline__ int tile_put(const SortArgs& a, int slot, int mine, double2 xy, double2 zw, double th,
+                                        double2 u)
+{
+    if (slot >= a.N) return 1;
+    double2* w = reinterpret_cast<double2*>(&a.out.pos[slot]);
+    w[0] = xy;
+    w[1] = zw;
+    a.out.th[slot] = th;
+    a.out.U[slot] = u;
+    a.out.id[slot] = mine;
+    if ((unsigned)mine < (unsigned)a.N) a.inv[mine] = slot;
+    return 0;
+}
 
 __global__ __launch_bounds__(kBlock) void k_sort_tiles(SortArgs a)
 {
@@ -450,30 +480,46 @@ __global__ __launch_bounds__(kBlock) void k_sort_tiles(SortArgs a)
     }
     const int total = p[6];
 
-    // collect the particles whose new cell lies in the tile
-    for (int q = t; q < total; q += kBlock) {
-        int o = off[0];
+    // Collect the particles whose new cell lies in the tile.  Each round requests all of a thread's
+    // candidates (kCollect) before looking at any, so a tile pays one memory round trip per round,
+    // not one per candidate.
+    constexpr int kCollect = 8;
+    for (int q0 = t; q0 < total; q0 += kBlock * kCollect) {
+        int sl[kCollect], idv[kCollect];
+        double2 xy[kCollect];
 #pragma unroll
-        for (int k = 1; k < 6; ++k) o = q >= p[k] ? off[k] : o;
-        const int s = q + o;
-        const double2 xy = *reinterpret_cast<const double2*>(&a.in.pos[s].x);
-        const int cy = cell_coord(xy.y, a.G.Ly, a.G.Ly2, H, a.G.quirk);
-        if (cy != y) continue;
-        const int cx = cell_coord(xy.x, a.G.Lx, a.G.Lx2, W, a.G.quirk);
-        if (cx < c0 || cx >= c1) continue;
-        const int e = atomicAdd(&s_n, 1);
-        if (e < kSortCap) {
-            e_old[e] = s;
-            e_col[e] = (uint16_t)(cx - c0);
-            e_id[e] = a.in.id[s];
-            atomicAdd(&hist[cx - c0], 1);
+        for (int u = 0; u < kCollect; ++u) {
+            const int q = q0 + u * kBlock;
+            int o = off[0];
+#pragma unroll
+            for (int k = 1; k < 6; ++k) o = q >= p[k] ? off[k] : o;
+            sl[u] = q < total ? q + o : -1;
+            if (sl[u] >= 0) {
+                xy[u] = *reinterpret_cast<const double2*>(&a.in.pos[sl[u]].x);
+                idv[u] = a.in.id[sl[u]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kCollect; ++u) {
+            if (sl[u] < 0) continue;
+            const int cy = cell_coord(xy[u].y, a.G.Ly, a.G.Ly2, H, a.G.quirk);
+            if (cy != y) continue;
+            const int cx = cell_coord(xy[u].x, a.G.Lx, a.G.Lx2, W, a.G.quirk);
+            if (cx < c0 || cx >= c1) continue;
+            const int e = atomicAdd(&s_n, 1);
+            if (e < kSortCap) {
+                e_old[e] = sl[u];
+                e_col[e] = (uint16_t)(cx - c0);
+                e_id[e] = idv[u];
+                atomicAdd(&hist[cx - c0], 1);
+            }
         }
     }
     __syncthreads();
     const int n = min(s_n, kSortCap);
     if (t == 0) {
         if (s_n > kSortCap) s_bad = 1;
-        // publish the count at once, so later tiles can look past this one
+        // publish the count at once, so later tiles can sum it
         tile_store(&a.tstate[tile], ((unsigned long long)(tag | 1u) << 32) | (unsigned)n);
     }
 
@@ -499,19 +545,52 @@ __global__ __launch_bounds__(kBlock) void k_sort_tiles(SortArgs a)
         const int c = e_col[e];
         order[hist[c] + atomicAdd(&fillc[c], 1)] = (uint16_t)e;
     }
+    __syncthreads();
 
-    // The tile's first slot: the sum of the counts of all earlier tiles, read by every thread at
-    // once (a tile whose count is not published yet is read again, a bounded number of times).
+    // Each particle's slot inside the tile: its cell's start + the number of cell mates with a
+    // smaller caller index (the general path's canonical order).  The first kMove particles of each
+    // thread have their state requested now, so those loads overlap the wait for the tile's base.
+    // The first two particles of each thread: slots inside the tile and their state, requested now
+    // so the loads overlap the wait for the tile's first slot (lanes without a particle read slot 0
+    // and store nothing: unconditional loads keep the state in registers).
+    int ls0 = -1, id0 = 0, sr0 = 0, ls1 = -1, id1 = 0, sr1 = 0;
+    if (t < n) tile_rank(t, order, e_col, e_id, e_old, hist, ls0, id0, sr0);
+    if (t + kBlock < n) tile_rank(t + kBlock, order, e_col, e_id, e_old, hist, ls1, id1, sr1);
+    const double2 xy0 = reinterpret_cast<const double2*>(&a.in.pos[sr0])[0];
+    const double2 zw0 = reinterpret_cast<const double2*>(&a.in.pos[sr0])[1];
+    const double th0 = a.in.th[sr0];
+    const double2 u0 = a.in.U[sr0];
+    const double2 xy1 = reinterpret_cast<const double2*>(&a.in.pos[sr1])[0];
+    const double2 zw1 = reinterpret_cast<const double2*>(&a.in.pos[sr1])[1];
+    const double th1 = a.in.th[sr1];
+    const double2 u1 = a.in.U[sr1];
+
+    // The tile's first slot: the sum of the counts of all earlier tiles, all of a thread's reads
+    // issued together; tiles whose count is not published yet are read again, a bounded number of
+    // times.
     {
+        constexpr int kPre = 4;
         long long part = 0;
         bool bad = false;
-        for (int j = t; j < tile && !bad; j += kBlock) {
+        for (int j0 = t; j0 < tile && !bad; j0 += kBlock * kPre) {
+            unsigned long long st[kPre];
+            bool ready[kPre];
+#pragma unroll
+            for (int u = 0; u < kPre; ++u) ready[u] = j0 + u * kBlock >= tile;
             for (int spins = 0;; ++spins) {
-                const unsigned long long st = tile_load(&a.tstate[j]);
-                if ((unsigned)(st >> 32) == (tag | 1u)) {
-                    part += (unsigned)st;
-                    break;
+#pragma unroll
+                for (int u = 0; u < kPre; ++u)
+                    if (!ready[u]) st[u] = tile_load(&a.tstate[j0 + u * kBlock]);
+                bool all = true;
+#pragma unroll
+                for (int u = 0; u < kPre; ++u) {
+                    if (!ready[u] && (unsigned)(st[u] >> 32) == (tag | 1u)) {
+                        ready[u] = true;
+                        part += (unsigned)st[u];
+                    }
+                    all = all && ready[u];
                 }
+                if (all) break;
                 if (spins >= kSpinLimit) {
                     bad = true;
                     break;
@@ -537,26 +616,18 @@ __global__ __launch_bounds__(kBlock) void k_sort_tiles(SortArgs a)
     __syncthreads();
     const int base = s_base;
 
-    // cell starts of the tile's cells, then every particle to its slot: cell start + the number of
-    // cell mates with a smaller caller index (the general path's canonical order)
     for (int c = t; c < w; c += kBlock) a.cs_new[y * W + c0 + c] = clamp_slot(base + hist[c], N);
     if (tile == a.ntiles - 1 && t == 0) a.cs_new[H * W] = clamp_slot(base + n, N);
-    for (int p = t; p < n; p += kBlock) {
-        const int e = order[p];
-        const int c = e_col[e], c_s = hist[c], c_e = hist[c + 1], mine = e_id[e];
-        int r = 0;
-        for (int q = c_s; q < c_e; ++q) r += e_id[order[q]] < mine;
-        const int slot = base + c_s + r;
-        if (slot >= N) {
-            s_bad = 1;
-            continue;
-        }
-        const int src = e_old[e];
-        a.out.pos[slot] = a.in.pos[src];
-        a.out.th[slot] = a.in.th[src];
-        a.out.U[slot] = a.in.U[src];
-        a.out.id[slot] = mine;
-        if ((unsigned)mine < (unsigned)N) a.inv[mine] = slot;
+    int put_bad = 0;
+    if (ls0 >= 0) put_bad |= tile_put(a, base + ls0, id0, xy0, zw0, th0, u0);
+    if (ls1 >= 0) put_bad |= tile_put(a, base + ls1, id1, xy1, zw1, th1, u1);
+    if (put_bad) s_bad = 1;
+    // particles beyond two per thread (crowded tiles)
+    for (int q = t + 2 * kBlock; q < n; q += kBlock) {
+        int ls, me, sr;
+        tile_rank(q, order, e_col, e_id, e_old, hist, ls, me, sr);
+        const double2* r = reinterpret_cast<const double2*>(&a.in.pos[sr]);
+        if (tile_put(a, base + ls, me, r[0], r[1], a.in.th[sr], a.in.U[sr])) s_bad = 1;
     }
     __syncthreads();
     if (t == 0 && s_bad) atomicOr(a.fail, 1);
@@ -798,10 +869,13 @@ __global__ __launch_bounds__(kChunk) void k_nlist_chunk(NlistArgs a)
 // Constants of the single-bead active Brownian model (zero for the Vicsek models).
 struct AbpConst {
     int n_membrane;     // ring beads: caller indices [0, n_membrane)
-    int pad;
+    int nb;             // beads per swimmer (1: single-bead swimmers)
     double F0;          // swimmers' self-propulsion force
-    double mob_par, mob_perp, mob_rot, torque0;
+    double mob_par, mob_perp, mob_rot, torque0;  // ring beads (and single-bead swimmers)
+    double c_par, c_perp, c_rot;                 // chains of nb beads (Set_nb, particle.h:856-876)
     double k_bond, l_bond, kappa;
+    // bead-pair law of chains: R12_Repulsive_Truncated(sigma_b, rc_b, A_b), predicate d2 < T_b
+    double sep, sigma_b, A_b, rc13_b, T_b;
 };
 
 struct StageArgs {
@@ -2023,9 +2097,34 @@ void dfree(void* p)
     if (p) (void)hipFree(p);
 }
 
+// ActiveBrownianChain::Set_nb (shared/particle.h:856-876): mobilities of a chain of nb beads
+// (single beads 0.5 / 0.5 / 1; longer chains from the aspect ratio of a prolate spheroid).
+void chain_mobilities(int nb, double sep, double sigma, double& mpar, double& mperp, double& kperp)
+{
+    if (nb <= 1) {
+        mpar = 0.5;
+        mperp = 0.5;
+        kperp = 1;
+        return;
+    }
+    const double aspect_ratio = 1 / ((sep * (nb - 1) / sigma) + 1);
+    const double e2 = 1 - aspect_ratio * aspect_ratio;
+    const double ecc = std::sqrt(e2);
+    const double L = std::log((1 + ecc) / (1 - ecc));
+    mpar = 1.0;
+    mperp = 0.5 * (2 * ecc + (3 * e2 - 1) * L) / ((1 + e2) * L - 2 * ecc);
+    kperp = 6 / (1 + aspect_ratio * aspect_ratio) / sigma / sigma;
+}
+
 double max_cutoff(const spp_params& p)
 {
-    if (p.model == SPP_MODEL_ABP) return p.repulsion != SPP_REP_NONE ? p.rc_rep : 0.0;
+    if (p.model == SPP_MODEL_ABP) {
+        double c = p.repulsion != SPP_REP_NONE ? p.rc_rep : 0.0;
+        // two chains interact while bead pairs, up to bead_sep (nb - 1) / 2 from either centre,
+        // are closer than rc_bead: the list covers centre distances up to rc_bead + bead_sep (nb - 1)
+        if (p.nb_swimmer >= 2) c = std::max(c, p.rc_bead + p.bead_sep * (p.nb_swimmer - 1));
+        return c;
+    }
     double c = p.R_align;
     if (p.model == SPP_MODEL_REPULSIVE && p.repulsion != SPP_REP_NONE && p.rc_rep > c) c = p.rc_rep;
     return c;
@@ -2838,6 +2937,12 @@ int spp_params_default(spp_params* p, int32_t model)
         p->v0 = 1;
         p->r_list = 0.8;
     }
+    // multi-bead swimmers (particle.h:1048-1056): single beads unless nb_swimmer >= 2
+    p->nb_swimmer = 1;
+    p->bead_sep = 1;     // ActiveBrownianChain::separation
+    p->sigma_bead = 1;   // sigma_p
+    p->rc_bead = 1.1;    // repulsion_radius
+    p->A_bead = 1;       // A_p
     if (model == SPP_MODEL_CONTINUOUS) {
         p->repulsion = SPP_REP_NONE;
         p->R_align = 1;      // d2 < 1, particle.h:305
@@ -2877,6 +2982,9 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
             return fail(nullptr, SPP_ERR_ARG, "n_membrane must be 0 or in [3, n]");
         if (p->n_membrane > 0 && !(p->l_bond > 0))
             return fail(nullptr, SPP_ERR_ARG, "l_bond must be positive");
+        if (p->nb_swimmer >= 2 &&
+            (p->nb_swimmer > 16 || !(p->bead_sep > 0) || !(p->sigma_bead > 0) || !(p->rc_bead > 0)))
+            return fail(nullptr, SPP_ERR_ARG, "chains need 2 <= nb_swimmer <= 16 and positive bead_sep, sigma_bead, rc_bead");
     }
     if (p->repulsion < SPP_REP_NONE || p->repulsion > SPP_REP_YUKAWA)
         return fail(nullptr, SPP_ERR_ARG, "repulsion %d", p->repulsion);
@@ -3026,6 +3134,7 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
     }
     if (p->model == SPP_MODEL_ABP) {
         e->abp.n_membrane = p->n_membrane;
+        e->abp.nb = p->nb_swimmer >= 2 ? p->nb_swimmer : 1;
         e->abp.F0 = p->v0;
         e->abp.mob_par = p->mob_par;
         e->abp.mob_perp = p->mob_perp;
@@ -3034,6 +3143,13 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
         e->abp.k_bond = p->k_bond;
         e->abp.l_bond = p->l_bond;
         e->abp.kappa = p->kappa_bend;
+        chain_mobilities(e->abp.nb, p->bead_sep, p->sigma_bead, e->abp.c_par, e->abp.c_perp, e->abp.c_rot);
+        e->abp.sep = p->bead_sep;
+        e->abp.sigma_b = p->sigma_bead;
+        e->abp.A_b = p->A_bead;
+        const double sc = p->sigma_bead / p->rc_bead, sc2 = sc * sc, sc4 = sc2 * sc2, sc8 = sc4 * sc4;
+        e->abp.rc13_b = sc8 * sc4 * sc;
+        e->abp.T_b = sqrt_threshold(p->rc_bead);
     }
     e->T_list = sqrt_threshold(p->r_list);
     e->r_cut = rcut;

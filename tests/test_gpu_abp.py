@@ -182,3 +182,90 @@ def test_abp_rejects_unsupported():
         S.Engine(p.copy().update(kernel_variant=S.KERNEL_STRIP_LDS))
     with pytest.raises(S.SppError):
         S.Engine(p, [S.system(32, 16.0)])
+
+
+# ---------------------------------------------------------------------------------------------
+# Chains of several beads (nb > 1) inside a ring membrane
+# ---------------------------------------------------------------------------------------------
+
+def chain_vesicle_params(nm, ns, R, nb, **kw):
+    """Ring beads with harmonic mutual repulsion and bonds (k = 100, bending 10); swimmers are chains
+    of nb beads one unit apart with the reference's bead law (R12 truncated, sigma 1, cutoff 1.1,
+    amplitude 1) and Set_nb mobilities; RK2 dt = 1e-3, rotational diffusion 0.1."""
+    p = S.default_params(S.MODEL_ABP)
+    p.update(n=nm + ns, n_membrane=nm, Lx=R + 6, Ly=R + 6, integrator=S.RK2, dt=1e-3,
+             repulsion=S.REP_HARMONIC, A_rep=10.0, sigma_rep=1.0, rc_rep=1.0, D_rot=0.1, v0=2.0,
+             mob_par=0.5, mob_perp=0.5, mob_rot=1.0, k_bond=100.0, l_bond=1.0, kappa_bend=10.0, seed=11,
+             nb_swimmer=nb, bead_sep=1.0, sigma_bead=1.0, rc_bead=1.1, A_bead=1.0,
+             r_list=1.1 + (nb - 1) * 1.0 + 0.4)
+    return p.update(**kw)
+
+
+def chain_lattice_state(nm, R, spacing, seed):
+    """Ring of nm beads at radius R (bonds at rest length 1 for R = 0.5 / sin(pi / nm)); chains on a
+    square lattice inside radius R - spacing / 2, random headings (no initial bead overlaps)."""
+    a = 2 * np.pi * np.arange(nm) / nm
+    g = np.arange(-R, R + 1e-9, spacing)
+    gx, gy = [v.ravel() for v in np.meshgrid(g, g)]
+    keep = np.hypot(gx, gy) < R - spacing / 2
+    gx, gy = gx[keep], gy[keep]
+    rng = np.random.default_rng(seed)
+    x = np.concatenate([R * np.cos(a), gx + rng.uniform(-0.2, 0.2, gx.size)])
+    y = np.concatenate([R * np.sin(a), gy + rng.uniform(-0.2, 0.2, gy.size)])
+    th = np.concatenate([np.zeros(nm), rng.uniform(-np.pi, np.pi, gx.size)])
+    return x, y, th
+
+
+@pytest.mark.parametrize("nb", [2, 3])
+def test_chain_forces_match_oracle(oracle_mod, nb):
+    """One interaction of a ring plus randomly placed, overlapping chains: forces and torques
+    (bead-pair torques s x F included) within 1e-12 of the oracle."""
+    nm, ns = 128, 700
+    R = 0.5 / math.sin(math.pi / nm)
+    p = chain_vesicle_params(nm, ns, R, nb)
+    x, y, th = vesicle_state(nm, ns, R, 5)
+    e, o = pair(p, x, y, th, oracle_mod)
+    e.interact()
+    o.interact()
+    gf, of = e.get_forces(), o.get_forces()
+    assert vec_rel_err(gf[0], gf[1], of[0], of[1]) < FORCE_TOL
+    tau_err = np.linalg.norm(gf[2] - of[2]) / np.linalg.norm(of[2])
+    assert np.linalg.norm(of[2][nm:]) > 0 and tau_err < FORCE_TOL, tau_err
+
+
+@pytest.mark.parametrize("nb,integ", [(2, S.RK2), (3, S.RK2), (3, S.EULER)])
+def test_chain_trajectory_matches_oracle(oracle_mod, nb, integ):
+    """100 steps of chains swimming inside a bending ring, against the oracle under the identical
+    noise stream, within 1e-9."""
+    nm = 256
+    R = 0.5 / math.sin(math.pi / nm)
+    x, y, th = chain_lattice_state(nm, R, nb + 0.6, 9)
+    p = chain_vesicle_params(nm, x.size - nm, R, nb, integrator=integ, v0=5.0)
+    e, o = pair(p, x, y, th, oracle_mod)
+    e.multi_step(100)
+    o.multi_step(100)
+    assert traj_err(e, o) < TRAJ_TOL
+
+
+def test_chain_split_stages_match_oracle(oracle_mod):
+    """Box::Interact + Move_Runge_Kutta2_1/2 one call at a time with chains: every stage's forces,
+    torques and state against the oracle."""
+    nm = 128
+    R = 0.5 / math.sin(math.pi / nm)
+    x, y, th = chain_lattice_state(nm, R, 2.6, 4)
+    p = chain_vesicle_params(nm, x.size - nm, R, 2, v0=5.0, noise_mode=S.NOISE_INJECTED)
+    e, o = pair(p, x, y, th, oracle_mod)
+    rng = np.random.default_rng(3)
+    for _ in range(3):
+        dn = rng.normal(0, 0.05, p.n)
+        e.inject_noise(dn)
+        o.set_noise(dn)
+        for st in (1, 2):
+            e.interact()
+            o.interact()
+            gf, of = e.get_forces(), o.get_forces()
+            assert vec_rel_err(gf[0], gf[1], of[0], of[1]) < FORCE_TOL
+            assert np.abs(gf[2] - of[2]).max() <= FORCE_TOL * max(1.0, np.abs(of[2]).max())
+            e.move_stage(st)
+            o.move_stage(st)
+    assert traj_err(e, o) < 1e-12

@@ -132,3 +132,107 @@ def test_chiral_swimmer_circle(oracle_mod):
     c = np.array([0.0, Rd])
     r = np.hypot(pts[:, 0] - c[0], pts[:, 1] - c[1])
     assert np.abs(r - Rd).max() < 1e-10
+
+
+# ---------------------------------------------------------------------------------------------
+# Chains of several beads (ActiveBrownianChain with nb > 1, particle.h:856-876, 983-1011)
+# ---------------------------------------------------------------------------------------------
+
+def _r12_energy(d, sigma, rc, A):
+    """Potential of R12_Repulsive_Truncated: -dU/dd = A [(sigma/d)^13 - (sigma/rc)^13], U(rc) = 0."""
+    u = lambda r: A * (sigma / 12.0 * (sigma / r) ** 12 + (sigma / rc) ** 13 * r)
+    return np.where(d < rc, u(d) - u(rc), 0.0)
+
+
+def _chain_energy(x, y, th, nbead, p, L):
+    """Total pair energy: bead pairs of chains (and chain-single pairs) with the bead law, pairs of
+    two single beads with the configured R12 law; minimum image on the bead separation of centres."""
+    n = x.size
+    E = 0.0
+    for i in range(n):
+        for j in range(i + 1, n):
+            dx0 = x[i] - x[j]
+            dy0 = y[i] - y[j]
+            dx0 -= 2 * L * np.round(dx0 / (2 * L))
+            dy0 -= 2 * L * np.round(dy0 / (2 * L))
+            if nbead[i] == 1 and nbead[j] == 1:
+                E += _r12_energy(np.hypot(dx0, dy0), p.sigma_rep, p.rc_rep, p.A_rep)
+                continue
+            oi = p.bead_sep * ((1 - nbead[i]) / 2.0 + np.arange(nbead[i]))
+            oj = p.bead_sep * ((1 - nbead[j]) / 2.0 + np.arange(nbead[j]))
+            bx = dx0 + np.cos(th[i]) * oi[:, None] - np.cos(th[j]) * oj[None, :]
+            by = dy0 + np.sin(th[i]) * oi[:, None] - np.sin(th[j]) * oj[None, :]
+            E += _r12_energy(np.hypot(bx, by), p.sigma_bead, p.rc_bead, p.A_bead).sum()
+    return E
+
+
+def test_chain_forces_and_torques_are_minus_energy_gradient(oracle_mod):
+    """Chains of three beads and single ring beads (bonds off) in near contact: the oracle's forces
+    are -dE/dr and its torques -dE/dtheta of the bead-pair energy (central differences); this pins
+    the bead offsets, both laws, the class rule and the s x F torques."""
+    L = 6.0
+    p = abp_params(n=9, n_membrane=3, Lx=L, Ly=L, v0=0.0, torque0=0.0, k_bond=0.0, kappa_bend=0.0,
+                   nb_swimmer=3, bead_sep=0.9, sigma_bead=1.0, rc_bead=1.1, A_bead=1.3,
+                   repulsion=S.REP_R12_TRUNCATED, sigma_rep=0.5, rc_rep=0.55, A_rep=0.7, r_list=4.0)
+    rng = np.random.default_rng(3)
+    # ring beads: two overlapping singles and one touching a chain; six chains around the centre
+    x = np.array([-4.0, -3.55, 2.6, 0.0, 1.4, -1.3, 0.3, 3.9, -2.5])
+    y = np.array([-4.0, -4.05, 3.3, 0.0, 0.6, 0.9, -1.5, 4.2, 2.4])
+    th = rng.uniform(-np.pi, np.pi, 9)
+    o = oracle_mod.Oracle(p)
+    o.set_state(x, y, th)
+    o.interact()
+    fx, fy, tau = o.get_forces()
+    nbead = np.array([1, 1, 1, 3, 3, 3, 3, 3, 3])
+    h = 1e-6
+    gx, gy, gt = np.zeros(9), np.zeros(9), np.zeros(9)
+    for i in range(9):
+        for arr, g in ((x, gx), (y, gy), (th, gt)):
+            a0 = arr[i]
+            arr[i] = a0 + h
+            ep = _chain_energy(x, y, th, nbead, p, L)
+            arr[i] = a0 - h
+            em = _chain_energy(x, y, th, nbead, p, L)
+            arr[i] = a0
+            g[i] = -(ep - em) / (2 * h)
+    scale = max(np.abs(gx).max(), np.abs(gy).max(), np.abs(gt).max())
+    assert scale > 1.0  # the configuration has real contacts
+    assert np.abs(fx - gx).max() < 1e-6 * scale and np.abs(fy - gy).max() < 1e-6 * scale
+    assert np.abs(tau - gt).max() < 1e-6 * scale
+    assert abs(fx.sum()) < 1e-12 * scale and abs(fy.sum()) < 1e-12 * scale  # Newton 3
+
+
+def test_chain_mobility_limits(oracle_mod):
+    """Set_nb's mobilities: single beads 0.5 / 0.5 / 1; a chain shrinking to a sphere (separation
+    -> 0) becomes isotropic, m_perp -> m_par = 1 and k_perp -> 3 / sigma^2; a slender chain (aspect
+    ratio -> 0) moves sideways at half its lengthwise mobility (slender-body limit)."""
+    assert oracle_mod.chain_mobilities(1, 1.0, 1.0) == (0.5, 0.5, 1.0)
+    mpar, mperp, krot = oracle_mod.chain_mobilities(2, 1e-5, 1.0)
+    assert mpar == 1.0 and abs(mperp - 1.0) < 1e-4 and abs(krot - 3.0) < 1e-4
+    # slender chains: the ratio falls towards 1/2 with corrections of order 1 / ln(aspect ratio)
+    r = [oracle_mod.chain_mobilities(16, sep, 1.0) for sep in (10.0, 100.0, 1000.0)]
+    ratios = [m[1] / m[0] for m in r]
+    assert ratios[0] > ratios[1] > ratios[2] > 0.5 and ratios[2] < 0.56
+    # monotone: longer chains turn slower and drift sideways less
+    m = [oracle_mod.chain_mobilities(nb, 1.0, 1.0) for nb in range(2, 8)]
+    assert all(a[1] > b[1] for a, b in zip(m, m[1:])) and all(a[2] < b[2] for a, b in zip(m, m[1:]))
+
+
+def test_free_chain_moves_at_its_parallel_mobility(oracle_mod):
+    """A chain without neighbours and noise: one Euler step moves it m_par F0 dt along its new
+    heading, which turns by k_perp dt torque0."""
+    p = abp_params(n=1, n_membrane=0, Lx=6.0, Ly=6.0, v0=2.0, torque0=0.3, integrator=S.EULER, dt=0.01,
+                   nb_swimmer=3, bead_sep=1.0, sigma_bead=1.0, rc_bead=1.1, r_list=3.5)
+    mpar, mperp, krot = oracle_mod.chain_mobilities(3, 1.0, 1.0)
+    o = oracle_mod.Oracle(p)
+    o.set_state(np.array([0.2]), np.array([-0.1]), np.array([0.7]))
+    o.multi_step(1)
+    st = o.get_state()
+    th1 = 0.7 + krot * 0.01 * 0.3
+    assert abs(st["theta"][0] - th1) < 1e-15
+    # the step projects f = F0 v_old on the new heading: displacement m_perp f dt + (m_par - m_perp)(f.v) v dt
+    v0 = np.array([math.cos(0.7), math.sin(0.7)])
+    v1 = np.array([math.cos(th1), math.sin(th1)])
+    f = 2.0 * v0
+    want = np.array([0.2, -0.1]) + mperp * f * 0.01 + (mpar - mperp) * (f @ v1) * v1 * 0.01
+    assert abs(st["xu"][0] - want[0]) < 1e-15 and abs(st["yu"][0] - want[1]) < 1e-15
