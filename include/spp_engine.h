@@ -239,6 +239,16 @@ int spp_get_cells(spp_engine* e, int32_t* cell_x, int32_t* cell_y, int32_t n);
 int spp_interact(spp_engine* e);
 /* f (including the self-propulsion term) and tau of the last spp_interact, original order.       */
 int spp_get_forces(spp_engine* e, double* fx, double* fy, double* tau, int32_t n);
+/* The Particle-level surface, for callers that loop over pairs themselves.
+   spp_reset_forces = Particle::Reset for every particle (particle.h:559-564, 878-883): f = v0 v
+   (F0 v for swimmers, 0 for ring beads), torque 0 (torque0 for SPP_MODEL_ABP).
+   spp_pair_interact = particles[i].Interact(particles[j]) (particle.h:728-757, 296-338, 964-1012):
+   adds the pair's force and torque to both particles' accumulated ones, whatever their distance
+   (the law's own cutoffs apply).  Ring bonds and bending are Box-level (spp_interact) only.  The
+   accumulated forces are what spp_get_forces reads and spp_move_stage applies, so
+   reset + pair calls + move stages is the reference's hand-written Interact loop.                */
+int spp_reset_forces(spp_engine* e);
+int spp_pair_interact(spp_engine* e, int32_t i, int32_t j);
 /* Noise for the next step (original order): replaces Noise_Gen (particle.h:566-569) when
    noise_mode == SPP_NOISE_INJECTED.                                                              */
 int spp_inject_noise(spp_engine* e, const double* dtheta, int32_t n);
@@ -313,6 +323,22 @@ int spp_lyapunov_vectors(spp_engine* e, int32_t replica, double* out, int64_t le
    step counter (the counter-based stream makes replay exact).                                    */
 int spp_snapshot(spp_engine* e, int32_t slot);
 int spp_restore(spp_engine* e, int32_t slot);
+/* What a host-side copy of the state needs besides positions and headings to replay a run bit for
+   bit.  Box::Load(const State_Hyper_Vector&) restores the saved copy of the random generator with
+   the state (Set_C2DVector_Rand_Generator, state-hyper-vector.h:169-172, parallel/box.h:236); with
+   the counter-based stream that copy is the step counter.  The rebuild estimates decide at which
+   steps the neighbour list is rebuilt under SPP_REBUILD_DISPLACEMENT, and with that the order in
+   which pair terms are summed (forces agree to rounding for any schedule).                       */
+typedef struct spp_clock {
+    int64_t step;               /* noise-stream step counter (spp_info.steps)                          */
+    double t;                   /* simulated time                                                      */
+    double rebuild_estimate[3]; /* last three estimates of the steps until a particle crosses the half
+                                   skin (0 = none yet)                                                 */
+} spp_clock;
+int spp_get_clock(spp_engine* e, spp_clock* clock);
+/* SPP_ERR_ARG for a negative step or a negative or non-finite estimate; SPP_ERR_STATE inside a
+   split step.  A host-side state restored with spp_set_state plus this call replays the run.      */
+int spp_set_clock(spp_engine* e, const spp_clock* clock);
 
 /* ---- timing -------------------------------------------------------------------------------------- */
 
@@ -339,6 +365,13 @@ int spp_time_stage(spp_engine* e, int32_t stage, int32_t reps, double* avg_us);
    (Random_Formation, shared/set-up.h:152-163, with a counter-based generator).                   */
 int spp_random_state(uint64_t seed, int32_t n, double Lx, double Ly, double* x, double* y,
                      double* theta);
+/* Random_Formation(particle, N, sigma) (set-up.h:152-163): positions uniform in
+   [-(Lx - sigma), Lx - sigma) x [-(Ly - sigma), Ly - sigma), headings uniform in [0, 2 pi);
+   sigma = 0 gives spp_random_state's positions.  0 <= sigma < min(Lx, Ly).                       */
+int spp_random_formation(uint64_t seed, int32_t n, double Lx, double Ly, double sigma, double* x, double* y,
+                         double* theta);
+/* Polar_Formation (set-up.h:181-192): Random_Formation's positions, every heading 0 (v = (1, 0)). */
+int spp_polar_formation(uint64_t seed, int32_t n, double Lx, double Ly, double* x, double* y, double* theta);
 
 /* Triangle_Lattice_Formation (shared/set-up.h:116-145): n sites of a triangular lattice with
    spacing (2 Lx - sigma) / Nx, Nx = (int)sqrt(Lx n sqrt(3) / (2 Ly)) + 1, offset by -(L - sigma)

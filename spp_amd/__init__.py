@@ -113,6 +113,16 @@ class Info(C.Structure):
     ]
 
 
+class Clock(C.Structure):
+    """Mirror of ``spp_clock``."""
+
+    _fields_ = [
+        ("step", C.c_int64),
+        ("t", C.c_double),
+        ("rebuild_estimate", C.c_double * 3),
+    ]
+
+
 class System(C.Structure):
     """Mirror of ``spp_system`` (one box of a batched engine)."""
 
@@ -165,11 +175,17 @@ SIGNATURES = {
     "spp_snapshot": (C.c_int, [C.c_void_p, C.c_int32]),
     "spp_restore": (C.c_int, [C.c_void_p, C.c_int32]),
     "spp_synchronize": (C.c_int, [C.c_void_p]),
+    "spp_get_clock": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "spp_set_clock": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "spp_reset_forces": (C.c_int, [C.c_void_p]),
+    "spp_pair_interact": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "spp_graph_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
     "spp_profile_enable": (C.c_int, [C.c_void_p, C.c_int32]),
     "spp_profile_read": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, _D, _I64, C.c_int32, _I32]),
     "spp_time_stage": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _D]),
     "spp_random_state": (C.c_int, [C.c_uint64, C.c_int32, C.c_double, C.c_double, _D, _D, _D]),
+    "spp_random_formation": (C.c_int, [C.c_uint64, C.c_int32, C.c_double, C.c_double, C.c_double, _D, _D, _D]),
+    "spp_polar_formation": (C.c_int, [C.c_uint64, C.c_int32, C.c_double, C.c_double, _D, _D, _D]),
     "spp_noise_normal": (C.c_double, [C.c_uint64, C.c_int32, C.c_int64]),
     "spp_triangle_lattice": (C.c_int, [C.c_int32, C.c_double, C.c_double, C.c_double, _D, _D]),
     "spp_ring_membrane": (C.c_int, [C.c_int32, C.c_double, C.c_double, _D, _D]),
@@ -221,6 +237,24 @@ def random_state(seed: int, n: int, Lx: float, Ly: float):
     rc = lib().spp_random_state(seed, n, Lx, Ly, _ptr(x), _ptr(y), _ptr(th))
     if rc:
         raise SppError(rc, "spp_random_state")
+    return x, y, th
+
+
+def random_formation(seed: int, n: int, Lx: float, Ly: float, sigma: float = 0.0):
+    """Random_Formation with the sigma inset (positions within L - sigma of the centre)."""
+    x, y, th = np.empty(n), np.empty(n), np.empty(n)
+    rc = lib().spp_random_formation(seed, n, Lx, Ly, sigma, _ptr(x), _ptr(y), _ptr(th))
+    if rc:
+        raise SppError(rc, "spp_random_formation")
+    return x, y, th
+
+
+def polar_formation(seed: int, n: int, Lx: float, Ly: float):
+    """Polar_Formation: random positions, every heading 0."""
+    x, y, th = np.empty(n), np.empty(n), np.empty(n)
+    rc = lib().spp_polar_formation(seed, n, Lx, Ly, _ptr(x), _ptr(y), _ptr(th))
+    if rc:
+        raise SppError(rc, "spp_polar_formation")
     return x, y, th
 
 
@@ -443,6 +477,24 @@ class Engine:
 
     def restore(self, slot: int = 0):
         self._check(self._lib.spp_restore(self._h, slot))
+
+    def reset_forces(self):
+        """Particle::Reset of every particle (the split path's force accumulators)."""
+        self._check(self._lib.spp_reset_forces(self._h))
+
+    def pair_interact(self, i: int, j: int):
+        """particles[i].Interact(particles[j]): adds the pair's force and torque to both."""
+        self._check(self._lib.spp_pair_interact(self._h, i, j))
+
+    def get_clock(self) -> Clock:
+        """Noise-stream step, time and rebuild estimates: with get_state, a copy that replays."""
+        c = Clock()
+        self._check(self._lib.spp_get_clock(self._h, C.byref(c)))
+        return c
+
+    def set_clock(self, clock: Clock):
+        """The random-generator and schedule part of Box::Load(State)."""
+        self._check(self._lib.spp_set_clock(self._h, C.byref(clock)))
 
     def synchronize(self):
         self._check(self._lib.spp_synchronize(self._h))

@@ -37,7 +37,7 @@ def build_engine(verbose: bool = False) -> str:
     return LIB
 
 
-DEMOS = ("box_demo", "vesicle_demo")
+DEMOS = ("box_demo", "vesicle_demo", "state_demo")
 
 
 def build_demo(verbose: bool = False) -> list:

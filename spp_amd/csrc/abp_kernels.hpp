@@ -149,6 +149,30 @@ __device__ __forceinline__ void abp_forces(const StageArgs& a, const Phys& P0, i
     if (bead) ring_forces(a, P, pid, me, fx, fy);
 }
 
+// One pair's contribution to particle si's force and torque (caller's Particle::Interact): the
+// class rules of chain_gather for the single neighbour sj.
+template <int REP>
+__device__ __forceinline__ void abp_pair_side(const StageArgs& a, const Phys& P, int si, int sj, double& fx,
+                                              double& fy, double& tq)
+{
+    const AbpConst& A = a.abp;
+    const P4 me = a.gather[si], q = a.gather[sj];
+    double mc, ms;
+    spp::heading_unit(me.z, mc, ms);
+    const int na = A.nb > 1 && !((unsigned)a.id[si] < (unsigned)A.n_membrane) ? A.nb : 1;
+    const int nbj = A.nb > 1 && !((unsigned)a.id[sj] < (unsigned)A.n_membrane) ? A.nb : 1;
+    if (na == 1 && nbj == 1) {
+        double s1 = 0, s2 = 0;
+        spp::pair_accumulate<SPP_MODEL_REPULSIVE, REP>(P, me, mc, ms, q, fx, fy, s1, s2);
+    } else {
+        const double dx0 = spp::min_image(me.x - q.x, P.Lx2, P.invLx2);
+        const double dy0 = spp::min_image(me.y - q.y, P.Ly2, P.invLy2);
+        double cj, sj_;
+        spp::heading_unit(q.z, cj, sj_);
+        bead_pairs(A, dx0, dy0, mc, ms, na, cj, sj_, nbj, fx, fy, tq);
+    }
+}
+
 // Mobilities of a particle: (parallel, perpendicular, rotational).
 __device__ __forceinline__ void abp_mobility(const AbpConst& A, int pid, double& mpar, double& mperp, double& krot)
 {

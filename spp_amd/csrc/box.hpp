@@ -7,6 +7,7 @@
 // kernels.  Errors throw spp_gpu::Error (the reference prints and calls exit(0)).
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <fstream>
 #include <istream>
@@ -24,6 +25,118 @@ namespace spp_gpu {
 struct Error : std::runtime_error {
     int code;
     Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// State_Hyper_Vector (shared/state-hyper-vector.h:6-258) on the host: a box's state as the 3N
+// vector (r, theta) with the reference's arithmetic -- +, -, scaling and their compound forms end
+// in Periodic_Transform (minimum-image positions, theta into (-pi, pi], :83-150, 218-225); the
+// product is sum(r.r' + theta theta') (:157-167); Square wraps theta first (:201-211).  It also
+// carries
+//   * the unwrapped positions, which follow the same arithmetic without the wrap, so a deviation
+//     added to a state moves r_original with r (the reference's vector holds r only, and Load leaves
+//     r_original as it was, parallel/box.h:229-235, which loses a positional deviation after a step);
+//   * the clock (spp_clock): the noise stream's step, the counter-based stand-in for the generator
+//     copy the reference keeps with each vector (:43-57, 169-177), the time, and the displacement
+//     policy's rebuild estimates, so that a Load replays the saved run bit for bit.
+class State {
+public:
+    int N = 0;
+    double Lx = 0, Ly = 0;                   // half widths of the box the positions wrap in
+    std::vector<double> x, y, theta, xu, yu;
+    spp_clock clock{};  // noise step, time and list-rebuild schedule (exact replay)
+
+    State() = default;
+    State(int n, double lx, double ly) : N(n), Lx(lx), Ly(ly), x(n), y(n), theta(n), xu(n), yu(n) {}
+
+    State operator+(const State& s) const { return combine(s, +1.0); }
+    State operator-(const State& s) const { return combine(s, -1.0); }
+    State& operator+=(const State& s) { return *this = combine(s, +1.0); }
+    State& operator-=(const State& s) { return *this = combine(s, -1.0); }
+    State operator*(double f) const { return scaled(f, false); }
+    State operator/(double f) const { return scaled(f, true); }
+    double operator*(const State& s) const
+    {
+        check_size(s);
+        double r = 0;
+        for (int i = 0; i < N; ++i) {
+            r += x[i] * s.x[i] + y[i] * s.y[i];
+            r += theta[i] * s.theta[i];
+        }
+        return r;
+    }
+    void Null()
+    {
+        for (auto* v : {&x, &y, &theta, &xu, &yu}) std::fill(v->begin(), v->end(), 0.0);
+    }
+    double Square() const
+    {
+        double r = 0;
+        for (int i = 0; i < N; ++i) {
+            r += x[i] * x[i] + y[i] * y[i];
+            const double d = wrap_theta(theta[i]);
+            r += d * d;
+        }
+        return r;
+    }
+    double Magnitude() const { return std::sqrt(Square()); }
+    void Periodic_Transform()
+    {
+        for (int i = 0; i < N; ++i) {
+            x[i] = wrap(x[i], 2 * Lx);
+            y[i] = wrap(y[i], 2 * Ly);
+            theta[i] = wrap_theta(theta[i]);
+        }
+    }
+    void Unit() { *this = *this / Magnitude(); }
+    int Max_Index() const  // largest wrapped |theta| (state-hyper-vector.h:228-245)
+    {
+        int index = 0;
+        double mx = 0;
+        for (int i = 0; i < N; ++i) {
+            const double d = std::fabs(wrap_theta(theta[i]));
+            if (d > mx) {
+                mx = d;
+                index = i;
+            }
+        }
+        return index;
+    }
+
+private:
+    // C2DVector::Periodic_Transform, one component (c2dvector.h:67-73)
+    static double wrap(double v, double L2) { return v - L2 * (double)(int)std::floor(v / L2 + 0.5); }
+    static double wrap_theta(double th) { return th - 2 * M_PI * std::ceil((th - M_PI) / (2 * M_PI)); }
+    void check_size(const State& s) const
+    {
+        if (s.N != N) throw Error(SPP_ERR_ARG, "state vectors hold different particle numbers");
+    }
+    State combine(const State& s, double sign) const  // the result keeps this vector's clock
+    {
+        check_size(s);
+        State r = *this;
+        for (int i = 0; i < N; ++i) {
+            r.x[i] = x[i] + sign * s.x[i];
+            r.y[i] = y[i] + sign * s.y[i];
+            r.theta[i] = theta[i] + sign * s.theta[i];
+            r.xu[i] = xu[i] + sign * s.xu[i];
+            r.yu[i] = yu[i] + sign * s.yu[i];
+        }
+        r.Periodic_Transform();
+        return r;
+    }
+    State scaled(double f, bool divide) const
+    {
+        State r = *this;
+        for (int i = 0; i < N; ++i) {
+            r.x[i] = divide ? x[i] / f : x[i] * f;
+            r.y[i] = divide ? y[i] / f : y[i] * f;
+            r.theta[i] = divide ? theta[i] / f : theta[i] * f;
+            r.xu[i] = divide ? xu[i] / f : xu[i] * f;
+            r.yu[i] = divide ? yu[i] / f : yu[i] * f;
+        }
+        r.Periodic_Transform();
+        return r;
+    }
 };
 
 class Box {
@@ -104,6 +217,57 @@ public:
 
     void Save(int slot) { check(spp_snapshot(e_, slot)); }          // parallel/box.h:246-267
     void Load(int slot) { check(spp_restore(e_, slot)); sync_time(); }  // :222-243
+
+    // Box::Save(State_Hyper_Vector&) (parallel/box.h:246-267): the state and the noise clock into a
+    // host vector; re-bins first like the reference, so a Load of it reproduces the same order.
+    void Save(State& sv)
+    {
+        if (sv.N != N) throw Error(SPP_ERR_ARG, "Box::Save: the state vector holds another particle number");
+        check(spp_build_neighbors(e_));
+        Get_State(sv.x, sv.y, sv.xu, sv.yu, sv.theta);
+        check(spp_get_clock(e_, &sv.clock));
+        sv.clock.t = t;
+        sv.Lx = params_.Lx;
+        sv.Ly = params_.Ly;
+    }
+    // Box::Load(const State_Hyper_Vector&) (:222-243): positions, headings and the noise clock.
+    void Load(const State& sv)
+    {
+        if (sv.N != N) throw Error(SPP_ERR_ARG, "Box::Load: the state vector holds another particle number");
+        check(spp_set_state(e_, sv.x.data(), sv.y.data(), sv.xu.data(), sv.yu.data(), sv.theta.data(), N));
+        check(spp_set_clock(e_, &sv.clock));
+        set_time_origin(sv.clock.t);
+    }
+
+    // The Particle-level surface (shared/particle.h): particle(i).Interact(particle(j)) adds the
+    // pair's force and torque to both (Particle::Interact, :728-757, 296-338, 964-1012);
+    // Reset_Forces is Particle::Reset for all.  Moves stay Box-level (Move*, above), as every
+    // reference caller applies them (cell.h:137-147).
+    class Particle {
+    public:
+        void Interact(const Particle& other) const { box_->check(spp_pair_interact(box_->e_, i_, other.i_)); }
+        int index() const { return i_; }
+
+    private:
+        friend class Box;
+        Particle(Box* b, int i) : box_(b), i_(i) {}
+        Box* box_;
+        int i_;
+    };
+    Particle particle(int i)
+    {
+        if (i < 0 || i >= N) throw Error(SPP_ERR_ARG, "particle index out of range");
+        return Particle(this, i);
+    }
+    void Reset_Forces() { check(spp_reset_forces(e_)); }
+    // every particle's accumulated force and torque (the reference's Particle::f and torque)
+    void Get_Forces(std::vector<double>& fx, std::vector<double>& fy, std::vector<double>& torque)
+    {
+        fx.resize(N);
+        fy.resize(N);
+        torque.resize(N);
+        check(spp_get_forces(e_, fx.data(), fy.data(), torque.data(), N));
+    }
 
     void Compute_Polarization()                                     // serial/box.h:531-538
     {

@@ -1768,6 +1768,83 @@ InteractFn interact_fn(int variant)
     return k_interact_flat<MODEL, REP, false>;
 }
 
+// Particle::Reset for every particle (particle.h:559-564, 878-883): f = v0 v (F0 v for swimmers,
+// 0 for ring beads), torque 0 (torque0 for the active Brownian model), into the force buffer that
+// the split-call moves apply.
+__global__ __launch_bounds__(kBlock) void k_reset_forces(StageArgs a, int abp, P4* force)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.N) return;
+    Phys P = a.P;
+    if (a.sys_of) apply_sys(P, a.sys_tab[a.sys_of[i]]);
+    double mc, ms;
+    spp::heading_unit(a.gather[i].z, mc, ms);
+    double v0 = P.v0, tq = 0.0;
+    if (abp) {
+        const bool bead = (unsigned)a.id[i] < (unsigned)a.abp.n_membrane;
+        v0 = bead ? 0.0 : a.abp.F0;
+        tq = a.abp.torque0;
+    }
+    force[i] = P4{mc * v0, ms * v0, tq, 0.0};
+}
+
+// Particle::Interact(Particle&) for one pair of slots (particle.h:728-757, 296-338, 964-1012): each
+// side's force and torque are added to its accumulated ones (one thread, so no other update races).
+template <int MODEL, int REP>
+__device__ __forceinline__ void pair_side(const StageArgs& a, const Phys& P, int si, int sj, double& fx,
+                                          double& fy, double& tq)
+{
+    if (MODEL == SPP_MODEL_ABP) {
+        abp_pair_side<REP>(a, P, si, sj, fx, fy, tq);
+        return;
+    }
+    const P4 me = a.gather[si];
+    double mc, ms, s1 = 0, s2 = 0;
+    spp::heading_unit(me.z, mc, ms);
+    spp::pair_accumulate<MODEL, REP>(P, me, mc, ms, a.gather[sj], fx, fy, s1, s2);
+    tq += spp::drift_torque<MODEL>(P, s1, s2);
+}
+
+template <int MODEL, int REP>
+__global__ void k_pair_interact(StageArgs a, int ci, int cj, P4* force)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int si = a.inv[ci], sj = a.inv[cj];
+    Phys P = a.P;
+    if (a.sys_of) apply_sys(P, a.sys_tab[a.sys_of[si]]);
+    double fi[3] = {0, 0, 0}, fj[3] = {0, 0, 0};
+    pair_side<MODEL, REP>(a, P, si, sj, fi[0], fi[1], fi[2]);
+    pair_side<MODEL, REP>(a, P, sj, si, fj[0], fj[1], fj[2]);
+    force[si].x += fi[0];
+    force[si].y += fi[1];
+    force[si].z += fi[2];
+    force[sj].x += fj[0];
+    force[sj].y += fj[1];
+    force[sj].z += fj[2];
+}
+
+using PairFn = void (*)(StageArgs, int, int, P4*);
+
+template <int MODEL>
+PairFn pair_fn(int rep)
+{
+    switch (rep) {
+    case SPP_REP_NONE: return k_pair_interact<MODEL, SPP_REP_NONE>;
+    case SPP_REP_R12_TRUNCATED: return k_pair_interact<MODEL, SPP_REP_R12_TRUNCATED>;
+    case SPP_REP_HARMONIC: return k_pair_interact<MODEL, SPP_REP_HARMONIC>;
+    case SPP_REP_R12: return k_pair_interact<MODEL, SPP_REP_R12>;
+    case SPP_REP_YUKAWA_TRUNCATED: return k_pair_interact<MODEL, SPP_REP_YUKAWA_TRUNCATED>;
+    default: return k_pair_interact<MODEL, SPP_REP_YUKAWA>;
+    }
+}
+
+PairFn pick_pair(int model, int rep)
+{
+    if (model == SPP_MODEL_ABP) return pair_fn<SPP_MODEL_ABP>(rep);
+    if (model == SPP_MODEL_CONTINUOUS) return pair_fn<SPP_MODEL_CONTINUOUS>(SPP_REP_NONE);
+    return pair_fn<SPP_MODEL_REPULSIVE>(rep);
+}
+
 InteractFn pick_interact(int model, int rep, int variant)
 {
     if (model == SPP_MODEL_ABP) {
@@ -3499,6 +3576,47 @@ int spp_interact(spp_engine* e)
     return launch(e, "interact", fn, e->nblk, a, e->force);
 }
 
+StageArgs pair_args(spp_engine* e)
+{
+    StageArgs a{};
+    a.N = e->N;
+    a.P = e->P;
+    a.sys_tab = e->dsys;
+    a.sys_of = e->sys_of;
+    a.abp = e->abp;
+    a.inv = e->inv;
+    a.gather = e->set[e->cur].pos;
+    a.id = e->set[e->cur].id;
+    return a;
+}
+
+int spp_reset_forces(spp_engine* e)
+{
+    TRY(check_ready(e));
+    return launch(e, "reset_forces", k_reset_forces, e->nblk, pair_args(e), e->p.model == SPP_MODEL_ABP ? 1 : 0,
+                  e->force);
+}
+
+int spp_pair_interact(spp_engine* e, int32_t i, int32_t j)
+{
+    TRY(check_ready(e));
+    if (i < 0 || j < 0 || i >= e->N || j >= e->N || i == j)
+        return fail(e, SPP_ERR_ARG, "particles %d and %d: two distinct indices in [0, %d) needed", i, j, e->N);
+    if (e->dsys) {
+        int si = 0, sj = 0;
+        for (int s = 0; s < e->nsys; ++s) {
+            if (i >= e->hsys[s].pid_off && i < e->hsys[s].pid_off + e->hsys[s].n) si = s;
+            if (j >= e->hsys[s].pid_off && j < e->hsys[s].pid_off + e->hsys[s].n) sj = s;
+        }
+        if (si != sj) return fail(e, SPP_ERR_ARG, "particles %d and %d belong to different systems", i, j);
+    }
+    PairFn fn = pick_pair(e->p.model, e->p.repulsion);
+    if (e->dry) return SPP_OK;
+    hipLaunchKernelGGL(fn, dim3(1), dim3(64), 0, e->stream, pair_args(e), i, j, e->force);
+    HIPC(e, hipGetLastError());
+    return SPP_OK;
+}
+
 int spp_get_forces(spp_engine* e, double* fx, double* fy, double* tau, int32_t n)
 {
     TRY(check_ready(e));
@@ -3898,6 +4016,37 @@ int spp_restore(spp_engine* e, int32_t slot)
     return rebuild_sync(e, false);
 }
 
+int spp_get_clock(spp_engine* e, spp_clock* clock)
+{
+    if (!e || !clock) return e ? fail(e, SPP_ERR_ARG, "null clock") : SPP_ERR_ARG;
+    clock->step = e->step;
+    clock->t = e->t;
+    for (int k = 0; k < 3; ++k) clock->rebuild_estimate[k] = e->T_hist[k];
+    return SPP_OK;
+}
+
+int spp_set_clock(spp_engine* e, const spp_clock* clock)
+{
+    TRY(check_ready(e));
+    if (!clock) return fail(e, SPP_ERR_ARG, "null clock");
+    if (clock->step < 0) return fail(e, SPP_ERR_ARG, "step %lld < 0", (long long)clock->step);
+    for (int k = 0; k < 3; ++k)
+        if (!(clock->rebuild_estimate[k] >= 0.0) || !std::isfinite(clock->rebuild_estimate[k]))
+            return fail(e, SPP_ERR_ARG, "rebuild estimate %d is %g", k, clock->rebuild_estimate[k]);
+    if (e->unfused_stage != 0) return fail(e, SPP_ERR_STATE, "a split step is in progress");
+    e->step = clock->step;
+    e->t = clock->t;
+    // the schedule's estimate is the smallest recorded one (run_intervals), 0 with none
+    double T = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        e->T_hist[k] = clock->rebuild_estimate[k];
+        if (e->T_hist[k] > 0 && (T == 0.0 || e->T_hist[k] < T)) T = e->T_hist[k];
+    }
+    e->T_cross = T;
+    e->force_m = 0;
+    return SPP_OK;
+}
+
 int spp_synchronize(spp_engine* e)
 {
     if (!e) return SPP_ERR_ARG;
@@ -4029,17 +4178,32 @@ int spp_time_stage(spp_engine* e, int32_t stage, int32_t reps, double* avg_us)
 int spp_random_state(uint64_t seed, int32_t n, double Lx, double Ly, double* x, double* y,
                      double* theta)
 {
-    if (n < 0 || !x || !y || !theta) return SPP_ERR_ARG;
+    return spp_random_formation(seed, n, Lx, Ly, 0.0, x, y, theta);
+}
+
+int spp_random_formation(uint64_t seed, int32_t n, double Lx, double Ly, double sigma, double* x, double* y,
+                         double* theta)
+{
+    if (n < 0 || !x || !y || !theta || !(sigma >= 0) || !(sigma < Lx) || !(sigma < Ly)) return SPP_ERR_ARG;
+    const double ax = Lx - sigma, ay = Ly - sigma;  // C2DVector::Rand(Lx - sigma, Ly - sigma)
     for (int32_t i = 0; i < n; ++i) {
         uint32_t c[4] = {(uint32_t)i, 0u, 0u, 0x53505069u};
         spp::philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
         double u0 = (double)c[0] * (1.0 / 4294967296.0);
         double u1 = (double)c[1] * (1.0 / 4294967296.0);
         double u2 = (double)c[2] * (1.0 / 4294967296.0);
-        x[i] = -Lx + 2 * Lx * u0;
-        y[i] = -Ly + 2 * Ly * u1;
+        x[i] = -ax + 2 * ax * u0;
+        y[i] = -ay + 2 * ay * u1;
         theta[i] = spp::kTwoPi * u2;
     }
+    return SPP_OK;
+}
+
+int spp_polar_formation(uint64_t seed, int32_t n, double Lx, double Ly, double* x, double* y, double* theta)
+{
+    const int rc = spp_random_formation(seed, n, Lx, Ly, 0.0, x, y, theta);
+    if (rc != SPP_OK) return rc;
+    for (int32_t i = 0; i < n; ++i) theta[i] = 0.0;  // every heading along +x (v = (1, 0))
     return SPP_OK;
 }
 
