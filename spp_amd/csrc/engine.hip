@@ -1936,7 +1936,7 @@ struct Snapshot {
     SetPtrs s{nullptr, nullptr, nullptr, nullptr};
     int64_t step = 0;
     double t = 0;
-    double T_cross = 0, T_hist[3] = {0, 0, 0};  // scheduler state: a replay rebuilds at the same steps
+    double T_hist[3] = {0, 0, 0};  // scheduler state: a replay rebuilds at the same steps
     bool valid = false;
 };
 
@@ -2047,7 +2047,8 @@ struct spp_engine {
     bool have_state = false, list_valid = false;
     bool inj_pending = false;
     double T_cross = 0;          // estimated steps from a rebuild to the displacement trigger
-    double T_hist[3] = {0, 0, 0};
+    double T_hist[3] = {0, 0, 0};  // the last three estimates, newest first (T_cross derives from them)
+    double T_margin = 0.85;      // fraction of T_cross an interval may run on one list
     int force_m = 0;             // length of the next interval after a rollback (0 = predict)
     int max_nbrs = 0;
     double mean_nbrs = 0;
@@ -2593,7 +2594,25 @@ bool rebuild_due(const spp_engine* e)
     if (e->p.rebuild == SPP_REBUILD_PERIOD) return e->steps_since_build >= e->p.rebuild_period;
     if (e->steps_since_build == 0) return false;
     if (e->T_cross <= 0) return true;  // no displacement history yet
-    return 0.85 * e->T_cross - (double)e->steps_since_build < 1.0;
+    return e->T_margin * e->T_cross - (double)e->steps_since_build < 1.0;
+}
+
+// The displacement policy's estimate of the steps from a rebuild to the trigger, from the last
+// three per-interval estimates (each a linear extrapolation of that interval's largest
+// displacement).  Normally the smallest of them with a 15 % margin, which covers fluctuations and
+// acceleration.  While the estimates grow by more than 5 % twice in a row and stay short (the
+// particles slow down, as after a random start whose overlaps push particles apart), the newest one
+// without margin: linear extrapolation of a decelerating displacement already overestimates it, and
+// a mistake costs at most a short interval (the crossing is still caught step by step).
+void derive_schedule(spp_engine* e)
+{
+    const double* h = e->T_hist;
+    double T = 0.0;
+    for (int k = 0; k < 3; ++k)
+        if (h[k] > 0 && (T == 0.0 || h[k] < T)) T = h[k];
+    const bool slowing = h[2] > 0 && h[0] > 1.05 * h[1] && h[1] > 1.05 * h[2] && h[0] < 8.0;
+    e->T_cross = slowing ? h[0] : T;
+    e->T_margin = slowing ? 1.0 : 0.85;
 }
 
 int predict_m(spp_engine* e, int remaining, bool rebuild)
@@ -2608,10 +2627,10 @@ int predict_m(spp_engine* e, int remaining, bool rebuild)
     } else if (e->T_cross <= 0) {
         m = 4;  // no history yet: a short interval to measure the displacement rate
     } else if (rebuild) {
-        m = std::max(1, (int)(0.85 * e->T_cross));
+        m = std::max(1, (int)(e->T_margin * e->T_cross));
     } else {
         // the rest of the current list's predicted safe length
-        m = std::max(1, (int)(0.85 * e->T_cross - (double)e->steps_since_build));
+        m = std::max(1, (int)(e->T_margin * e->T_cross - (double)e->steps_since_build));
     }
     return std::min(std::min(m, remaining), 4096);
 }
@@ -2654,10 +2673,7 @@ int run_intervals(spp_engine* e, int steps)
                 e->T_hist[2] = e->T_hist[1];
                 e->T_hist[1] = e->T_hist[0];
                 e->T_hist[0] = est;
-                double T = est;
-                for (double h : e->T_hist)
-                    if (h > 0) T = std::min(T, h);
-                e->T_cross = T;
+                derive_schedule(e);
             }
             continue;
         }
@@ -2698,8 +2714,9 @@ int run_intervals(spp_engine* e, int steps)
         // The rebuild at the interval's start (identical, the state there is already sorted) is
         // redone; the valid prefix [0, bad) is replayed as its own interval and verified again.
         e->force_m = bad > 0 ? bad : 0;
-        e->T_cross = std::max(1.0, 0.8 * (iv.since0 + std::max(bad, 1)));
-        for (double& h : e->T_hist) h = h > 0 ? std::min(h, e->T_cross) : h;
+        // the crossing came earlier than predicted: every estimate becomes 0.8 of the steps it took
+        for (double& h : e->T_hist) h = std::max(1.0, 0.8 * (iv.since0 + std::max(bad, 1)));
+        derive_schedule(e);
     }
     return SPP_OK;
 }
@@ -3994,7 +4011,6 @@ int spp_snapshot(spp_engine* e, int32_t slot)
     TRY(copy_set(e, e->set[e->cur], s.s));
     s.step = e->step;
     s.t = e->t;
-    s.T_cross = e->T_cross;
     for (int k = 0; k < 3; ++k) s.T_hist[k] = e->T_hist[k];
     s.valid = true;
     return SPP_OK;
@@ -4008,8 +4024,8 @@ int spp_restore(spp_engine* e, int32_t slot)
     TRY(copy_set(e, it->second.s, e->set[e->cur]));
     e->step = it->second.step;
     e->t = it->second.t;
-    e->T_cross = it->second.T_cross;
     for (int k = 0; k < 3; ++k) e->T_hist[k] = it->second.T_hist[k];
+    derive_schedule(e);
     e->force_m = 0;
     e->unfused_stage = 0;
     // Box::Load ends in a cell update (parallel/box.h:241)
@@ -4036,13 +4052,8 @@ int spp_set_clock(spp_engine* e, const spp_clock* clock)
     if (e->unfused_stage != 0) return fail(e, SPP_ERR_STATE, "a split step is in progress");
     e->step = clock->step;
     e->t = clock->t;
-    // the schedule's estimate is the smallest recorded one (run_intervals), 0 with none
-    double T = 0.0;
-    for (int k = 0; k < 3; ++k) {
-        e->T_hist[k] = clock->rebuild_estimate[k];
-        if (e->T_hist[k] > 0 && (T == 0.0 || e->T_hist[k] < T)) T = e->T_hist[k];
-    }
-    e->T_cross = T;
+    for (int k = 0; k < 3; ++k) e->T_hist[k] = clock->rebuild_estimate[k];
+    derive_schedule(e);
     e->force_m = 0;
     return SPP_OK;
 }
