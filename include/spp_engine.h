@@ -98,8 +98,8 @@ enum {
 /* ---- interaction-kernel variants (same forces, different gather strategy or launch shape) ----- */
 enum {
     SPP_KERNEL_SLOT = 0,       /* one lane per particle, neighbour records gathered from L2         */
-    SPP_KERNEL_SLOT_SPLIT = 1, /* as SLOT, heading half of a record fetched only inside the cutoff  */
-    SPP_KERNEL_STRIP_LDS = 2,  /* one workgroup per cell strip, 3-row neighbourhood staged in LDS  */
+    /* 1 and 2 are retired (SPP_ERR_UNSUPPORTED): a heading fetched only inside the cutoff, and cell
+       strips staged in LDS; both were slower than SLOT at every measured size                       */
     SPP_KERNEL_PERSIST = 3,    /* one workgroup per system (<= 4096 particles, Vicsek models, noise
                                   stream) runs a whole spp_multi_step in one launch: rebuilds when a
                                   system's own displacement (or period) requires; single or batched */

@@ -26,7 +26,7 @@ BIN_EXACT, BIN_REF_QUIRK = 0, 1
 REBUILD_DISPLACEMENT, REBUILD_PERIOD = 0, 1
 NOISE_STREAM, NOISE_INJECTED = 0, 1
 REBUILD_SORT_AUTO, REBUILD_SORT_GENERAL = 0, 1
-KERNEL_SLOT, KERNEL_SLOT_SPLIT, KERNEL_STRIP_LDS, KERNEL_PERSIST, KERNEL_SLOT_DEEP = 0, 1, 2, 3, 4
+KERNEL_SLOT, KERNEL_PERSIST, KERNEL_SLOT_DEEP = 0, 3, 4  # 1 and 2 are retired
 PERSIST_MAX_N = 4096  # particles per system of the persistent kernel (KERNEL_PERSIST)
 
 OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_CAPACITY, ERR_UNSUPPORTED, ERR_NO_DEVICE = 0, -1, -2, -3, -4, -5, -6

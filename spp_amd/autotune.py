@@ -3,11 +3,10 @@
 The knobs do not change the physics; they trade gather strategy against list length and rebuild
 frequency:
 
-* ``kernel_variant`` -- the interaction tile strategy: one lane per slot with L2 gathers (0), the
-  split-record gather (1), 256-slot chunks staged in LDS (2; single box only), one workgroup per
-  system running the whole call in one launch (3; systems of at most 4096 particles), or the slot
-  kernel with eight neighbour records in flight per lane at every size (4; the slot kernel takes
-  eight only below 131,072 particles);
+* ``kernel_variant`` -- the interaction tile strategy: one lane per slot with L2 gathers (0), one
+  workgroup per system running the whole call in one launch (3; systems of at most 4096
+  particles), or the slot kernel with eight neighbour records in flight per lane at every size (4;
+  the slot kernel takes eight only below 131,072 particles);
 * ``r_list`` -- the Verlet radius: a wider skin lengthens every list but rebuilds less often;
 * the cell size -- ``ncell_x/y`` below floor(2L / r_list) makes cells wider than the list radius
   (more candidates per list build, fewer empty cells).
@@ -22,8 +21,8 @@ import math
 import time
 from dataclasses import dataclass, field
 
-from . import (KERNEL_PERSIST, KERNEL_SLOT, KERNEL_SLOT_DEEP, KERNEL_SLOT_SPLIT, KERNEL_STRIP_LDS, MODEL_ABP,
-               MODEL_REPULSIVE, PERSIST_MAX_N, REP_NONE, Engine, SppError, System)
+from . import (KERNEL_PERSIST, KERNEL_SLOT, KERNEL_SLOT_DEEP, MODEL_ABP, MODEL_REPULSIVE, PERSIST_MAX_N, REP_NONE,
+               Engine, SppError, System)
 
 
 def interaction_cutoff(p):
@@ -73,11 +72,9 @@ def autotune(params, x, y, theta, systems=None, steps=200, warm=50, variants=Non
     """Time every candidate on the state (x, y, theta) and return the fastest configuration."""
     cut = interaction_cutoff(params)
     if variants is None:
-        variants = [KERNEL_SLOT, KERNEL_SLOT_SPLIT]
+        variants = [KERNEL_SLOT]
         if params.model != MODEL_ABP:
             variants.append(KERNEL_SLOT_DEEP)
-        if systems is None and params.model != MODEL_ABP:
-            variants.append(KERNEL_STRIP_LDS)
         sizes = [s.n for s in systems] if systems is not None else [params.n]
         if params.model != MODEL_ABP and max(sizes) <= PERSIST_MAX_N:
             variants.append(KERNEL_PERSIST)

@@ -140,7 +140,7 @@ __device__ __forceinline__ void abp_forces(const StageArgs& a, const Phys& P0, i
     tq = a.abp.torque0;
     if (a.abp.nb <= 1) {
         double t_unused;
-        gather_forces_flat<SPP_MODEL_REPULSIVE, REP, false>(a, P, i, me, mc, ms, fx, fy, t_unused);
+        gather_forces_flat<SPP_MODEL_REPULSIVE, REP>(a, P, i, me, mc, ms, fx, fy, t_unused);
     } else {
         fx = mc * P.v0;
         fy = ms * P.v0;

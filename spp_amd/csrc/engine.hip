@@ -634,236 +634,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_tiles(SortArgs a)
 }
 
 // ------------------------------------------------------------------------------------------------
-// Chunks: the unit of work of the LDS kernel variant.  Chunk b is the slot range
-// [256 b, 256 b + 256) -- exactly one workgroup of particles, so the grid has no tail.  Slots are
-// sorted by cell key (row-major, W cells per row), so a chunk covers the cell keys [k0, k1] and,
-// away from the periodic edges, every neighbour of its particles lies in a cell with key in
-// [k0 - 1 + dy W, k1 + 1 + dy W] for dy = -1, 0, 1.  Those three key ranges (clipped to the grid,
-// merged when they overlap) are contiguous slot ranges ("pieces"); a workgroup copies them into LDS
-// with coalesced loads once per kernel and serves pair gathers from LDS through 16-bit offsets.
-// Neighbours in cells outside the pieces (periodic wrap in x or y) carry the marker kGlobalNbr and
-// are read from global memory through the 32-bit slot list, which is always complete.  A chunk
-// whose pieces exceed the LDS budget gathers everything from global memory.
-// ------------------------------------------------------------------------------------------------
-constexpr int kChunk = 256;
-constexpr int kLdsCap = 1216;          // records (38 KB): four workgroups per CU
-constexpr int kPieceMax = 2 * kChunk;  // records of one piece that the unrolled staging covers
-constexpr uint16_t kGlobalNbr = 0xFFFF;
-
-// The chunk's LDS copy.  Declared at file scope so that every access is compiled as an LDS
-// (address space 3) access -- through a generic pointer the compiler would emit flat loads.
-__shared__ P4 s_lds[kLdsCap];
-
-struct Chunk {
-    int p0, np;                    // the chunk's particle slots
-    int fb;                        // 1: nothing staged, every gather from global memory
-    int npieces;
-    int kst[3], kend[3];           // cell-key range [kst, kend) of each piece
-    int pst[3], plen[3], poff[3];  // slot range and LDS offset of each piece
-};
-
-// One thread per chunk, after the sort (perm[slot] = the particle's index in the binning order, whose
-// cell key is key[]): pieces and LDS offsets from the fresh cell starts.
-__global__ __launch_bounds__(kBlock) void k_chunks(int N, int nchunks, int W, int nkeys,
-                                                   const int* __restrict__ start, const int* __restrict__ key,
-                                                   const int* __restrict__ perm, Chunk* __restrict__ chunks)
-{
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nchunks) return;
-    const int p0 = b * kChunk, np = min(kChunk, N - p0);
-    const int k0 = key[perm[p0]], k1 = key[perm[p0 + np - 1]];
-    // the ranges for dy = -1, 0, 1 ascend, so overlapping neighbours merge into the previous piece
-    int lo[3], hi[3];
-    for (int d = 0; d < 3; ++d) {
-        lo[d] = max(0, k0 - 1 + (d - 1) * W);
-        hi[d] = min(nkeys, k1 + 2 + (d - 1) * W);
-    }
-    Chunk C;
-    C.p0 = p0;
-    C.np = np;
-    int n = 0;
-    int kst0 = 0, kend0 = 0, kst1 = 0, kend1 = 0, kst2 = 0, kend2 = 0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        if (lo[d] >= hi[d]) continue;
-        int last_end = n == 1 ? kend0 : kend1;
-        if (n > 0 && lo[d] <= last_end) {
-            if (n == 1)
-                kend0 = max(kend0, hi[d]);
-            else
-                kend1 = max(kend1, hi[d]);
-        } else {
-            if (n == 0) { kst0 = lo[d]; kend0 = hi[d]; }
-            else if (n == 1) { kst1 = lo[d]; kend1 = hi[d]; }
-            else { kst2 = lo[d]; kend2 = hi[d]; }
-            ++n;
-        }
-    }
-    C.npieces = n;
-    C.kst[0] = kst0; C.kend[0] = kend0;
-    C.kst[1] = kst1; C.kend[1] = kend1;
-    C.kst[2] = kst2; C.kend[2] = kend2;
-    int off = 0, big = 0;
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        int ps = 0, pl = 0;
-        if (q < n) {
-            ps = start[C.kst[q]];
-            pl = start[C.kend[q]] - ps;
-        }
-        C.pst[q] = ps;
-        C.plen[q] = pl;
-        C.poff[q] = off;
-        off += pl;
-        big |= pl > kPieceMax;
-    }
-    C.fb = (off > kLdsCap || big) ? 1 : 0;
-    chunks[b] = C;
-}
-
-// Copy a chunk's pieces of `src` into LDS: every load is issued before the first LDS store.  (Six
-// named records, not an array: an array indexed in a loop could be placed in scratch memory.)
-__device__ __forceinline__ void stage_chunk(const Chunk& C, const P4* __restrict__ src)
-{
-    const int t = threadIdx.x, u = t + kChunk;
-    const bool v0 = t < C.plen[0], w0 = u < C.plen[0];
-    const bool v1 = t < C.plen[1], w1 = u < C.plen[1];
-    const bool v2 = t < C.plen[2], w2 = u < C.plen[2];
-    // unconditional loads (slot 0 stands in for lanes past a piece): conditionally assigned
-    // records would be kept in scratch memory
-    const P4 a0 = src[v0 ? C.pst[0] + t : 0], b0 = src[w0 ? C.pst[0] + u : 0];
-    const P4 a1 = src[v1 ? C.pst[1] + t : 0], b1 = src[w1 ? C.pst[1] + u : 0];
-    const P4 a2 = src[v2 ? C.pst[2] + t : 0], b2 = src[w2 ? C.pst[2] + u : 0];
-    if (v0) s_lds[C.poff[0] + t] = a0;
-    if (w0) s_lds[C.poff[0] + u] = b0;
-    if (v1) s_lds[C.poff[1] + t] = a1;
-    if (w1) s_lds[C.poff[1] + u] = b1;
-    if (v2) s_lds[C.poff[2] + t] = a2;
-    if (w2) s_lds[C.poff[2] + u] = b2;
-}
-
-// LDS offset minus slot of the piece holding the whole cell-key range [k_lo, k_hi], or INT_MIN
-// (selects only: an indexed read of the descriptor would move it to scratch memory).
-__device__ __forceinline__ int piece_shift(const Chunk& C, int k_lo, int k_hi)
-{
-    int r = INT_MIN;
-    if (2 < C.npieces && k_lo >= C.kst[2] && k_hi < C.kend[2]) r = C.poff[2] - C.pst[2];
-    if (1 < C.npieces && k_lo >= C.kst[1] && k_hi < C.kend[1]) r = C.poff[1] - C.pst[1];
-    if (0 < C.npieces && k_lo >= C.kst[0] && k_hi < C.kend[0]) r = C.poff[0] - C.pst[0];
-    return r;
-}
-
-// Chunks are numbered along the slot order; consecutive chunks share neighbour pieces, so the
-// remap keeps runs of consecutive chunks on one XCD (one L2).
-__device__ __forceinline__ int chunk_block()
-{
-    int nblk = gridDim.x, b = blockIdx.x;
-    if ((nblk & 7) == 0) return (b & 7) * (nblk >> 3) + (b >> 3);
-    return b;
-}
-
-struct NlistArgs {
-    int N, cap, nchunks;
-    Grid G;
-    double T_list;  // sqrt(d2) < r_list  <=>  d2 < T_list
-    const P4* pos;
-    const int* start;
-    const Chunk* chunks;
-    uint16_t* nbr16;
-    int* nbr32;
-    int* ncount;
-    float2* build;
-    int* cellxy;
-    int* blklen;
-    int* blknnz;
-};
-
-// Cell::Neighbor_List (shared/cell.h:74-103) over the 3x3 periodic stencil, full list, predicate
-// sqrt(d2) < rv (cell.h:84-85) evaluated as d2 < T(rv); the same candidate order as k_nlist_flat
-// (per stencil row: the contiguous columns, then the wrapped column), candidates read from the
-// chunk's LDS copy where staged.
-__global__ __launch_bounds__(kChunk) void k_nlist_chunk(NlistArgs a)
-{
-    __shared__ int red_len[kChunk / 64], red_nnz[kChunk / 64];
-    int b = chunk_block();
-    if (b >= a.nchunks) return;  // grid padding (uniform)
-    const Chunk& C = a.chunks[b];
-    if (!C.fb) stage_chunk(C, a.pos);
-    __syncthreads();
-    const Grid& G = a.G;
-    int c = 0;
-    if ((int)threadIdx.x < C.np) {
-        int i = C.p0 + threadIdx.x;
-        P4 me = a.pos[i];
-        int cx = cell_coord(me.x, G.Lx, G.Lx2, G.ncx, G.quirk);
-        int cy = cell_coord(me.y, G.Ly, G.Ly2, G.ncy, G.quirk);
-        int x0 = cx - 1, x1 = cx + 1, xw = -1;
-        if (x0 < 0) {
-            xw = G.ncx - 1;
-            x0 = 0;
-        } else if (x1 >= G.ncx) {
-            xw = 0;
-            x1 = G.ncx - 1;
-        }
-        for (int oy = -1; oy <= 1; ++oy) {
-            int y = cy + oy;
-            y = y < 0 ? y + G.ncy : (y >= G.ncy ? y - G.ncy : y);
-            int row = y * G.ncx;
-            for (int seg = 0; seg < 2; ++seg) {
-                int ka, kb;
-                if (seg == 0) {
-                    ka = row + x0;
-                    kb = row + x1;
-                } else {
-                    if (xw < 0) break;
-                    ka = kb = row + xw;
-                }
-                int js = a.start[ka], je = a.start[kb + 1];
-                int sh = C.fb ? INT_MIN : piece_shift(C, ka, kb);
-                bool staged = sh != INT_MIN;
-                int lbase = staged ? sh + js : 0;
-                for (int j = js; j < je; ++j) {
-                    if (j == i) continue;
-                    P4 pj = staged ? s_lds[lbase + (j - js)] : a.pos[j];
-                    double dx = spp::min_image(me.x - pj.x, G.Lx2, G.invLx2);
-                    double dy = spp::min_image(me.y - pj.y, G.Ly2, G.invLy2);
-                    double d2 = dx * dx + dy * dy;
-                    if (d2 < a.T_list) {
-                        if (c < a.cap) {
-                            a.nbr32[(size_t)c * a.N + i] = j;
-                            a.nbr16[(size_t)c * a.N + i] = staged ? (uint16_t)(lbase + (j - js)) : kGlobalNbr;
-                        }
-                        ++c;
-                    }
-                }
-            }
-        }
-        a.ncount[i] = c < a.cap ? c : a.cap;
-        a.build[i] = make_float2((float)me.x, (float)me.y);
-        a.cellxy[i] = cx | (cy << 16);
-    }
-    int m = c;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
-    int t = wave_sum(c);
-    if ((threadIdx.x & 63) == 0) {
-        red_len[threadIdx.x >> 6] = m;
-        red_nnz[threadIdx.x >> 6] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int mm = 0, tt = 0;
-        for (int k = 0; k < kChunk / 64; ++k) {
-            mm = max(mm, red_len[k]);
-            tt += red_nnz[k];
-        }
-        a.blklen[b] = mm;
-        a.blknnz[b] = tt;
-    }
-}
-
-// ------------------------------------------------------------------------------------------------
-// Stage kernels: interact + Move fused, one particle per lane, one chunk per workgroup.
+// Stage kernels: interact + Move fused, one particle per lane.
 // ------------------------------------------------------------------------------------------------
 
 // Constants of the single-bead active Brownian model (zero for the Vicsek models).
@@ -885,8 +656,6 @@ struct StageArgs {
     const int* inv;        // caller index -> slot (ring neighbours of the active Brownian model)
     const SysInfo* sys_tab;  // batched engine (null: single system, constants in P)
     const int* sys_of;
-    int nchunks;
-    const Chunk* chunks;
     const P4* gather;      // stage input: neighbours' (and own) position and heading
     const double* th_in;   // own heading angle of the stage input
     P4* out;               // stage output
@@ -896,7 +665,6 @@ struct StageArgs {
     double2* U_out;        // written by the final stage
     double* dth;           // angular noise of this step
     P4* kacc;              // RK4 accumulator
-    const uint16_t* nbr16;
     const int* nbr32;
     const int* ncount;
     const int* id;
@@ -920,98 +688,6 @@ __device__ __forceinline__ int64_t noise_step(const StageArgs& a)
     return a.clock ? a.step + *a.clock : a.step;
 }
 
-// The first kPfList rows of a staged chunk's LDS-offset list, requested before the staging loads so
-// that the pair loop does not wait on global memory for them.
-constexpr int kPfList = 12;
-
-struct ListHead {
-    int cnt;
-    int l[kPfList];
-};
-
-__device__ __forceinline__ void load_list_head(const StageArgs& a, int i, bool live, ListHead& h)
-{
-    h.cnt = live ? a.ncount[i] : 0;
-#pragma unroll
-    for (int u = 0; u < kPfList; ++u) h.l[u] = live ? a.nbr16[(size_t)u * a.N + i] : 0;
-}
-
-// Neighbour record of list entry k (LDS offset l, or global through the slot list when marked).
-__device__ __forceinline__ P4 chunk_record(const StageArgs& a, int i, int k, int l)
-{
-    if (l != kGlobalNbr) return s_lds[l];
-    return a.gather[a.nbr32[(size_t)k * a.N + i]];
-}
-
-template <int MODEL, int REP>
-__device__ __forceinline__ void gather_forces_lds(const StageArgs& a, int i, const P4& me, double mc, double ms,
-                                                  const ListHead& h, double& fx, double& fy, double& tq)
-{
-    const int cnt = h.cnt;
-    double s1 = 0, s2 = 0;
-    fx = mc * a.P.v0;  // Reset: f = v (speed v0), particle.h:559-564
-    fy = ms * a.P.v0;
-#pragma unroll
-    for (int k0 = 0; k0 < kPfList; k0 += 4) {
-        if (k0 >= cnt) break;
-        P4 q[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) q[u] = chunk_record(a, i, k0 + u, k0 + u < cnt ? h.l[k0 + u] : 0);
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (k0 + u < cnt) spp::pair_accumulate<MODEL, REP>(a.P, me, mc, ms, q[u], fx, fy, s1, s2);
-    }
-    for (int k = kPfList; k < cnt; k += 4) {
-        int l[4];
-        P4 q[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) l[u] = k + u < cnt ? a.nbr16[(size_t)(k + u) * a.N + i] : 0;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) q[u] = chunk_record(a, i, k + u, l[u]);
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (k + u < cnt) spp::pair_accumulate<MODEL, REP>(a.P, me, mc, ms, q[u], fx, fy, s1, s2);
-    }
-    tq = spp::drift_torque<MODEL>(a.P, s1, s2);
-}
-
-// Pair sums of slot i over its list: LDS offsets (global slot indices where marked) for a staged
-// chunk, global slot indices otherwise (FB).
-template <int MODEL, int REP, bool FB>
-__device__ __forceinline__ void gather_forces(const StageArgs& a, int i, const P4& me, double mc, double ms,
-                                              double& fx, double& fy, double& tq)
-{
-    const int cnt = a.ncount[i];
-    double s1 = 0, s2 = 0;
-    fx = mc * a.P.v0;  // Reset: f = v (speed v0), particle.h:559-564
-    fy = ms * a.P.v0;
-    for (int k = 0; k < cnt; k += 4) {
-        P4 q[4];
-        if (!FB) {
-            int l[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) l[u] = k + u < cnt ? a.nbr16[(size_t)(k + u) * a.N + i] : 0;
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (l[u] != kGlobalNbr)
-                    q[u] = s_lds[l[u]];
-                else
-                    q[u] = a.gather[a.nbr32[(size_t)(k + u) * a.N + i]];
-            }
-        } else {
-            int j[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) j[u] = k + u < cnt ? a.nbr32[(size_t)(k + u) * a.N + i] : i;
-#pragma unroll
-            for (int u = 0; u < 4; ++u) q[u] = a.gather[j[u]];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (k + u < cnt) spp::pair_accumulate<MODEL, REP>(a.P, me, mc, ms, q[u], fx, fy, s1, s2);
-    }
-    tq = spp::drift_torque<MODEL>(a.P, s1, s2);
-}
-
 // Displacement trigger: marks the step (per workgroup, atomics stay inside one workgroup's slot)
 // and, on the last step of an interval, records the workgroup's largest squared displacement.
 __device__ __forceinline__ void check_disp(const StageArgs& a, const Phys& P, int s, bool live, int i,
@@ -1030,126 +706,22 @@ __device__ __forceinline__ void check_disp(const StageArgs& a, const Phys& P, in
 
 __device__ __forceinline__ void finish_disp(const StageArgs& a, int s, double dmax)
 {
-    __shared__ double red[kChunk / 64];
+    __shared__ double red[kBlock / 64];
     if (!a.record) return;
     double m = wave_max(dmax);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
         double r = red[0];
-        for (int k = 1; k < kChunk / 64; ++k) r = fmax(r, red[k]);
+        for (int k = 1; k < kBlock / 64; ++k) r = fmax(r, red[k]);
         a.blockmax[s] = r;
     }
-}
-
-// INTEG/STAGE: Euler (1,0); RK2 (2,1..2); RK4 (4,1..4).  Reference updates: Move
-// (particle.h:571-587), Move_Runge_Kutta2_1/2 (:589-631), Move_Runge_Kutta4_1..4 (:634-725).
-template <int MODEL, int REP, int INTEG, int STAGE, bool FB>
-__device__ __forceinline__ void stage_body(const StageArgs& a, int s, const Chunk& C, const ListHead& h)
-{
-    const Phys& P = a.P;
-    constexpr bool final_stage = (INTEG == SPP_EULER) || (INTEG == SPP_RK2 && STAGE == 2) || (STAGE == 4);
-    double dmax = 0;
-    {
-        int t = threadIdx.x;
-        bool live = t < C.np;
-        int i = C.p0 + t;
-        double x = 0, y = 0;
-        if (live) {
-            P4 me = a.gather[i];
-            double mc, ms;
-            spp::heading_unit(me.z, mc, ms);
-            double dn = 0, th_in = 0, thold = 0;
-            double2 u = make_double2(0, 0);
-            P4 kk{0, 0, 0, 0};
-            if (STAGE <= 1) {
-                int pid = a.id[i];
-                if (a.id_out) a.id_out[i] = pid;
-                th_in = me.z;
-                if (INTEG == SPP_EULER) u = a.U_in[i];
-                dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)pid, noise_step(a));
-            } else {
-                thold = me.w;  // carried by the middle stages' records
-                dn = a.dth[i];
-                u = a.U_in[i];
-                if (INTEG == SPP_RK4) kk = a.kacc[i];
-            }
-            double fx, fy, tq;
-            if (FB)
-                gather_forces<MODEL, REP, true>(a, i, me, mc, ms, fx, fy, tq);
-            else
-                gather_forces_lds<MODEL, REP>(a, i, me, mc, ms, h, fx, fy, tq);
-            double th;
-            if (STAGE <= 1) {
-                if (INTEG == SPP_EULER) {
-                    th = th_in + P.dt * (tq);
-                    th += dn;
-                    u.x += fx * P.dt;
-                    u.y += fy * P.dt;
-                    a.U_out[i] = u;
-                    x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
-                    y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
-                } else {
-                    a.dth[i] = dn;
-                    th = th_in;
-                    th += P.half_dt * tq;
-                    th += dn / 2;
-                    th = spp::wrap_angle(th);
-                    x = spp::periodic_wrap(me.x + fx * P.half_dt, P.Lx, P.Lx2);
-                    y = spp::periodic_wrap(me.y + fy * P.half_dt, P.Ly, P.Ly2);
-                    if (INTEG == SPP_RK4) a.kacc[i] = P4{fx, fy, tq, 0.0};
-                }
-            } else {
-                double h = (INTEG == SPP_RK4 && STAGE == 2) ? P.half_dt : P.dt;
-                if (INTEG == SPP_RK4 && STAGE == 4) {
-                    th = thold + (kk.z + tq) * P.dt6;
-                    th += dn;
-                    u.x = u.x + (kk.x + fx) * P.dt6;
-                    u.y = u.y + (kk.y + fy) * P.dt6;
-                } else {
-                    th = thold + h * tq;
-                    th += (INTEG == SPP_RK4 && STAGE == 2) ? dn / 2 : dn;
-                    u.x = u.x + fx * h;
-                    u.y = u.y + fy * h;
-                    if (INTEG == SPP_RK4) a.kacc[i] = P4{kk.x + fx * 2, kk.y + fy * 2, kk.z + tq * 2, 0.0};
-                }
-                th = spp::wrap_angle(th);
-                if (INTEG == SPP_RK2 || STAGE == 4) a.U_out[i] = u;
-                x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
-                y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
-            }
-            // middle stages carry the step's starting heading for the later stages
-            a.out[i] = P4{x, y, th, final_stage ? 0.0 : (STAGE <= 1 ? th_in : thold)};
-            a.th_out[i] = th;
-        }
-        check_disp(a, P, s, live, i, x, y, final_stage ? a.kstep + 1 : a.kstep, dmax);
-    }
-    finish_disp(a, s, dmax);
-}
-
-template <int MODEL, int REP, int INTEG, int STAGE>
-__global__ __launch_bounds__(kChunk) void k_stage(StageArgs a)
-{
-    int s = chunk_block();
-    if (s >= a.nchunks) return;  // grid padding (uniform)
-    const Chunk& C = a.chunks[s];
-    ListHead h;
-    if (!C.fb) {
-        load_list_head(a, C.p0 + (int)threadIdx.x, (int)threadIdx.x < C.np, h);
-        stage_chunk(C, a.gather);
-    }
-    __syncthreads();
-    if (C.fb)
-        stage_body<MODEL, REP, INTEG, STAGE, true>(a, s, C, h);
-    else
-        stage_body<MODEL, REP, INTEG, STAGE, false>(a, s, C, h);
 }
 
 // ------------------------------------------------------------------------------------------------
 // Slot-parallel variant: one lane per particle over the whole slot range, neighbour records
 // gathered from L2 through a 32-bit slot-index list.  Loads of lanes past their list length are
-// masked off (they would cost texture-address cycles for nothing); with SPLIT the heading of a
-// record is requested only for neighbours inside the interaction cutoff.
+// masked off (they would cost texture-address cycles for nothing).
 // ------------------------------------------------------------------------------------------------
 
 struct NlistFlatArgs {
@@ -1281,7 +853,7 @@ __global__ __launch_bounds__(kBlock) void k_nlist_flat(NlistFlatArgs a)
 
 // GU: neighbour records in flight per lane (a block of the list loop).  Small systems, whose stage
 // time is one wavefront's chain of loads, take 8; large ones 4 (fewer VGPRs, same speed or better).
-template <int MODEL, int REP, bool SPLIT, int GU = 4>
+template <int MODEL, int REP, int GU = 4>
 __device__ __forceinline__ void gather_forces_flat(const StageArgs& a, const Phys& P, int i, const P4& me,
                                                    double mc, double ms, double& fx, double& fy, double& tq)
 {
@@ -1295,41 +867,13 @@ __device__ __forceinline__ void gather_forces_flat(const StageArgs& a, const Phy
 #pragma unroll
         for (int u = 0; u < GU; ++u)
             if (k + u < cnt) j[u] = nb[(size_t)(k + u) * a.N];
-        if (!SPLIT) {
-            P4 q[GU];
+        P4 q[GU];
 #pragma unroll
-            for (int u = 0; u < GU; ++u)
-                if (k + u < cnt) q[u] = a.gather[j[u]];
+        for (int u = 0; u < GU; ++u)
+            if (k + u < cnt) q[u] = a.gather[j[u]];
 #pragma unroll
-            for (int u = 0; u < GU; ++u)
-                if (k + u < cnt) spp::pair_accumulate<MODEL, REP>(P, me, mc, ms, q[u], fx, fy, s1, s2);
-        } else {
-            P4 q[GU];
-#pragma unroll
-            for (int u = 0; u < GU; ++u)
-                if (k + u < cnt) {
-                    double2 xy = *reinterpret_cast<const double2*>(&a.gather[j[u]].x);
-                    q[u].x = xy.x;
-                    q[u].y = xy.y;
-                }
-            bool in[GU];
-#pragma unroll
-            for (int u = 0; u < GU; ++u) {
-                in[u] = false;
-                if (k + u < cnt) {
-                    double dx = spp::min_image(me.x - q[u].x, P.Lx2, P.invLx2);
-                    double dy = spp::min_image(me.y - q[u].y, P.Ly2, P.invLy2);
-                    double d2 = dx * dx + dy * dy;
-                    in[u] = MODEL == SPP_MODEL_CONTINUOUS ? d2 < P.Ra2 : d2 < P.T_max;
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < GU; ++u)
-                if (in[u]) q[u].z = a.gather[j[u]].z;
-#pragma unroll
-            for (int u = 0; u < GU; ++u)
-                if (in[u]) spp::pair_accumulate<MODEL, REP>(P, me, mc, ms, q[u], fx, fy, s1, s2);
-        }
+        for (int u = 0; u < GU; ++u)
+            if (k + u < cnt) spp::pair_accumulate<MODEL, REP>(P, me, mc, ms, q[u], fx, fy, s1, s2);
     }
     tq = spp::drift_torque<MODEL>(P, s1, s2);
 }
@@ -1337,7 +881,7 @@ __device__ __forceinline__ void gather_forces_flat(const StageArgs& a, const Phy
 // One particle's fused stage (interaction at the stage input + the move), for slot i of a system
 // whose constants P already hold and whose particle indices start at pid_off.  Returns the wrapped
 // output position for the displacement check.
-template <int MODEL, int REP, int INTEG, int STAGE, bool SPLIT, int GU = 4>
+template <int MODEL, int REP, int INTEG, int STAGE, int GU = 4>
 __device__ __forceinline__ void stage_particle(const StageArgs& a, const Phys& P, int i, int pid_off, double& x,
                                                double& y)
 {
@@ -1362,7 +906,7 @@ __device__ __forceinline__ void stage_particle(const StageArgs& a, const Phys& P
         if (INTEG == SPP_RK4) kk = a.kacc[i];
     }
     double fx, fy, tq;
-    gather_forces_flat<MODEL, REP, SPLIT, GU>(a, P, i, me, mc, ms, fx, fy, tq);
+    gather_forces_flat<MODEL, REP, GU>(a, P, i, me, mc, ms, fx, fy, tq);
     double th;
     if (STAGE <= 1) {
         if (INTEG == SPP_EULER) {
@@ -1410,7 +954,7 @@ __device__ __forceinline__ void stage_particle(const StageArgs& a, const Phys& P
     if (kFinal) spp::store_out(a.stream, &a.th_out[i], th);
 }
 
-template <int MODEL, int REP, int INTEG, int STAGE, bool SPLIT, bool BATCH, int GU = 4>
+template <int MODEL, int REP, int INTEG, int STAGE, bool BATCH, int GU = 4>
 __global__ __launch_bounds__(kBlock) void k_stage_flat(StageArgs a)
 {
     int b = xcd_block();
@@ -1425,7 +969,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_flat(StageArgs a)
     }
     constexpr bool final_stage = (INTEG == SPP_EULER) || (INTEG == SPP_RK2 && STAGE == 2) || (STAGE == 4);
     double x = 0, y = 0, dmax = 0;
-    if (live) stage_particle<MODEL, REP, INTEG, STAGE, SPLIT, GU>(a, P, i, pid_off, x, y);
+    if (live) stage_particle<MODEL, REP, INTEG, STAGE, GU>(a, P, i, pid_off, x, y);
     check_disp(a, P, b, live, i, x, y, final_stage ? a.kstep + 1 : a.kstep, dmax);
     finish_disp(a, b, dmax);
 }
@@ -1440,7 +984,7 @@ __global__ __launch_bounds__(kBlock) void k_interact_flat(StageArgs a, P4* force
     P4 me = a.gather[i];
     double mc, ms, fx, fy, tq;
     spp::heading_unit(me.z, mc, ms);
-    gather_forces_flat<MODEL, REP, false>(a, P, i, me, mc, ms, fx, fy, tq);
+    gather_forces_flat<MODEL, REP>(a, P, i, me, mc, ms, fx, fy, tq);
     force[i] = P4{fx, fy, tq, 0.0};
 }
 
@@ -1709,11 +1253,9 @@ using InteractFn = void (*)(StageArgs, P4*);
 template <int MODEL, int REP, int I, int ST>
 StageFn variant_fn(int variant, bool small)
 {
-    if (variant == SPP_KERNEL_STRIP_LDS) return k_stage<MODEL, REP, I, ST>;
-    if (variant == SPP_KERNEL_SLOT_SPLIT) return k_stage_flat<MODEL, REP, I, ST, true, false>;
     if (variant == kVariantBatch)
-        return small ? k_stage_flat<MODEL, REP, I, ST, false, true, 8> : k_stage_flat<MODEL, REP, I, ST, false, true, 4>;
-    return small ? k_stage_flat<MODEL, REP, I, ST, false, false, 8> : k_stage_flat<MODEL, REP, I, ST, false, false, 4>;
+        return small ? k_stage_flat<MODEL, REP, I, ST, true, 8> : k_stage_flat<MODEL, REP, I, ST, true, 4>;
+    return small ? k_stage_flat<MODEL, REP, I, ST, false, 8> : k_stage_flat<MODEL, REP, I, ST, false, 4>;
 }
 
 template <int MODEL, int REP>
@@ -1994,15 +1536,11 @@ struct spp_engine {
     double *dth = nullptr, *dth_alt = nullptr, *inj_host_order = nullptr, *inj = nullptr;
     P4 *kacc = nullptr, *force = nullptr, *old = nullptr;
     P4* aux_alt[3] = {nullptr, nullptr, nullptr};
-    uint16_t* nbr16 = nullptr;   // list of LDS offsets (LDS variant)
     int* nbr32 = nullptr;        // list of slot indices (always complete)
     int *ncount = nullptr, *cellxy = nullptr;
-    int nchunks = 0;             // 256-slot chunks of the LDS variant
-    int chunk_grid = 0;          // nchunks rounded up to a multiple of 8 (XCD remap)
     int variant = SPP_KERNEL_SLOT;
     bool deep = false;           // SPP_KERNEL_SLOT_DEEP: eight records in flight at every size
     int nstat = 0;               // entries of the per-workgroup statistics arrays
-    Chunk* chunks = nullptr;
     float2* build = nullptr;
     int *key = nullptr, *perm = nullptr, *cell_count = nullptr, *cell_start = nullptr,
         *block_sums = nullptr;
@@ -2143,26 +1681,6 @@ int launch(spp_engine* e, const char* name, F kernel, int grid, Args... args)
     return SPP_OK;
 }
 
-// Chunk kernels: one workgroup of kChunk threads per chunk.
-template <typename F, typename... Args>
-int launch_chunks(spp_engine* e, const char* name, F kernel, Args... args)
-{
-    if (e->dry) return SPP_OK;
-    ProfRec rec{name, nullptr, nullptr};
-    if (e->profiling) {
-        rec.a = take_event(e);
-        rec.b = take_event(e);
-        HIPC(e, hipEventRecord(rec.a, e->stream));
-    }
-    hipLaunchKernelGGL(kernel, dim3(e->chunk_grid), dim3(kChunk), 0, e->stream, args...);
-    HIPC(e, hipGetLastError());
-    if (e->profiling) {
-        HIPC(e, hipEventRecord(rec.b, e->stream));
-        e->prof.push_back(rec);
-    }
-    return SPP_OK;
-}
-
 template <typename T>
 int dalloc(spp_engine* e, T** p, size_t n)
 {
@@ -2218,11 +1736,8 @@ int alloc_nbr(spp_engine* e)
 {
     e->alloc_gen++;
     drop_graphs(e);
-    dfree(e->nbr16);
     dfree(e->nbr32);
-    e->nbr16 = nullptr;
     e->nbr32 = nullptr;
-    TRY(dalloc(e, &e->nbr16, (size_t)e->cap * e->N));
     return dalloc(e, &e->nbr32, (size_t)e->cap * e->N);
 }
 
@@ -2249,15 +1764,9 @@ void free_set(SetPtrs& s)
 // ------------------------------------------------------------------------------------------------
 int launch_nlist(spp_engine* e, int slot)
 {
-    if (e->variant != SPP_KERNEL_STRIP_LDS) {
-        NlistFlatArgs fa{e->N, e->cap, e->G, e->dsys, e->sys_of, e->T_list, e->set[e->cur].pos, e->cell_start, e->nbr32,
-                         e->ncount, e->build, e->cellxy, e->slotb[slot].blklen, e->slotb[slot].blknnz};
-        return launch(e, "nlist", k_nlist_flat, e->nblk, fa);
-    }
-    NlistArgs na{e->N, e->cap, e->nchunks, e->G, e->T_list, e->set[e->cur].pos, e->cell_start,
-                 e->chunks, e->nbr16, e->nbr32, e->ncount, e->build, e->cellxy,
-                 e->slotb[slot].blklen, e->slotb[slot].blknnz};
-    return launch_chunks(e, "nlist", k_nlist_chunk, na);
+    NlistFlatArgs fa{e->N, e->cap, e->G, e->dsys, e->sys_of, e->T_list, e->set[e->cur].pos, e->cell_start, e->nbr32,
+                     e->ncount, e->build, e->cellxy, e->slotb[slot].blklen, e->slotb[slot].blknnz};
+    return launch(e, "nlist", k_nlist_flat, e->nblk, fa);
 }
 
 int launch_report(spp_engine* e, int slot, bool with_list)
@@ -2325,9 +1834,6 @@ int launch_rebuild(spp_engine* e, int dst, int slot, bool aux, bool tile)
     }
     TRY(launch(e, "rank_permute", k_rank_permute, nblocks(N), pa, (const int*)e->key, (const int*)e->cell_start,
                (const int*)e->perm, e->perm2));
-    if (e->variant == SPP_KERNEL_STRIP_LDS)
-        TRY(launch(e, "chunks", k_chunks, nblocks(e->nchunks), e->N, e->nchunks, e->G.ncx, e->nkeys,
-                   (const int*)e->cell_start, (const int*)e->key, (const int*)e->perm2, e->chunks));
     if (aux) {
         e->alloc_gen++;
         drop_graphs(e);
@@ -2399,11 +1905,8 @@ int launch_step(spp_engine* e, int S, int W, int kstep, int slot, bool record)
     a.sys_of = e->sys_of;
     a.abp = e->abp;
     a.inv = e->inv;
-    a.nchunks = e->nchunks;
-    a.chunks = e->chunks;
     a.dth = e->dth;
     a.kacc = e->kacc;
-    a.nbr16 = e->nbr16;
     a.nbr32 = e->nbr32;
     a.ncount = e->ncount;
     a.id = e->set[S].id;
@@ -2449,10 +1952,7 @@ int launch_step(spp_engine* e, int S, int W, int kstep, int slot, bool record)
         }
         a.stream = e->N <= kStreamMaxN;
         StageFn fn = pick_stage(e->p.model, e->p.repulsion, integ, stage, e->variant, e->deep || e->N < kSmallN);
-        if (e->variant == SPP_KERNEL_STRIP_LDS)
-            TRY(launch_chunks(e, stage_name(integ, stage), fn, a));
-        else
-            TRY(launch(e, stage_name(integ, stage), fn, e->nblk, a));
+        TRY(launch(e, stage_name(integ, stage), fn, e->nblk, a));
         a.inj = nullptr;
         a.id_out = nullptr;
     }
@@ -2837,9 +2337,6 @@ int run_persist(spp_engine* e, int steps)
         int* old32 = e->nbr32;
         e->nbr32 = nullptr;
         e->cap = ((need + 7) / 8) * 8 + 8;
-        dfree(e->nbr16);
-        e->nbr16 = nullptr;
-        TRY(dalloc(e, &e->nbr16, (size_t)e->cap * e->N));
         TRY(dalloc(e, &e->nbr32, (size_t)e->cap * e->N));
         HIPC(e, hipMemcpyAsync(e->nbr32, old32, (size_t)old_cap * e->N * sizeof(int), hipMemcpyDeviceToDevice,
                                e->stream));
@@ -3089,6 +2586,9 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
         return fail(nullptr, SPP_ERR_ARG, "rebuild %d", p->rebuild);
     if (p->kernel_variant < SPP_KERNEL_SLOT || p->kernel_variant > SPP_KERNEL_SLOT_DEEP)
         return fail(nullptr, SPP_ERR_ARG, "kernel_variant %d", p->kernel_variant);
+    if (p->kernel_variant == 1 || p->kernel_variant == 2)
+        return fail(nullptr, SPP_ERR_UNSUPPORTED, "kernel_variant %d is retired (slower than 0 everywhere)",
+                    p->kernel_variant);
     if (systems && p->kernel_variant != SPP_KERNEL_SLOT && p->kernel_variant != SPP_KERNEL_PERSIST &&
         p->kernel_variant != SPP_KERNEL_SLOT_DEEP)
         return fail(nullptr, SPP_ERR_UNSUPPORTED, "batched engines run the slot kernels (kernel_variant 0, 3 or 4)");
@@ -3164,18 +2664,15 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
     e->nkeys = (int)keys;
     e->nscan_blocks = (e->nkeys + kScanTile - 1) / kScanTile;
     e->nblk = nblocks(e->N);
-    e->nchunks = (e->N + kChunk - 1) / kChunk;
-    e->chunk_grid = (e->nchunks + 7) / 8 * 8;
     const bool deep = p->kernel_variant == SPP_KERNEL_SLOT_DEEP;
     e->variant = systems ? kVariantBatch : ((persist || deep) ? SPP_KERNEL_SLOT : p->kernel_variant);
     e->deep = deep;
     e->persist = persist;
-    e->nstat = std::max(e->chunk_grid, e->nblk);
-    // Tile path of the rebuild: single boxes on the slot kernels (the LDS variant needs the general
-    // path's chunk table; captured graphs would freeze the ping-pong of the cell starts).  A tile is
-    // about kSortTarget particles of one cell row, at most kSortMaxW cells.
-    e->sort_ok = !systems && !persist && p->rebuild_sort == SPP_REBUILD_SORT_AUTO &&
-                 e->variant != SPP_KERNEL_STRIP_LDS && !p->graph_intervals;
+    e->nstat = e->nblk;
+    // Tile path of the rebuild: single boxes on the slot kernels (captured graphs would freeze the
+    // ping-pong of the cell starts).  A tile is about kSortTarget particles of one cell row, at most
+    // kSortMaxW cells.
+    e->sort_ok = !systems && !persist && p->rebuild_sort == SPP_REBUILD_SORT_AUTO && !p->graph_intervals;
     if (e->sort_ok) {
         const int W = e->G.ncx;
         const double per_cell = (double)e->N / ((double)W * e->G.ncy);
@@ -3186,8 +2683,7 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
         e->sort_seg = (W + segw - 1) / segw;
         e->sort_ntiles = e->sort_seg * e->G.ncy;
     }
-    // the LDS stage kernels read list rows 0..kPfList-1 unconditionally
-    e->cap = std::max(kPfList, p->ell_capacity > 0 ? p->ell_capacity : 32);
+    e->cap = p->ell_capacity > 0 ? p->ell_capacity : 32;
     {
         spp_params q = *p;
         q.Lx = sd[0].Lx;
@@ -3297,7 +2793,6 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
         A(dalloc(e, &e->slotb[sl].blklen, (size_t)e->nstat));
         A(dalloc(e, &e->slotb[sl].blknnz, (size_t)e->nstat));
     }
-    A(dalloc(e, &e->chunks, (size_t)e->nchunks));
     A(dalloc(e, &e->dpolar, (size_t)e->nblk));
     A(alloc_nbr(e));
     if (systems) {
@@ -3386,7 +2881,7 @@ void spp_destroy(spp_engine* e)
     for (int k = 0; k < kSets; ++k) free_set(e->set[k]);
     void* bufs[] = {e->pos1, e->pos2, e->th1, e->th2, e->dth, e->dth_alt, e->inj_host_order,
                     e->inj, e->kacc, e->force, e->old, e->aux_alt[0], e->aux_alt[1], e->aux_alt[2],
-                    e->nbr16, e->nbr32, e->chunks, e->ncount, e->cellxy, e->build, e->key, e->perm, e->perm2, e->inv, e->cell_count,
+                    e->nbr32, e->ncount, e->cellxy, e->build, e->key, e->perm, e->perm2, e->inv, e->cell_count,
                     e->cell_start, e->block_sums, e->dpolar, e->dsys, e->sys_of, e->dpolar_sys, e->dsys1, e->dsamples,
                     e->pstatus, e->cell_start_alt, e->tile_state, e->sort_fail};
     for (void* b : bufs) dfree(b);
@@ -3510,15 +3005,8 @@ int spp_get_neighbors(spp_engine* e, int32_t* offsets, int32_t* idx, int64_t cap
 {
     TRY(check_ready(e));
     if (!e->list_valid) TRY(spp_build_neighbors(e));
-    const bool lds = e->variant == SPP_KERNEL_STRIP_LDS;
-    int N = e->N, C = e->cap, NC = e->nchunks;
-    std::vector<uint16_t> n16(lds ? (size_t)C * N : 0);
+    int N = e->N, C = e->cap;
     std::vector<int> n32((size_t)C * N), cnt(N), id(N);
-    std::vector<Chunk> ch(lds ? NC : 0);
-    if (lds) {
-        HIPC(e, hipMemcpyAsync(n16.data(), e->nbr16, n16.size() * sizeof(uint16_t), hipMemcpyDeviceToHost, e->stream));
-        HIPC(e, hipMemcpyAsync(ch.data(), e->chunks, NC * sizeof(Chunk), hipMemcpyDeviceToHost, e->stream));
-    }
     HIPC(e, hipMemcpyAsync(n32.data(), e->nbr32, n32.size() * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipMemcpyAsync(cnt.data(), e->ncount, N * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipMemcpyAsync(id.data(), e->set[e->cur].id, N * sizeof(int), hipMemcpyDeviceToHost, e->stream));
@@ -3536,19 +3024,6 @@ int spp_get_neighbors(spp_engine* e, int32_t* offsets, int32_t* idx, int64_t cap
         int64_t o = rowlen[id[k]];
         for (int c = 0; c < cnt[k]; ++c) {
             int slot = n32[(size_t)c * N + k];
-            if (lds) {
-                // the LDS-offset list must name the same neighbour
-                const Chunk& Q = ch[k / kChunk];
-                int l = n16[(size_t)c * N + k];
-                if (!Q.fb && l != kGlobalNbr) {
-                    int q = 0;
-                    while (q < Q.npieces && !(l >= Q.poff[q] && l < Q.poff[q] + Q.plen[q])) ++q;
-                    int s16 = q < Q.npieces ? Q.pst[q] + (l - Q.poff[q]) : -1;
-                    if (s16 != slot)
-                        return fail(e, SPP_ERR_STATE, "LDS list entry %d of slot %d names slot %d, not %d", c, k,
-                                    s16, slot);
-                }
-            }
             idx[o + c] = id[slot];
         }
     }
@@ -3582,11 +3057,8 @@ int spp_interact(spp_engine* e)
     a.sys_of = e->sys_of;
     a.abp = e->abp;
     a.inv = e->inv;
-    a.nchunks = e->nchunks;
-    a.chunks = e->chunks;
     a.gather = e->set[e->cur].pos;
     a.id = e->set[e->cur].id;
-    a.nbr16 = e->nbr16;
     a.nbr32 = e->nbr32;
     a.ncount = e->ncount;
     InteractFn fn = pick_interact(e->p.model, e->p.repulsion, e->variant);
@@ -4137,9 +3609,6 @@ int spp_time_stage(spp_engine* e, int32_t stage, int32_t reps, double* avg_us)
     a.sys_of = e->sys_of;
     a.abp = e->abp;
     a.inv = e->inv;
-    a.nchunks = e->nchunks;
-    a.chunks = e->chunks;
-    a.nbr16 = e->nbr16;
     a.nbr32 = e->nbr32;
     a.ncount = e->ncount;
     a.id = S.id;
@@ -4166,7 +3635,6 @@ int spp_time_stage(spp_engine* e, int32_t stage, int32_t reps, double* avg_us)
     bool prof = e->profiling;
     e->profiling = false;
     auto go = [&]() -> int {
-        if (e->variant == SPP_KERNEL_STRIP_LDS) return launch_chunks(e, "time", fn, a);
         return launch(e, "time", fn, e->nblk, a);
     };
     int rc = go();  // warm

@@ -222,7 +222,7 @@ __device__ int stage_system(const Args& a, const SysInfo& S, const Phys& P, cons
     int bad = 0;
     for (int i = off + threadIdx.x; i < end; i += kBlock1024) {
         double x, y;
-        stage_particle<MODEL, REP, INTEG, STAGE, false>(sa, P, i, off, x, y);
+        stage_particle<MODEL, REP, INTEG, STAGE>(sa, P, i, off, x, y);
         if (a.period == 0) {
             const float2 b = a.nl.build[i];
             const double dx = spp::min_image(x - (double)b.x, P.Lx2, P.invLx2);

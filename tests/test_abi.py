@@ -62,6 +62,11 @@ def test_create_rejects_bad_parameters(engine_mod):
     with pytest.raises(S.SppError) as ei:
         S.Engine(p)
     assert ei.value.code == S.ERR_ARG
+    for v in (1, 2):  # retired variants
+        p = engine_mod.default_params(S.MODEL_REPULSIVE, n=100, kernel_variant=v)
+        with pytest.raises(S.SppError) as ei:
+            S.Engine(p)
+        assert ei.value.code == S.ERR_UNSUPPORTED
     p = engine_mod.default_params(S.MODEL_ABP, n=100, kernel_variant=S.KERNEL_SLOT_DEEP)
     with pytest.raises(S.SppError) as ei:
         S.Engine(p)  # the active Brownian model runs on the slot kernel only

@@ -179,7 +179,7 @@ def test_abp_rejects_unsupported():
     with pytest.raises(S.SppError):
         S.Engine(p.copy().update(n_membrane=2))
     with pytest.raises(S.SppError):
-        S.Engine(p.copy().update(kernel_variant=S.KERNEL_STRIP_LDS))
+        S.Engine(p.copy().update(kernel_variant=S.KERNEL_SLOT_DEEP))  # the slot kernel only
     with pytest.raises(S.SppError):
         S.Engine(p, [S.system(32, 16.0)])
 

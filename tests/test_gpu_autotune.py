@@ -17,10 +17,9 @@ def test_autotune_single_box_keeps_the_physics():
     p = c2_params(n=n, L=L)
     x, y, th = S.random_state(4, n, L, L)
     tuned = autotune(p, x, y, th, steps=60, warm=20, cell_scales=(1.0, 1.5))
-    # slot, split, deep slot and LDS variants x three list radii x two cell sizes
-    assert len(tuned.trials) == 4 * 3 * 2
-    assert {t.kernel_variant for t in tuned.trials} == {S.KERNEL_SLOT, S.KERNEL_SLOT_SPLIT, S.KERNEL_SLOT_DEEP,
-                                                       S.KERNEL_STRIP_LDS}
+    # slot and deep slot variants x three list radii x two cell sizes
+    assert len(tuned.trials) == 2 * 3 * 2
+    assert {t.kernel_variant for t in tuned.trials} == {S.KERNEL_SLOT, S.KERNEL_SLOT_DEEP}
     # a skin too thin for the workload's fastest particles is an invalid candidate, with its reason
     for t in tuned.trials:
         assert (t.rate > 0) != bool(t.error), t

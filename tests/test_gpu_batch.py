@@ -238,8 +238,8 @@ def test_batch_rejects_bad_input():
         S.Engine(p, [])
     with pytest.raises(S.SppError):  # box too small for three cells of r_list
         S.Engine(p, [S.system(10, 1.0)])
-    with pytest.raises(S.SppError):  # LDS strips are single-box only
-        S.Engine(p.copy().update(kernel_variant=S.KERNEL_STRIP_LDS), [S.system(100, 5.0)])
+    with pytest.raises(S.SppError):  # a retired kernel variant
+        S.Engine(p.copy().update(kernel_variant=2), [S.system(100, 5.0)])
 
 
 # ---------------------------------------------------------------------------------------------
