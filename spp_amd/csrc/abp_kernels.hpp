@@ -38,8 +38,8 @@ __device__ __forceinline__ void ring_forces(const StageArgs& a, const Phys& P, i
 {
     const AbpConst& A = a.abp;
     const int Nm = A.n_membrane;
-    const P4 qm1 = a.gather[a.inv[(pid + Nm - 1) % Nm]];
-    const P4 qp1 = a.gather[a.inv[(pid + 1) % Nm]];
+    const P4 qm1 = stage_record(a, a.inv[(pid + Nm - 1) % Nm]);
+    const P4 qp1 = stage_record(a, a.inv[(pid + 1) % Nm]);
     // bond vectors along the ring: t1 = r_k - r_{k-1}, t2 = r_{k+1} - r_k
     double t1x = spp::min_image(me.x - qm1.x, P.Lx2, P.invLx2);
     double t1y = spp::min_image(me.y - qm1.y, P.Ly2, P.invLy2);
@@ -53,8 +53,8 @@ __device__ __forceinline__ void ring_forces(const StageArgs& a, const Phys& P, i
     fx -= A.k_bond * (A.l_bond - d1) * (-t1x) / d1;
     fy -= A.k_bond * (A.l_bond - d1) * (-t1y) / d1;
     if (A.kappa != 0.0) {
-        const P4 qm2 = a.gather[a.inv[(pid + Nm - 2) % Nm]];
-        const P4 qp2 = a.gather[a.inv[(pid + 2) % Nm]];
+        const P4 qm2 = stage_record(a, a.inv[(pid + Nm - 2) % Nm]);
+        const P4 qp2 = stage_record(a, a.inv[(pid + 2) % Nm]);
         double t0x = spp::min_image(qm1.x - qm2.x, P.Lx2, P.invLx2);
         double t0y = spp::min_image(qm1.y - qm2.y, P.Ly2, P.invLy2);
         double t3x = spp::min_image(qp2.x - qp1.x, P.Lx2, P.invLx2);
@@ -114,7 +114,7 @@ __device__ __forceinline__ void chain_gather(const StageArgs& a, const Phys& P, 
     double s1 = 0, s2 = 0;
     for (int k = 0; k < cnt; ++k) {
         const int j = a.nbr32[(size_t)k * a.N + i];
-        const P4 q = a.gather[j];
+        const P4 q = stage_record(a, j);
         const int nbj = (unsigned)a.id[j] < (unsigned)A.n_membrane ? 1 : A.nb;
         if (na == 1 && nbj == 1) {
             spp::pair_accumulate<SPP_MODEL_REPULSIVE, REP>(P, me, mc, ms, q, fx, fy, s1, s2);
@@ -156,7 +156,7 @@ __device__ __forceinline__ void abp_pair_side(const StageArgs& a, const Phys& P,
                                               double& fy, double& tq)
 {
     const AbpConst& A = a.abp;
-    const P4 me = a.gather[si], q = a.gather[sj];
+    const P4 me = stage_record(a, si), q = stage_record(a, sj);
     double mc, ms;
     spp::heading_unit(me.z, mc, ms);
     const int na = A.nb > 1 && !((unsigned)a.id[si] < (unsigned)A.n_membrane) ? A.nb : 1;
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_abp(StageArgs a)
     constexpr bool final_stage = INTEG == SPP_EULER || STAGE == 2;
     double x = 0, y = 0, dmax = 0;
     if (live) {
-        P4 me = a.gather[i];
+        P4 me = stage_record(a, i);
         double mc, ms;  // heading of the stage input (stage 1: the old one; stage 2: the midpoint)
         spp::heading_unit(me.z, mc, ms);
         int pid = a.id[i];
@@ -201,7 +201,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_abp(StageArgs a)
         double2 u = make_double2(0, 0);
         if (STAGE <= 1) {
             if (a.id_out) a.id_out[i] = pid;
-            th_in = a.th_in[i];
+            th_in = me.z;
             if (INTEG == SPP_EULER) u = a.U_in[i];
             dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)pid, noise_step(a));
         } else {
@@ -252,8 +252,8 @@ __global__ __launch_bounds__(kBlock) void k_stage_abp(StageArgs a)
             x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
             y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
         }
-        a.out[i] = P4{x, y, th, 0.0};
-        a.th_out[i] = th;
+        a.oxy[i] = make_double2(x, y);
+        a.oth[i] = th;
     }
     check_disp(a, P, b, live, i, x, y, final_stage ? a.kstep + 1 : a.kstep, dmax);
     finish_disp(a, b, dmax);
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(kBlock) void k_interact_abp(StageArgs a, P4* force)
 {
     int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.N) return;
-    P4 me = a.gather[i];
+    P4 me = stage_record(a, i);
     double mc, ms, fx, fy, tq;
     spp::heading_unit(me.z, mc, ms);
     abp_forces<REP>(a, a.P, i, a.id[i], me, mc, ms, fx, fy, tq);
@@ -278,10 +278,10 @@ __global__ __launch_bounds__(kBlock) void k_move_abp(MoveArgs a)
     if (i >= a.N) return;
     const Phys& P = a.P;
     const AbpConst& A = a.abp;
-    P4 me = a.cur.pos[i];
-    double mc, ms;
-    spp::heading_unit(me.z, mc, ms);
+    const double2 me = a.cur.xy[i];
     double th_in = a.cur.th[i];
+    double mc, ms;
+    spp::heading_unit(th_in, mc, ms);
     P4 f = a.force[i];
     double2 u = a.cur.U[i];
     double mpar, mperp, krot;
@@ -330,6 +330,6 @@ __global__ __launch_bounds__(kBlock) void k_move_abp(MoveArgs a)
         x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
         y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
     }
-    a.cur.pos[i] = P4{x, y, th, 0.0};
+    a.cur.xy[i] = make_double2(x, y);
     a.cur.th[i] = th;
 }

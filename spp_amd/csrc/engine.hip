@@ -2,12 +2,11 @@
 // declared in include/spp_engine.h.
 //
 // Device layout (all per-particle arrays in CELL-SORTED slot order; id[slot] = caller's index):
-//   state set (x3)  pos P4[N] (x, y, th, 0): wrapped position and heading angle, the record
-//                   neighbours gather; th double[N] (the same angles, read by the host-facing
-//                   and Lyapunov code); U double2[N] (unwrapped positions, reference
-//                   r_original); id int[N]
-//   pos1/th1, pos2/th2  RK stage states (pos1 = RK2 midpoint); their records carry the step's
-//                   starting angle theta_old in the fourth field
+//   state set (x3-4)  xy double2[N] (wrapped positions) and th double[N] (heading angles): the
+//                   record neighbours gather, 24 B in two arrays; U double2[N] (unwrapped
+//                   positions, reference r_original); id int[N]
+//   xy1/th1, xy2/th2  RK stage states (xy1 = RK2 midpoint); the step's starting angle
+//                   theta_old is the input set's th, which no stage overwrites before the last
 //   dth    double[N]   this step's angular noise
 //   kacc   P4[N]       RK4 accumulator k1 + 2 k2 + 2 k3 (fx, fy, tau)
 //   nbr    int[cap*N]  full Verlet list, ELL, slot-major (nbr[k*N + i]): a wavefront's k-th
@@ -167,7 +166,7 @@ __device__ __forceinline__ double wave_max(double v)
 // rebuilds, so equal keys mostly form runs of neighbouring lanes: the first lane of a run reserves
 // the whole run with one atomic and its lanes take consecutive ranks (equal keys in separate runs
 // reserve separately; any arrival order is fine, the id order inside cells is restored later).
-__global__ __launch_bounds__(kBlock) void k_bin(int N, Grid G0, const P4* __restrict__ pos,
+__global__ __launch_bounds__(kBlock) void k_bin(int N, Grid G0, const double2* __restrict__ pos,
                                                 int* __restrict__ key, int* __restrict__ count,
                                                 int* __restrict__ rank, const SysInfo* __restrict__ sys_tab,
                                                 const int* __restrict__ sys_of)
@@ -184,7 +183,7 @@ __global__ __launch_bounds__(kBlock) void k_bin(int N, Grid G0, const P4* __rest
             G = S.G;
             koff = S.key_off;
         }
-        P4 p = pos[i];
+        const double2 p = pos[i];
         int cx = cell_coord(p.x, G.Lx, G.Lx2, G.ncx, G.quirk);
         int cy = cell_coord(p.y, G.Ly, G.Ly2, G.ncy, G.quirk);
         k = koff + cy * G.ncx + cx;
@@ -295,11 +294,13 @@ __global__ __launch_bounds__(kBlock) void k_scatter(int N, const int* __restrict
     perm[start[key[i]] + rank[i]] = i;
 }
 
+// One state set in slot order: the record neighbours gather is (xy, th), two arrays so that a
+// gather touches 24 B per neighbour (16 B + 8 B) where an interleaved 32-B record would touch 32.
 struct SetPtrs {
-    P4* pos;
-    double* th;
-    double2* U;
-    int* id;
+    double2* xy;  // wrapped position (reference r)
+    double* th;   // heading angle (reference theta)
+    double2* U;   // unwrapped position (reference r_original)
+    int* id;      // caller index
 };
 
 struct PermuteArgs {
@@ -317,7 +318,7 @@ struct PermuteArgs {
 // Canonical order inside each cell (the scatter leaves arrival order) and the permutation in one
 // pass: a particle's slot is its cell's start plus the number of cell mates with a smaller id (one
 // pass over the cell per particle, so a crowded cell costs each of its particles one pass), and its
-// state is copied there.  perm2 keeps the order for the chunk builder.
+// state is copied there.  perm2 keeps the permutation.
 __global__ __launch_bounds__(kBlock) void k_rank_permute(PermuteArgs a, const int* __restrict__ key,
                                                          const int* __restrict__ start, const int* __restrict__ perm,
                                                          int* __restrict__ perm2)
@@ -331,7 +332,7 @@ __global__ __launch_bounds__(kBlock) void k_rank_permute(PermuteArgs a, const in
     for (int q = s; q < e; ++q) r += a.in.id[perm[q]] < mine;
     const int k = s + r;
     perm2[k] = me;
-    a.out.pos[k] = a.in.pos[me];
+    a.out.xy[k] = a.in.xy[me];
     a.out.th[k] = a.in.th[me];
     a.out.U[k] = a.in.U[me];
     a.out.id[k] = mine;
@@ -414,13 +415,10 @@ __device__ __forceinline__ void tile_rank(int q, const uint16_t* order, const ui
 
 // Moves one particle's state to its slot; returns 1 when the slot falls outside the engine (a
 // failed sort).
-__device__ __forceinline__ int tile_put(const SortArgs& a, int slot, int mine, double2 xy, double2 zw, double th,
-                                        double2 u)
+__device__ __forceinline__ int tile_put(const SortArgs& a, int slot, int mine, double2 xy, double th, double2 u)
 {
     if (slot >= a.N) return 1;
-    double2* w = reinterpret_cast<double2*>(&a.out.pos[slot]);
-    w[0] = xy;
-    w[1] = zw;
+    a.out.xy[slot] = xy;
     a.out.th[slot] = th;
     a.out.U[slot] = u;
     a.out.id[slot] = mine;
@@ -495,7 +493,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_tiles(SortArgs a)
             for (int k = 1; k < 6; ++k) o = q >= p[k] ? off[k] : o;
             sl[u] = q < total ? q + o : -1;
             if (sl[u] >= 0) {
-                xy[u] = *reinterpret_cast<const double2*>(&a.in.pos[sl[u]].x);
+                xy[u] = a.in.xy[sl[u]];
                 idv[u] = a.in.id[sl[u]];
             }
         }
@@ -556,12 +554,10 @@ __global__ __launch_bounds__(kBlock) void k_sort_tiles(SortArgs a)
     int ls0 = -1, id0 = 0, sr0 = 0, ls1 = -1, id1 = 0, sr1 = 0;
     if (t < n) tile_rank(t, order, e_col, e_id, e_old, hist, ls0, id0, sr0);
     if (t + kBlock < n) tile_rank(t + kBlock, order, e_col, e_id, e_old, hist, ls1, id1, sr1);
-    const double2 xy0 = reinterpret_cast<const double2*>(&a.in.pos[sr0])[0];
-    const double2 zw0 = reinterpret_cast<const double2*>(&a.in.pos[sr0])[1];
+    const double2 xy0 = a.in.xy[sr0];
     const double th0 = a.in.th[sr0];
     const double2 u0 = a.in.U[sr0];
-    const double2 xy1 = reinterpret_cast<const double2*>(&a.in.pos[sr1])[0];
-    const double2 zw1 = reinterpret_cast<const double2*>(&a.in.pos[sr1])[1];
+    const double2 xy1 = a.in.xy[sr1];
     const double th1 = a.in.th[sr1];
     const double2 u1 = a.in.U[sr1];
 
@@ -619,15 +615,14 @@ __global__ __launch_bounds__(kBlock) void k_sort_tiles(SortArgs a)
     for (int c = t; c < w; c += kBlock) a.cs_new[y * W + c0 + c] = clamp_slot(base + hist[c], N);
     if (tile == a.ntiles - 1 && t == 0) a.cs_new[H * W] = clamp_slot(base + n, N);
     int put_bad = 0;
-    if (ls0 >= 0) put_bad |= tile_put(a, base + ls0, id0, xy0, zw0, th0, u0);
-    if (ls1 >= 0) put_bad |= tile_put(a, base + ls1, id1, xy1, zw1, th1, u1);
+    if (ls0 >= 0) put_bad |= tile_put(a, base + ls0, id0, xy0, th0, u0);
+    if (ls1 >= 0) put_bad |= tile_put(a, base + ls1, id1, xy1, th1, u1);
     if (put_bad) s_bad = 1;
     // particles beyond two per thread (crowded tiles)
     for (int q = t + 2 * kBlock; q < n; q += kBlock) {
         int ls, me, sr;
         tile_rank(q, order, e_col, e_id, e_old, hist, ls, me, sr);
-        const double2* r = reinterpret_cast<const double2*>(&a.in.pos[sr]);
-        if (tile_put(a, base + ls, me, r[0], r[1], a.in.th[sr], a.in.U[sr])) s_bad = 1;
+        if (tile_put(a, base + ls, me, a.in.xy[sr], a.in.th[sr], a.in.U[sr])) s_bad = 1;
     }
     __syncthreads();
     if (t == 0 && s_bad) atomicOr(a.fail, 1);
@@ -656,11 +651,12 @@ struct StageArgs {
     const int* inv;        // caller index -> slot (ring neighbours of the active Brownian model)
     const SysInfo* sys_tab;  // batched engine (null: single system, constants in P)
     const int* sys_of;
-    const P4* gather;      // stage input: neighbours' (and own) position and heading
-    const double* th_in;   // own heading angle of the stage input
-    P4* out;               // stage output
-    double* th_out;
-    const double* th0;     // step-start heading (theta_old) for RK stages >= 2
+    const double2* gxy;    // stage input positions (neighbours' and own)
+    const double* gth;     // stage input headings (neighbours' and own)
+    double2* oxy;          // stage output positions
+    double* oth;           // stage output headings
+    const double* th0;     // step-start heading (theta_old) for RK stages >= 2: the input set's,
+                           // which no stage overwrites before the step's last
     const double2* U_in;   // step-start unwrapped position
     double2* U_out;        // written by the final stage
     double* dth;           // angular noise of this step
@@ -686,6 +682,13 @@ struct StageArgs {
 __device__ __forceinline__ int64_t noise_step(const StageArgs& a)
 {
     return a.clock ? a.step + *a.clock : a.step;
+}
+
+// Slot j's record of the stage input: (x, y, theta) in a P4 (w unused).
+__device__ __forceinline__ P4 stage_record(const StageArgs& a, int j)
+{
+    const double2 p = a.gxy[j];
+    return P4{p.x, p.y, a.gth[j], 0.0};
 }
 
 // Displacement trigger: marks the step (per workgroup, atomics stay inside one workgroup's slot)
@@ -730,7 +733,7 @@ struct NlistFlatArgs {
     const SysInfo* sys_tab;  // batched engine (null: single system)
     const int* sys_of;
     double T_list;
-    const P4* pos;
+    const double2* pos;
     const int* start;
     int* nbr32;
     int* ncount;
@@ -749,7 +752,7 @@ struct NlistFlatArgs {
 __device__ __forceinline__ int nlist_particle(const NlistFlatArgs& a, int i, const Grid& G, int koff)
 {
     constexpr int kSeg = 6, kBatch = 8;
-    const double2 me = *reinterpret_cast<const double2*>(&a.pos[i].x);
+    const double2 me = a.pos[i];
     const int cx = cell_coord(me.x, G.Lx, G.Lx2, G.ncx, G.quirk);
     const int cy = cell_coord(me.y, G.Ly, G.Ly2, G.ncy, G.quirk);
     int x0 = cx - 1, x1 = cx + 1, xw = -1;
@@ -793,7 +796,7 @@ __device__ __forceinline__ int nlist_particle(const NlistFlatArgs& a, int i, con
 #pragma unroll
             for (int k = 1; k < kSeg; ++k) o = t >= p[k] ? off[k] : o;
             jj[u] = t + o;
-            if (t < total) q[u] = *reinterpret_cast<const double2*>(&a.pos[jj[u]].x);
+            if (t < total) q[u] = a.pos[jj[u]];
         }
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
@@ -814,23 +817,23 @@ __device__ __forceinline__ int nlist_particle(const NlistFlatArgs& a, int i, con
     return c;
 }
 
-__global__ __launch_bounds__(kBlock) void k_nlist_flat(NlistFlatArgs a)
+// Grid and first cell key of slot i's system.
+__device__ __forceinline__ void system_grid(const NlistFlatArgs& a, int i, Grid& G, int& koff)
+{
+    G = a.G;
+    koff = 0;
+    if (a.sys_of) {
+        const SysInfo& S = a.sys_tab[a.sys_of[i]];
+        G = S.G;
+        koff = S.key_off;
+    }
+}
+
+// Longest list and entry count of workgroup b (list lengths before the capacity cut), for the
+// interval report.
+__device__ __forceinline__ void list_stats(const NlistFlatArgs& a, int b, int c)
 {
     __shared__ int red_len[kBlock / 64], red_nnz[kBlock / 64];
-    const int b = xcd_block();
-    int i = b * blockDim.x + threadIdx.x;
-    bool live = i < a.N;
-    int c = 0;
-    if (live) {
-        Grid G = a.G;
-        int koff = 0;
-        if (a.sys_of) {
-            const SysInfo& S = a.sys_tab[a.sys_of[i]];
-            G = S.G;
-            koff = S.key_off;
-        }
-        c = nlist_particle(a, i, G, koff);
-    }
     int m = c;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
@@ -849,6 +852,21 @@ __global__ __launch_bounds__(kBlock) void k_nlist_flat(NlistFlatArgs a)
         a.blklen[b] = mm;
         a.blknnz[b] = tt;
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_nlist_flat(NlistFlatArgs a)
+{
+    const int b = xcd_block();
+    int i = b * blockDim.x + threadIdx.x;
+    bool live = i < a.N;
+    int c = 0;
+    if (live) {
+        Grid G;
+        int koff;
+        system_grid(a, i, G, koff);
+        c = nlist_particle(a, i, G, koff);
+    }
+    list_stats(a, b, c);
 }
 
 // GU: neighbour records in flight per lane (a block of the list loop).  Small systems, whose stage
@@ -870,7 +888,7 @@ __device__ __forceinline__ void gather_forces_flat(const StageArgs& a, const Phy
         P4 q[GU];
 #pragma unroll
         for (int u = 0; u < GU; ++u)
-            if (k + u < cnt) q[u] = a.gather[j[u]];
+            if (k + u < cnt) q[u] = stage_record(a, j[u]);
 #pragma unroll
         for (int u = 0; u < GU; ++u)
             if (k + u < cnt) spp::pair_accumulate<MODEL, REP>(P, me, mc, ms, q[u], fx, fy, s1, s2);
@@ -885,7 +903,7 @@ template <int MODEL, int REP, int INTEG, int STAGE, int GU = 4>
 __device__ __forceinline__ void stage_particle(const StageArgs& a, const Phys& P, int i, int pid_off, double& x,
                                                double& y)
 {
-    P4 me = a.gather[i];
+    P4 me = stage_record(a, i);
     double mc, ms;  // v = (cos, sin) of the stage input's heading
     spp::heading_unit(me.z, mc, ms);
     double dn = 0, th_in = 0, thold = 0;
@@ -900,7 +918,7 @@ __device__ __forceinline__ void stage_particle(const StageArgs& a, const Phys& P
         // equal seeds (Lyapunov clones) draw identical noise
         dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)(pid - pid_off), noise_step(a));
     } else {
-        thold = me.w;  // theta_old, carried by the middle stages' records
+        thold = a.th0[i];  // theta_old: the step's input set, which no stage has overwritten yet
         dn = a.dth[i];
         u = a.U_in[i];
         if (INTEG == SPP_RK4) kk = a.kacc[i];
@@ -946,12 +964,8 @@ __device__ __forceinline__ void stage_particle(const StageArgs& a, const Phys& P
         x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
         y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
     }
-    // The record holds the stage's heading angle; a middle RK stage's record also carries the
-    // step's starting heading (theta_old) for the later stages.  The angle array of the state set
-    // is written by a step's final stage only (the middle stages' angles live in their records).
-    constexpr bool kFinal = (INTEG == SPP_EULER) || (INTEG == SPP_RK2 && STAGE == 2) || (STAGE == 4);
-    spp::store_out(a.stream, &a.out[i], x, y, th, kFinal ? 0.0 : (STAGE <= 1 ? th_in : thold));
-    if (kFinal) spp::store_out(a.stream, &a.th_out[i], th);
+    spp::store_out(a.stream, &a.oxy[i], make_double2(x, y));
+    spp::store_out(a.stream, &a.oth[i], th);
 }
 
 template <int MODEL, int REP, int INTEG, int STAGE, bool BATCH, int GU = 4>
@@ -981,7 +995,7 @@ __global__ __launch_bounds__(kBlock) void k_interact_flat(StageArgs a, P4* force
     if (i >= a.N) return;
     Phys P = a.P;
     if (BATCH) apply_sys(P, a.sys_tab[a.sys_of[i]]);
-    P4 me = a.gather[i];
+    P4 me = stage_record(a, i);
     double mc, ms, fx, fy, tq;
     spp::heading_unit(me.z, mc, ms);
     gather_forces_flat<MODEL, REP>(a, P, i, me, mc, ms, fx, fy, tq);
@@ -1021,7 +1035,7 @@ __global__ __launch_bounds__(kBlock) void k_move(MoveArgs a)
         apply_sys(P, S);
         pid_off = S.pid_off;
     }
-    P4 me = a.cur.pos[i];
+    const double2 me = a.cur.xy[i];
     double th_in = a.cur.th[i];
     P4 f = a.force[i];
     double2 u = a.cur.U[i];
@@ -1072,7 +1086,7 @@ __global__ __launch_bounds__(kBlock) void k_move(MoveArgs a)
         x = spp::periodic_wrap(u.x, P.Lx, P.Lx2);
         y = spp::periodic_wrap(u.y, P.Ly, P.Ly2);
     }
-    a.cur.pos[i] = P4{x, y, th, 0.0};
+    a.cur.xy[i] = make_double2(x, y);
     a.cur.th[i] = th;
 }
 
@@ -1093,20 +1107,20 @@ __global__ __launch_bounds__(kBlock) void k_copy_set(int N, SetPtrs src, SetPtrs
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    dst.pos[i] = src.pos[i];
+    dst.xy[i] = src.xy[i];
     dst.th[i] = src.th[i];
     dst.U[i] = src.U[i];
     dst.id[i] = src.id[i];
 }
 
 // Polarization partial sums (Box::Compute_Polarization, serial/box.h:531-538), two-pass.
-__global__ __launch_bounds__(kBlock) void k_polar(int N, const P4* __restrict__ pos,
+__global__ __launch_bounds__(kBlock) void k_polar(int N, const double* __restrict__ th,
                                                   double2* __restrict__ part)
 {
     __shared__ double sx[kBlock / 64], sy[kBlock / 64];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double vx = 0, vy = 0;
-    if (i < N) spp::heading_unit(pos[i].z, vx, vy);
+    if (i < N) spp::heading_unit(th[i], vx, vy);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         vx += __shfl_xor(vx, o);
@@ -1128,7 +1142,7 @@ __global__ __launch_bounds__(kBlock) void k_polar(int N, const P4* __restrict__ 
 }
 
 // Per-system heading sums of a batched engine: one workgroup per system over its slot range.
-__global__ __launch_bounds__(kBlock) void k_polar_sys(const SysInfo* __restrict__ sys, const P4* __restrict__ pos,
+__global__ __launch_bounds__(kBlock) void k_polar_sys(const SysInfo* __restrict__ sys, const double* __restrict__ th,
                                                       double2* __restrict__ out)
 {
     __shared__ double sx[kBlock / 64], sy[kBlock / 64];
@@ -1136,7 +1150,7 @@ __global__ __launch_bounds__(kBlock) void k_polar_sys(const SysInfo* __restrict_
     double vx = 0, vy = 0;
     for (int i = S.pid_off + threadIdx.x; i < S.pid_off + S.n; i += kBlock) {
         double c, s;
-        spp::heading_unit(pos[i].z, c, s);
+        spp::heading_unit(th[i], c, s);
         vx += c;
         vy += s;
     }
@@ -1162,7 +1176,7 @@ __global__ __launch_bounds__(kBlock) void k_polar_sys(const SysInfo* __restrict_
 
 // Pair-distance histogram over the full Verlet list (each pair counted from both ends), integer
 // counts so the result does not depend on the order of the atomic additions.
-__global__ __launch_bounds__(kBlock) void k_pair_hist(int N, const P4* __restrict__ pos, const int* __restrict__ nbr,
+__global__ __launch_bounds__(kBlock) void k_pair_hist(int N, const double2* __restrict__ pos, const int* __restrict__ nbr,
                                                       const int* __restrict__ ncount, Grid G, double inv_dr,
                                                       int nbins, unsigned long long* __restrict__ hist)
 {
@@ -1171,10 +1185,10 @@ __global__ __launch_bounds__(kBlock) void k_pair_hist(int N, const P4* __restric
     __syncthreads();
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < N) {
-        P4 me = pos[i];
+        const double2 me = pos[i];
         int c = ncount[i];
         for (int k = 0; k < c; ++k) {
-            P4 q = pos[nbr[(size_t)k * N + i]];
+            const double2 q = pos[nbr[(size_t)k * N + i]];
             double dx = spp::min_image(me.x - q.x, G.Lx2, G.invLx2);
             double dy = spp::min_image(me.y - q.y, G.Ly2, G.invLy2);
             int b = (int)(sqrt(dx * dx + dy * dy) * inv_dr);
@@ -1320,7 +1334,7 @@ __global__ __launch_bounds__(kBlock) void k_reset_forces(StageArgs a, int abp, P
     Phys P = a.P;
     if (a.sys_of) apply_sys(P, a.sys_tab[a.sys_of[i]]);
     double mc, ms;
-    spp::heading_unit(a.gather[i].z, mc, ms);
+    spp::heading_unit(a.gth[i], mc, ms);
     double v0 = P.v0, tq = 0.0;
     if (abp) {
         const bool bead = (unsigned)a.id[i] < (unsigned)a.abp.n_membrane;
@@ -1340,10 +1354,10 @@ __device__ __forceinline__ void pair_side(const StageArgs& a, const Phys& P, int
         abp_pair_side<REP>(a, P, si, sj, fx, fy, tq);
         return;
     }
-    const P4 me = a.gather[si];
+    const P4 me = stage_record(a, si);
     double mc, ms, s1 = 0, s2 = 0;
     spp::heading_unit(me.z, mc, ms);
-    spp::pair_accumulate<MODEL, REP>(P, me, mc, ms, a.gather[sj], fx, fy, s1, s2);
+    spp::pair_accumulate<MODEL, REP>(P, me, mc, ms, stage_record(a, sj), fx, fy, s1, s2);
     tq += spp::drift_torque<MODEL>(P, s1, s2);
 }
 
@@ -1531,7 +1545,7 @@ struct spp_engine {
     SetPtrs set[kSets] = {};
     int nsets = 3;
     int cur = 0;
-    P4 *pos1 = nullptr, *pos2 = nullptr;
+    double2 *xy1 = nullptr, *xy2 = nullptr;
     double *th1 = nullptr, *th2 = nullptr;
     double *dth = nullptr, *dth_alt = nullptr, *inj_host_order = nullptr, *inj = nullptr;
     P4 *kacc = nullptr, *force = nullptr, *old = nullptr;
@@ -1743,7 +1757,7 @@ int alloc_nbr(spp_engine* e)
 
 int alloc_set(spp_engine* e, SetPtrs& s)
 {
-    TRY(dalloc(e, &s.pos, e->N));
+    TRY(dalloc(e, &s.xy, e->N));
     TRY(dalloc(e, &s.th, e->N));
     TRY(dalloc(e, &s.U, e->N));
     TRY(dalloc(e, &s.id, e->N));
@@ -1752,7 +1766,7 @@ int alloc_set(spp_engine* e, SetPtrs& s)
 
 void free_set(SetPtrs& s)
 {
-    dfree(s.pos);
+    dfree(s.xy);
     dfree(s.th);
     dfree(s.U);
     dfree(s.id);
@@ -1764,7 +1778,7 @@ void free_set(SetPtrs& s)
 // ------------------------------------------------------------------------------------------------
 int launch_nlist(spp_engine* e, int slot)
 {
-    NlistFlatArgs fa{e->N, e->cap, e->G, e->dsys, e->sys_of, e->T_list, e->set[e->cur].pos, e->cell_start, e->nbr32,
+    NlistFlatArgs fa{e->N, e->cap, e->G, e->dsys, e->sys_of, e->T_list, e->set[e->cur].xy, e->cell_start, e->nbr32,
                      e->ncount, e->build, e->cellxy, e->slotb[slot].blklen, e->slotb[slot].blknnz};
     return launch(e, "nlist", k_nlist_flat, e->nblk, fa);
 }
@@ -1810,7 +1824,7 @@ int launch_rebuild(spp_engine* e, int dst, int slot, bool aux, bool tile)
     e->force_general = false;
     e->order_valid = true;  // the general path sorts any order
     // the arrival ranks live in perm2 until the per-cell ordering overwrites it
-    TRY(launch(e, "bin", k_bin, nblocks(N), N, e->G, (const P4*)e->set[src].pos, e->key,
+    TRY(launch(e, "bin", k_bin, nblocks(N), N, e->G, (const double2*)e->set[src].xy, e->key,
                e->cell_count, e->perm2, (const SysInfo*)e->dsys, (const int*)e->sys_of));
     TRY(launch(e, "scan_partial", k_scan_partial, e->nscan_blocks, (const int*)e->cell_count, nk,
                e->block_sums));
@@ -1931,24 +1945,24 @@ int launch_step(spp_engine* e, int S, int W, int kstep, int slot, bool record)
         int stage = integ == SPP_EULER ? 0 : s;
         a.record = (record && s == nst) ? 1 : 0;
         if (integ == SPP_EULER) {
-            a.gather = e->set[S].pos;
-            a.th_in = e->set[S].th;
-            a.out = e->pos1;
-            a.th_out = e->th1;
+            a.gxy = e->set[S].xy;
+            a.gth = e->set[S].th;
+            a.oxy = e->xy1;
+            a.oth = e->th1;
         } else if (integ == SPP_RK2) {
-            a.gather = s == 1 ? e->set[S].pos : e->pos1;
-            a.th_in = s == 1 ? e->set[S].th : e->th1;
-            a.out = s == 1 ? e->pos1 : e->set[W].pos;
-            a.th_out = s == 1 ? e->th1 : e->set[W].th;
+            a.gxy = s == 1 ? e->set[S].xy : e->xy1;
+            a.gth = s == 1 ? e->set[S].th : e->th1;
+            a.oxy = s == 1 ? e->xy1 : e->set[W].xy;
+            a.oth = s == 1 ? e->th1 : e->set[W].th;
         } else {
-            const P4* g[5] = {nullptr, e->set[S].pos, e->pos1, e->pos2, e->pos1};
+            const double2* g[5] = {nullptr, e->set[S].xy, e->xy1, e->xy2, e->xy1};
             const double* gt[5] = {nullptr, e->set[S].th, e->th1, e->th2, e->th1};
-            P4* o[5] = {nullptr, e->pos1, e->pos2, e->pos1, e->set[W].pos};
+            double2* o[5] = {nullptr, e->xy1, e->xy2, e->xy1, e->set[W].xy};
             double* ot[5] = {nullptr, e->th1, e->th2, e->th1, e->set[W].th};
-            a.gather = g[s];
-            a.th_in = gt[s];
-            a.out = o[s];
-            a.th_out = ot[s];
+            a.gxy = g[s];
+            a.gth = gt[s];
+            a.oxy = o[s];
+            a.oth = ot[s];
         }
         a.stream = e->N <= kStreamMaxN;
         StageFn fn = pick_stage(e->p.model, e->p.repulsion, integ, stage, e->variant, e->deep || e->N < kSmallN);
@@ -1957,7 +1971,7 @@ int launch_step(spp_engine* e, int S, int W, int kstep, int slot, bool record)
         a.id_out = nullptr;
     }
     if (integ == SPP_EULER) {
-        std::swap(e->set[W].pos, e->pos1);
+        std::swap(e->set[W].xy, e->xy1);
         std::swap(e->set[W].th, e->th1);
     }
     e->cur = W;
@@ -1970,7 +1984,7 @@ int launch_step(spp_engine* e, int S, int W, int kstep, int slot, bool record)
         if (k >= 0 && k < e->sample_cap) {
             const int ns = e->dsys ? e->nsys : 1;
             TRY(launch(e, "polar_sample", k_polar_sys, ns, (const SysInfo*)(e->dsys ? e->dsys : e->dsys1),
-                       (const P4*)e->set[W].pos, e->dsamples + k * ns));
+                       (const double*)e->set[W].th, e->dsamples + k * ns));
         }
     }
     return SPP_OK;
@@ -2283,8 +2297,8 @@ int run_persist(spp_engine* e, int steps)
         a.sys_tab = e->dsys ? e->dsys : e->dsys1;
         a.cur = e->set[e->cur];
         a.spare = e->set[spare];
-        a.pos1 = e->pos1;
-        a.pos2 = e->pos2;
+        a.xy1 = e->xy1;
+        a.xy2 = e->xy2;
         a.th1 = e->th1;
         a.th2 = e->th2;
         a.key = e->key;
@@ -2382,7 +2396,7 @@ lyap::Args lyap_args(spp_engine* e)
     a.nchunk = L.nchunk;
     a.sys = e->dsys;
     a.inv = L.inv;
-    a.pos = s.pos;
+    a.pos = s.xy;
     a.th = s.th;
     a.U = s.U;
     a.dev = L.dev;
@@ -2760,8 +2774,8 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
     A(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) == hipSuccess ? SPP_OK : SPP_ERR_HIP);
     e->nsets = e->sort_ok ? 4 : 3;
     for (int k = 0; k < e->nsets; ++k) A(alloc_set(e, e->set[k]));
-    A(dalloc(e, &e->pos1, N));
-    A(dalloc(e, &e->pos2, N));
+    A(dalloc(e, &e->xy1, N));
+    A(dalloc(e, &e->xy2, N));
     A(dalloc(e, &e->th1, N));
     A(dalloc(e, &e->th2, N));
     A(dalloc(e, &e->dth, N));
@@ -2879,7 +2893,7 @@ void spp_destroy(spp_engine* e)
     if (!e) return;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     for (int k = 0; k < kSets; ++k) free_set(e->set[k]);
-    void* bufs[] = {e->pos1, e->pos2, e->th1, e->th2, e->dth, e->dth_alt, e->inj_host_order,
+    void* bufs[] = {e->xy1, e->xy2, e->th1, e->th2, e->dth, e->dth_alt, e->inj_host_order,
                     e->inj, e->kacc, e->force, e->old, e->aux_alt[0], e->aux_alt[1], e->aux_alt[2],
                     e->nbr32, e->ncount, e->cellxy, e->build, e->key, e->perm, e->perm2, e->inv, e->cell_count,
                     e->cell_start, e->block_sums, e->dpolar, e->dsys, e->sys_of, e->dpolar_sys, e->dsys1, e->dsamples,
@@ -2949,16 +2963,16 @@ int spp_set_state(spp_engine* e, const double* xw, const double* yw, const doubl
                 return fail(e, SPP_ERR_ARG, "particle %d has a non-finite heading or unwrapped position", i);
         }
     }
-    std::vector<P4> pos(n);
+    std::vector<double2> pos(n);
     std::vector<double2> U(n);
     std::vector<int> id(n);
     for (int i = 0; i < n; ++i) {
-        pos[i] = P4{xw[i], yw[i], theta[i], 0.0};
+        pos[i] = make_double2(xw[i], yw[i]);
         U[i] = make_double2(xu ? xu[i] : xw[i], yu ? yu[i] : yw[i]);
         id[i] = i;
     }
     SetPtrs& s = e->set[e->cur];
-    HIPC(e, hipMemcpyAsync(s.pos, pos.data(), n * sizeof(P4), hipMemcpyHostToDevice, e->stream));
+    HIPC(e, hipMemcpyAsync(s.xy, pos.data(), n * sizeof(double2), hipMemcpyHostToDevice, e->stream));
     HIPC(e, hipMemcpyAsync(s.th, theta, n * sizeof(double), hipMemcpyHostToDevice, e->stream));
     HIPC(e, hipMemcpyAsync(s.U, U.data(), n * sizeof(double2), hipMemcpyHostToDevice, e->stream));
     HIPC(e, hipMemcpyAsync(s.id, id.data(), n * sizeof(int), hipMemcpyHostToDevice, e->stream));
@@ -2973,12 +2987,12 @@ int spp_get_state(spp_engine* e, double* xw, double* yw, double* xu, double* yu,
 {
     TRY(check_ready(e));
     if (n != e->N) return fail(e, SPP_ERR_ARG, "n = %d, engine has %d particles", n, e->N);
-    std::vector<P4> pos(n);
+    std::vector<double2> pos(n);
     std::vector<double> th(n);
     std::vector<double2> U(n);
     std::vector<int> id(n);
     const SetPtrs& s = e->set[e->cur];
-    HIPC(e, hipMemcpyAsync(pos.data(), s.pos, n * sizeof(P4), hipMemcpyDeviceToHost, e->stream));
+    HIPC(e, hipMemcpyAsync(pos.data(), s.xy, n * sizeof(double2), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipMemcpyAsync(th.data(), s.th, n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipMemcpyAsync(U.data(), s.U, n * sizeof(double2), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipMemcpyAsync(id.data(), s.id, n * sizeof(int), hipMemcpyDeviceToHost, e->stream));
@@ -3057,7 +3071,8 @@ int spp_interact(spp_engine* e)
     a.sys_of = e->sys_of;
     a.abp = e->abp;
     a.inv = e->inv;
-    a.gather = e->set[e->cur].pos;
+    a.gxy = e->set[e->cur].xy;
+    a.gth = e->set[e->cur].th;
     a.id = e->set[e->cur].id;
     a.nbr32 = e->nbr32;
     a.ncount = e->ncount;
@@ -3074,7 +3089,8 @@ StageArgs pair_args(spp_engine* e)
     a.sys_of = e->sys_of;
     a.abp = e->abp;
     a.inv = e->inv;
-    a.gather = e->set[e->cur].pos;
+    a.gxy = e->set[e->cur].xy;
+    a.gth = e->set[e->cur].th;
     a.id = e->set[e->cur].id;
     return a;
 }
@@ -3240,7 +3256,7 @@ int spp_multi_step_sampled(spp_engine* e, int32_t steps, int32_t every, double* 
 int spp_polarization(spp_engine* e, double* p, double* sum_x, double* sum_y)
 {
     TRY(check_ready(e));
-    TRY(launch(e, "polar", k_polar, e->nblk, e->N, (const P4*)e->set[e->cur].pos, e->dpolar));
+    TRY(launch(e, "polar", k_polar, e->nblk, e->N, (const double*)e->set[e->cur].th, e->dpolar));
     std::vector<double2> part(e->nblk);
     HIPC(e, hipMemcpyAsync(part.data(), e->dpolar, e->nblk * sizeof(double2), hipMemcpyDeviceToHost, e->stream));
     HIPC(e, hipStreamSynchronize(e->stream));
@@ -3265,7 +3281,7 @@ int spp_batch_polarization(spp_engine* e, double* p, int32_t nsys)
         return spp_polarization(e, p, nullptr, nullptr);
     }
     if (nsys != e->nsys) return fail(e, SPP_ERR_ARG, "nsys = %d, engine has %d systems", nsys, e->nsys);
-    TRY(launch(e, "polar_sys", k_polar_sys, nsys, (const SysInfo*)e->dsys, (const P4*)e->set[e->cur].pos,
+    TRY(launch(e, "polar_sys", k_polar_sys, nsys, (const SysInfo*)e->dsys, (const double*)e->set[e->cur].th,
                e->dpolar_sys));
     std::vector<double2> v(nsys);
     HIPC(e, hipMemcpyAsync(v.data(), e->dpolar_sys, nsys * sizeof(double2), hipMemcpyDeviceToHost, e->stream));
@@ -3290,7 +3306,7 @@ int spp_pair_distribution(spp_engine* e, double r_max, int32_t nbins, int64_t* c
     const double dr = r_max / nbins;
     if (rc == SPP_OK) {
         hipLaunchKernelGGL(k_pair_hist, dim3(e->nblk), dim3(kBlock), nbins * sizeof(unsigned int), e->stream, e->N,
-                           (const P4*)e->set[e->cur].pos, (const int*)e->nbr32, (const int*)e->ncount, e->G,
+                           (const double2*)e->set[e->cur].xy, (const int*)e->nbr32, (const int*)e->ncount, e->G,
                            1.0 / dr, nbins, dh);
         if (hipGetLastError() != hipSuccess) rc = fail(e, SPP_ERR_HIP, "pair histogram launch failed");
     }
@@ -3328,7 +3344,7 @@ int spp_pair_distribution_noalloc(spp_engine* e, double r_max, int32_t nbins, in
     hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(nblocks(nbins)), dim3(kBlock), 0, e->stream, dh, 0ull, nbins);
     const double dr = r_max / nbins;
     hipLaunchKernelGGL(k_pair_hist, dim3(e->nblk), dim3(kBlock), nbins * sizeof(unsigned int), e->stream, e->N,
-                       (const P4*)e->set[e->cur].pos, (const int*)e->nbr32, (const int*)e->ncount, e->G, 1.0 / dr,
+                       (const double2*)e->set[e->cur].xy, (const int*)e->nbr32, (const int*)e->ncount, e->G, 1.0 / dr,
                        nbins, dh);
     HIPC(e, hipGetLastError());
     // the counts land in the caller's array (or, without one, in g, converted in place): 8 B per bin
@@ -3479,7 +3495,7 @@ int spp_snapshot(spp_engine* e, int32_t slot)
     // save-rebuild and restore-rebuild give the same order and list, and a replay is bitwise.
     TRY(rebuild_sync(e, false));
     Snapshot& s = e->slots[slot];
-    if (!s.s.pos) TRY(alloc_set(e, s.s));
+    if (!s.s.xy) TRY(alloc_set(e, s.s));
     TRY(copy_set(e, e->set[e->cur], s.s));
     s.step = e->step;
     s.t = e->t;
@@ -3617,13 +3633,13 @@ int spp_time_stage(spp_engine* e, int32_t stage, int32_t reps, double* avg_us)
     a.thresh2 = e->thresh * e->thresh;
     a.blkbad = e->slotb[0].blkbad;
     a.blockmax = e->slotb[0].blkmax;
-    a.gather = stage <= 1 ? S.pos : e->pos1;
-    a.th_in = stage <= 1 ? S.th : e->th1;
+    a.gxy = stage <= 1 ? S.xy : e->xy1;
+    a.gth = stage <= 1 ? S.th : e->th1;
     a.th0 = S.th;
     a.U_in = S.U;
     // every output goes to scratch: the engine state is left as it was
-    a.out = e->pos2;
-    a.th_out = e->th2;
+    a.oxy = e->xy2;
+    a.oth = e->th2;
     a.U_out = reinterpret_cast<double2*>(e->aux_alt[0]);
     a.dth = stage <= 1 ? e->dth_alt : e->dth;
     a.kacc = e->aux_alt[1];

@@ -25,7 +25,7 @@ struct Args {
     int nchunk;                  // Gram partial sums per replica
     const SysInfo* sys;          // per-system table (the box of replica r is sys[r*(K+1)].G)
     const int* inv;              // caller index -> slot
-    P4* pos;                     // current state set
+    double2* pos;                // current state set: wrapped positions
     double* th;
     double2* U;
     double* dev;                 // [R][K][3][n] deviation / direction components
@@ -71,11 +71,11 @@ __global__ __launch_bounds__(256) void k_deviation(Args a)
     int r = t / a.n, j = t - r * a.n;
     const Grid& G = a.sys[r * (a.K + 1)].G;
     int sb = a.inv[(r * (a.K + 1)) * a.n + j];
-    P4 b = a.pos[sb];
+    const double2 b = a.pos[sb];
     double tb = a.th[sb];
     for (int c = 1; c <= a.K; ++c) {
         int sc = a.inv[(r * (a.K + 1) + c) * a.n + j];
-        P4 q = a.pos[sc];
+        const double2 q = a.pos[sc];
         a.dev[dev_index(a, r, c - 1, 0, j)] = ref_wrap(q.x - b.x, G.Lx2);
         a.dev[dev_index(a, r, c - 1, 1, j)] = ref_wrap(q.y - b.y, G.Ly2);
         a.dev[dev_index(a, r, c - 1, 2, j)] = wrap_theta_diff(a.th[sc] - tb);
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(256) void k_apply(Args a)
     if (cf[K * K] == 0) return;
     const Grid& G = a.sys[r * (K + 1)].G;
     int sb = a.inv[(r * (K + 1)) * a.n + j];
-    P4 b = a.pos[sb];
+    const double2 b = a.pos[sb];
     double tb = a.th[sb];
     double2 ub = a.U[sb];
 #pragma unroll
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(256) void k_apply(Args a)
         double dx = sc * q[0][c], dy = sc * q[1][c], dt = sc * q[2][c];
         int s = a.inv[(r * (K + 1) + c + 1) * a.n + j];
         double th = tb + dt;
-        a.pos[s] = P4{ref_wrap(b.x + dx, G.Lx2), ref_wrap(b.y + dy, G.Ly2), th, 0.0};
+        a.pos[s] = make_double2(ref_wrap(b.x + dx, G.Lx2), ref_wrap(b.y + dy, G.Ly2));
         a.th[s] = th;
         a.U[s] = make_double2(ub.x + dx, ub.y + dy);
     }

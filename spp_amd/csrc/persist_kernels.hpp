@@ -44,8 +44,8 @@ struct Args {
     NlistFlatArgs nl;     // list arrays; positions set in-kernel
     const SysInfo* sys_tab;
     SetPtrs cur, spare;   // the engine's state set (in and out) and a set for the permutation
-    P4* pos1;
-    P4* pos2;
+    double2* xy1;
+    double2* xy2;
     double* th1;
     double* th2;
     int* key;
@@ -129,7 +129,7 @@ __device__ int rebuild_system(const Args& a, const SysInfo& S, const SetPtrs& fr
     const Grid& G = S.G;
     const int koff = S.key_off, nk = G.ncx * G.ncy;
     for (int i = off + t; i < end; i += kBlock1024) {  // k_bin
-        const P4 p = from.pos[i];
+        const double2 p = from.xy[i];
         const int cx = cell_coord(p.x, G.Lx, G.Lx2, G.ncx, G.quirk);
         const int cy = cell_coord(p.y, G.Ly, G.Ly2, G.ncy, G.quirk);
         const int k = koff + cy * G.ncx + cx;
@@ -166,7 +166,7 @@ __device__ int rebuild_system(const Args& a, const SysInfo& S, const SetPtrs& fr
     __syncthreads();
     for (int k = off + t; k < end; k += kBlock1024) {  // k_rank_permute, copying
         const int o = a.perm2[k];
-        to.pos[k] = from.pos[o];
+        to.xy[k] = from.xy[o];
         to.th[k] = from.th[o];
         to.U[k] = from.U[o];
         const int pid = from.id[o];
@@ -175,7 +175,7 @@ __device__ int rebuild_system(const Args& a, const SysInfo& S, const SetPtrs& fr
     }
     __syncthreads();
     NlistFlatArgs nl = a.nl;
-    nl.pos = to.pos;
+    nl.pos = to.xy;
     int mlen = 0;
     long long cnt = 0;
     for (int i = off + t; i < end; i += kBlock1024) {  // k_nlist_flat
@@ -203,20 +203,20 @@ __device__ int stage_system(const Args& a, const SysInfo& S, const Phys& P, cons
     sa.U_in = cur.U;
     sa.U_out = cur.U;
     if (INTEG == SPP_EULER) {
-        sa.gather = cur.pos;
-        sa.th_in = cur.th;
-        sa.out = a.pos1;
-        sa.th_out = a.th1;
+        sa.gxy = cur.xy;
+        sa.gth = cur.th;
+        sa.oxy = a.xy1;
+        sa.oth = a.th1;
     } else if (INTEG == SPP_RK2) {
-        sa.gather = STAGE == 1 ? cur.pos : a.pos1;
-        sa.th_in = STAGE == 1 ? cur.th : a.th1;
-        sa.out = STAGE == 1 ? a.pos1 : cur.pos;
-        sa.th_out = STAGE == 1 ? a.th1 : cur.th;
+        sa.gxy = STAGE == 1 ? cur.xy : a.xy1;
+        sa.gth = STAGE == 1 ? cur.th : a.th1;
+        sa.oxy = STAGE == 1 ? a.xy1 : cur.xy;
+        sa.oth = STAGE == 1 ? a.th1 : cur.th;
     } else {
-        sa.gather = STAGE == 1 ? cur.pos : (STAGE == 3 ? a.pos2 : a.pos1);
-        sa.th_in = STAGE == 1 ? cur.th : (STAGE == 3 ? a.th2 : a.th1);
-        sa.out = STAGE == 2 ? a.pos2 : (STAGE == 4 ? cur.pos : a.pos1);
-        sa.th_out = STAGE == 2 ? a.th2 : (STAGE == 4 ? cur.th : a.th1);
+        sa.gxy = STAGE == 1 ? cur.xy : (STAGE == 3 ? a.xy2 : a.xy1);
+        sa.gth = STAGE == 1 ? cur.th : (STAGE == 3 ? a.th2 : a.th1);
+        sa.oxy = STAGE == 2 ? a.xy2 : (STAGE == 4 ? cur.xy : a.xy1);
+        sa.oth = STAGE == 2 ? a.th2 : (STAGE == 4 ? cur.th : a.th1);
     }
     const int off = S.pid_off, end = S.pid_off + S.n;
     int bad = 0;
@@ -233,7 +233,7 @@ __device__ int stage_system(const Args& a, const SysInfo& S, const Phys& P, cons
     bad = wg_or(bad, sh);  // also the barrier before the next stage reads these outputs
     if (INTEG == SPP_EULER) {
         for (int i = off + threadIdx.x; i < end; i += kBlock1024) {
-            cur.pos[i] = a.pos1[i];
+            cur.xy[i] = a.xy1[i];
             cur.th[i] = a.th1[i];
         }
         __syncthreads();
@@ -312,7 +312,7 @@ __global__ __launch_bounds__(kBlock1024) void k_persist(Args a)
                 double vx = 0, vy = 0;
                 for (int i = S.pid_off + threadIdx.x; i < S.pid_off + S.n; i += kBlock) {
                     double c, sn;
-                    spp::heading_unit(cur.pos[i].z, c, sn);
+                    spp::heading_unit(cur.th[i], c, sn);
                     vx += c;
                     vy += sn;
                 }
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(kBlock1024) void k_persist(Args a)
     }
     if (moved) {  // the state ends in the engine's set
         for (int i = S.pid_off + threadIdx.x; i < S.pid_off + S.n; i += kBlock1024) {
-            a.cur.pos[i] = cur.pos[i];
+            a.cur.xy[i] = cur.xy[i];
             a.cur.th[i] = cur.th[i];
             a.cur.U[i] = cur.U[i];
             a.cur.id[i] = cur.id[i];
