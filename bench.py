@@ -46,13 +46,14 @@ def stage_bytes(kbar):
 
 def stage_impl_bytes(kbar):
     """Bytes the engine's fused RK2 stages actually move per particle (see stage_bytes for the
-    algorithmic minimum):
-      stage 1: read record x,y,theta,- (32) + list length (4) + list (4 kbar) + id (4, noise
-               counter) + build position (8, displacement check); write midpoint record (32,
-               theta_old in its fourth field) + noise (8)                             = 88 + 4 kbar
-      stage 2: read midpoint record (32) + length (4) + list (4 kbar) + noise (8) + x_u,y_u (16)
-               + build position (8); write record (32) + theta (8) + x_u,y_u (16)    = 124 + 4 kbar"""
-    return 88 + 4 * kbar, 124 + 4 * kbar
+    algorithmic minimum; a neighbour's record is counted once, as its owner's):
+      stage 1: read x,y (16) + theta (8) + list length (4) + list (4 kbar) + id (4, noise counter)
+               + build position (8, displacement check); write midpoint x,y (16) + theta (8)
+               + noise (8)                                                          = 72 + 4 kbar
+      stage 2: read midpoint x,y,theta (24) + length (4) + list (4 kbar) + noise (8) + theta_old
+               (8) + x_u,y_u (16) + build position (8); write x,y (16) + theta (8) + x_u,y_u (16)
+                                                                                    = 108 + 4 kbar"""
+    return 72 + 4 * kbar, 108 + 4 * kbar
 
 
 REBUILD_BYTES = 100.0  # algorithmic bytes per particle and rebuild (binning, sort, list)
@@ -132,6 +133,9 @@ def main():
     ap.add_argument("--variant", type=int, default=0, help="interaction-kernel variant (spp_params.kernel_variant)")
     ap.add_argument("--time-reps", type=int, default=50, help="launches per stage-kernel timing")
     ap.add_argument("--stage-samples", type=int, default=10, help="stage timings along the replayed timed region")
+    ap.add_argument("--calibration-copy", type=int, default=0,
+                    help="profiling runs: also snapshot an engine of this many particles twice (a plain copy of "
+                         "44 B per particle read and written, far larger than the L2) to calibrate byte counters")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -177,10 +181,19 @@ def main():
     # mean steps between rebuilds over the timed and the profiled runs together
     nreb = (info1.rebuilds - info0.rebuilds) + (info2.rebuilds - info1.rebuilds)
     interval = (args.steps + args.profile_steps) / max(1, nreb)
-    # A plain copy of known size (the snapshot copy: 60 B per particle read and written) -- the
+    # A plain copy of known size (the snapshot copy: 44 B per particle read and written) -- the
     # calibration kernel for hardware byte counters in profiling runs.
     e.snapshot(99)
     e.synchronize()
+    if args.calibration_copy > 0:
+        nc = args.calibration_copy
+        Lc = 0.5 * float(np.sqrt(nc))
+        ec = S.Engine(workload_params(S, nc, Lc, args.noise))
+        ec.set_state(*S.random_state(11, nc, Lc, Lc))
+        ec.snapshot(1)
+        ec.snapshot(2)
+        ec.synchronize()
+        del ec
     # Dominant kernel, measured along the timed region's trajectory: the region is replayed from its
     # starting state and, every steps/K steps, each fused RK2 stage is timed over back-to-back
     # launches between two HIP events on the engine's stream (outputs to scratch, state untouched).

@@ -41,9 +41,11 @@ HBM_PEAK_GBS = 8000.0
 
 
 def stage1_bytes(kbar, rk4=False):
-    """Stage 1 of the fused RK2 step: read x, y, theta, list length, list, id, build position;
-    write the midpoint state and the noise.  RK4's stage 1 also writes the 32-byte accumulator."""
-    return 72.0 + 4.0 * kbar + (32.0 if rk4 else 0.0)
+    """Algorithmic bytes per particle of the fused stage 1 (the minimum it must move, noise drawn in
+    registers, a neighbour's record counted once, as its owner's): read x, y, theta (24) and the
+    list (4 kbar), write the midpoint x, y, theta (24); RK4's stage 1 also writes the accumulator
+    (fx, fy, tau: 24)."""
+    return 48.0 + 4.0 * kbar + (24.0 if rk4 else 0.0)
 
 
 def timed(e, n, steps, chunk, warm):
