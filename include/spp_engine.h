@@ -191,6 +191,10 @@ typedef struct spp_info {
     double mean_neighbors;  /* mean full neighbour-list length at the last rebuild                 */
     int64_t tile_rebuilds;  /* rebuilds sorted by the one-launch tile path (SPP_REBUILD_SORT_AUTO)  */
     int64_t tile_fallbacks; /* tile sorts that failed and were redone by the general sort           */
+    double device_ms;       /* device time of the engine's stepping and measuring calls so far
+                               (spp_set_state, spp_multi_step, spp_multi_step_sampled,
+                               spp_batch_polarization, spp_lyapunov_*): HIP events on the engine's
+                               stream around each outermost call, summed                       */
 } spp_info;
 
 /* ---- lifetime ----------------------------------------------------------------------------------- */

@@ -110,6 +110,7 @@ class Info(C.Structure):
         ("mean_neighbors", C.c_double),
         ("tile_rebuilds", C.c_int64),
         ("tile_fallbacks", C.c_int64),
+        ("device_ms", C.c_double),
     ]
 
 

@@ -1588,6 +1588,9 @@ struct spp_engine {
     int64_t samples_alloc = 0, sample_cap = 0, sample_base = 0;
     int sample_every = 0;
     hipEvent_t slot_event[2] = {nullptr, nullptr};
+    hipEvent_t dev_t0 = nullptr, dev_t1 = nullptr;  // device time of the outermost stepping call
+    int dev_depth = 0;
+    double device_ms = 0;
     std::map<int, Snapshot> slots;
     LyapState ly;
 
@@ -2834,6 +2837,7 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
     if (rc == SPP_OK && hipHostGetDevicePointer((void**)&e->drep, e->hrep, 0) != hipSuccess) rc = SPP_ERR_HIP;
     for (int s = 0; s < 2 && rc == SPP_OK; ++s)
         if (hipEventCreateWithFlags(&e->slot_event[s], hipEventDisableTiming) != hipSuccess) rc = SPP_ERR_HIP;
+    if (hipEventCreate(&e->dev_t0) != hipSuccess || hipEventCreate(&e->dev_t1) != hipSuccess) rc = SPP_ERR_HIP;
     if (rc == SPP_OK && systems) {
         // systems occupy fixed slot ranges: cell keys are system-major, so every sort keeps them
         std::vector<int> so(N);
@@ -2916,6 +2920,8 @@ void spp_destroy(spp_engine* e)
     for (auto ev : e->event_pool) (void)hipEventDestroy(ev);
     for (auto ev : e->slot_event)
         if (ev) (void)hipEventDestroy(ev);
+    for (auto ev : {e->dev_t0, e->dev_t1})
+        if (ev) (void)hipEventDestroy(ev);
     if (e->hrep) (void)hipHostFree(e->hrep);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
@@ -2925,6 +2931,25 @@ const char* spp_last_error(const spp_engine* e)
 {
     return e ? e->err.c_str() : g_create_error.c_str();
 }
+
+// Device time of a public call that steps or measures: HIP events on the engine's stream around
+// the outermost such call, accumulated into spp_info.device_ms (the calls end synchronised).
+struct DeviceClock {
+    spp_engine* e;
+    bool on;
+    explicit DeviceClock(spp_engine* eng) : e(eng), on(eng && eng->dev_t0)
+    {
+        if (on && e->dev_depth++ == 0) (void)hipEventRecord(e->dev_t0, e->stream);
+    }
+    ~DeviceClock()
+    {
+        if (!on || --e->dev_depth != 0) return;
+        float ms = 0;
+        if (hipEventRecord(e->dev_t1, e->stream) == hipSuccess && hipEventSynchronize(e->dev_t1) == hipSuccess &&
+            hipEventElapsedTime(&ms, e->dev_t0, e->dev_t1) == hipSuccess)
+            e->device_ms += ms;
+    }
+};
 
 int spp_get_info(spp_engine* e, spp_info* info)
 {
@@ -2941,6 +2966,7 @@ int spp_get_info(spp_engine* e, spp_info* info)
     info->mean_neighbors = e->mean_nbrs;
     info->tile_rebuilds = e->tile_rebuilds;
     info->tile_fallbacks = e->tile_fallbacks;
+    info->device_ms = e->device_ms;
     return SPP_OK;
 }
 
@@ -2948,6 +2974,7 @@ int spp_set_state(spp_engine* e, const double* xw, const double* yw, const doubl
                   const double* yu, const double* theta, int32_t n)
 {
     if (!e || !xw || !yw || !theta) return e ? fail(e, SPP_ERR_ARG, "null array") : SPP_ERR_ARG;
+    DeviceClock dc(e);
     if (n != e->N) return fail(e, SPP_ERR_ARG, "n = %d, engine has %d particles", n, e->N);
     // Wrapped positions must lie in their box [-L, L] (the cell binning clamps, so a particle
     // outside would be binned into a wrong cell and miss neighbours); no value may be NaN or inf.
@@ -3202,6 +3229,7 @@ int spp_move_stage(spp_engine* e, int32_t stage)
 int spp_multi_step(spp_engine* e, int32_t steps)
 {
     TRY(check_ready(e));
+    DeviceClock dc(e);
     if (steps < 0) return fail(e, SPP_ERR_ARG, "steps < 0");
     if (e->unfused_stage != 0) return fail(e, SPP_ERR_STATE, "a split step is in progress");
     if (e->p.noise_mode == SPP_NOISE_INJECTED && (steps > 1 || (steps == 1 && !e->inj_pending)))
@@ -3216,6 +3244,7 @@ int spp_one_step(spp_engine* e) { return spp_multi_step(e, 1); }
 int spp_multi_step_sampled(spp_engine* e, int32_t steps, int32_t every, double* polar, int64_t len)
 {
     TRY(check_ready(e));
+    DeviceClock dc(e);
     if (every <= 0 || steps < 0 || steps % every) return fail(e, SPP_ERR_ARG, "steps must be a multiple of every > 0");
     const int ns = e->dsys ? e->nsys : 1;
     const int64_t nsamp = steps / every;
@@ -3275,6 +3304,7 @@ int spp_polarization(spp_engine* e, double* p, double* sum_x, double* sum_y)
 int spp_batch_polarization(spp_engine* e, double* p, int32_t nsys)
 {
     TRY(check_ready(e));
+    DeviceClock dc(e);
     if (!p) return fail(e, SPP_ERR_ARG, "null array");
     if (!e->dsys) {
         if (nsys != 1) return fail(e, SPP_ERR_ARG, "single-system engine: nsys must be 1");
@@ -3366,6 +3396,7 @@ int spp_lyapunov_init(spp_engine* e, int32_t replicas, int32_t vectors, double e
                       uint64_t seed)
 {
     TRY(check_ready(e));
+    DeviceClock dc(e);
     if (!e->dsys) return fail(e, SPP_ERR_STATE, "Lyapunov clones need a batched engine (spp_create_batch)");
     if (e->unfused_stage != 0) return fail(e, SPP_ERR_STATE, "a split step is in progress");
     if (replicas <= 0 || vectors < 1 || vectors > lyap::kMaxK)
@@ -3430,6 +3461,7 @@ int spp_lyapunov_init(spp_engine* e, int32_t replicas, int32_t vectors, double e
 int spp_lyapunov_renormalize(spp_engine* e, double* log_growth)
 {
     TRY(check_ready(e));
+    DeviceClock dc(e);
     LyapState& L = e->ly;
     if (!L.on) return fail(e, SPP_ERR_STATE, "call spp_lyapunov_init first");
     if (e->unfused_stage != 0) return fail(e, SPP_ERR_STATE, "a split step is in progress");
@@ -3451,6 +3483,7 @@ int spp_lyapunov_renormalize(spp_engine* e, double* log_growth)
 int spp_lyapunov_run(spp_engine* e, int32_t steps, int32_t interval)
 {
     TRY(check_ready(e));
+    DeviceClock dc(e);
     if (!e->ly.on) return fail(e, SPP_ERR_STATE, "call spp_lyapunov_init first");
     if (interval <= 0 || steps < 0 || steps % interval)
         return fail(e, SPP_ERR_ARG, "steps (%d) must be a non-negative multiple of interval (%d)", steps, interval);

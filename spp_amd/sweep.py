@@ -10,11 +10,13 @@ polarization-analysis.py:29-62, restated in observables.polarity_error):
 * ``batched_curve`` -- the same points as systems of one batched engine (one kernel launch advances
                        every point);
 * ``steered_curve`` -- the batched engine on points placed by the surrogate network: the network,
-                       trained on simulator output (``make_dataset``), predicts the curve, and the
-                       points are spread evenly along its predicted arc length, so that they crowd
-                       where P changes fast.  Every reported value is a simulated one; where
-                       neighbouring simulated values still differ by more than the resolution, the
-                       sweep adds simulated midpoints (the plain fallback).
+                       trained on simulator output (``make_dataset``), predicts the curve; a grid
+                       is refined where the predicted values jump, so the refinement a uniform
+                       grid needs extra sequential rounds for is simulated at once, and points of
+                       high predicted variance run as several replicas side by side.  Every
+                       reported value is a simulated one; where neighbouring simulated values
+                       still differ by more than the resolution, the sweep adds simulated
+                       midpoints (the plain fallback).
 
 The curve counts as resolved when every point's error bar is at most ``tol`` and neighbouring
 points differ by at most ``gap`` (default 3 tol: two values each within tol of the truth can differ
@@ -56,13 +58,20 @@ def features(D, rho, v0, n):
 
 @dataclass
 class Timer:
+    """Cost of engine calls: wall-clock seconds, device seconds (the engine's own HIP-event
+    accounting of its stepping calls, spp_info.device_ms) and particle-steps."""
     seconds: float = 0.0
+    device_seconds: float = 0.0
     particle_steps: float = 0.0
 
     def run(self, fn, *a, particles=0, steps=0):
+        eng = getattr(fn, "__self__", None)
+        d0 = eng.info().device_ms if isinstance(eng, Engine) else None
         t0 = time.perf_counter()
         r = fn(*a)
         self.seconds += time.perf_counter() - t0
+        if d0 is not None:
+            self.device_seconds += (eng.info().device_ms - d0) * 1e-3
         self.particle_steps += particles * steps
         return r
 
@@ -102,7 +111,7 @@ def run_points(points, steps, sample_every, seed, timer=None, dt=0.01, state=Non
             xs.append(x)
             ys.append(y)
             ts.append(t)
-    e.set_state(np.concatenate(xs), np.concatenate(ys), np.concatenate(ts))
+    timer.run(e.set_state, np.concatenate(xs), np.concatenate(ys), np.concatenate(ts))
     ntot = int(sum(n for *_, n in points))
     steps -= steps % sample_every
     series = timer.run(e.multi_step_sampled, steps, sample_every, particles=ntot, steps=steps)
@@ -138,7 +147,7 @@ def run_lyapunov(points, warm, steps, seed, sample_every=10, interval=10, eps=1e
             ys.append(y)
             ts.append(t)
     e = Engine(p, systems)
-    e.set_state(np.concatenate(xs), np.concatenate(ys), np.concatenate(ts))
+    timer.run(e.set_state, np.concatenate(xs), np.concatenate(ys), np.concatenate(ts))
     ntot = 2 * n * len(points)
     timer.run(e.multi_step, warm, particles=ntot, steps=warm)
     e.lyapunov_init(len(points), 1, eps, True, seed)
@@ -280,11 +289,12 @@ class Curve:
     D: np.ndarray
     P: np.ndarray
     err: np.ndarray
-    seconds: float
+    seconds: float                   # wall clock of the engine calls
     particle_steps: float
     points_simulated: int
     rounds: int = 1
     log: list = field(default_factory=list)
+    device_seconds: float = 0.0      # device time of the engine calls
 
     def resolved(self, tol, gap, dD_min):
         o = np.argsort(self.D)
@@ -347,7 +357,8 @@ def plain_curve(D_grid, rho, v0, n, tol=0.02, chunk=2000, sample_every=10, max_s
         m, e = _converge([(D, rho, v0, n)], tol, chunk, sample_every, max_steps, seed + 11 * k, timer)
         P.append(m[0])
         E.append(e[0])
-    return Curve(np.array(D_grid), np.array(P), np.array(E), timer.seconds, timer.particle_steps, len(D_grid))
+    return Curve(np.array(D_grid), np.array(P), np.array(E), timer.seconds, timer.particle_steps, len(D_grid),
+                 device_seconds=timer.device_seconds)
 
 
 def _refine(D, P, E, points_fn, tol, gap, dD_min, chunk, sample_every, max_steps, seed, timer, rounds, log,
@@ -383,17 +394,22 @@ def batched_curve(D_grid, rho, v0, n, tol=0.02, gap=None, dD_min=None, chunk=200
     log = [("grid", len(D))]
     D, P, E, rounds = _refine(D, m, e, pts, tol, gap, dD_min, chunk, sample_every, max_steps, seed, timer, 1, log,
                               max_rounds)
-    return Curve(D, P, E, timer.seconds, timer.particle_steps, len(D), rounds, log)
+    return Curve(D, P, E, timer.seconds, timer.particle_steps, len(D), rounds, log, timer.device_seconds)
 
 
-def steered_points(s: Surrogate, D_lo, D_hi, rho, v0, n, count, tol, fine=512):
-    """`count` noise values spread evenly along the predicted curve's arc length in the metric
-    (dD / (D_hi - D_lo), dP / tol): points crowd where the predicted P changes fast."""
-    Dg = np.linspace(D_lo, D_hi, fine)
-    Pm, _, _ = predicted_curve(s, Dg, rho, v0, n)
-    ds = np.hypot(np.diff(Dg) / (D_hi - D_lo) * count, np.diff(Pm) / tol)
-    arc = np.concatenate([[0.0], np.cumsum(ds)])
-    return np.interp(np.linspace(0, arc[-1], count), arc, Dg)
+def steered_points(s: Surrogate, D_lo, D_hi, rho, v0, n, count, gap, depth=3):
+    """A uniform grid of `count` noise values plus the midpoints the predicted curve says the
+    simulated refinement would add: wherever neighbouring predicted values differ by more than
+    `gap`, the midpoint is added, up to `depth` times.  The refinement that a uniform grid needs a
+    second (and third) sequential round for is simulated in the first round."""
+    D = np.linspace(D_lo, D_hi, count)
+    for _ in range(depth):
+        P, _, _ = predicted_curve(s, D, rho, v0, n)
+        bad = np.nonzero(np.abs(np.diff(P)) > gap)[0]
+        if bad.size == 0:
+            break
+        D = np.sort(np.concatenate([D, 0.5 * (D[bad] + D[bad + 1])]))
+    return D
 
 
 def replicas_for(s: Surrogate, D, rho, v0, n, tol, max_reps=16, per_rep=8.0):
@@ -404,23 +420,29 @@ def replicas_for(s: Surrogate, D, rho, v0, n, tol, max_reps=16, per_rep=8.0):
     return np.clip(np.ceil(var / (per_rep * tol * tol)), 1, max_reps).astype(int)
 
 
+def grid_size(s: Surrogate, D_lo, D_hi, rho, v0, n, gap):
+    """Grid points for the predicted total variation of P plus the end points."""
+    Dg = np.linspace(D_lo, D_hi, 512)
+    Pm, _, _ = predicted_curve(s, Dg, rho, v0, n)
+    return int(np.ceil(np.abs(np.diff(Pm)).sum() / gap)) + 3
+
+
 def steered_curve(s: Surrogate, D_lo, D_hi, rho, v0, n, count=None, tol=0.02, gap=None, dD_min=None, chunk=2000,
                   sample_every=10, max_steps=200000, seed=1, max_rounds=6, max_reps=16):
-    """Surrogate-placed points in one batched engine, each with as many replicas as its predicted
-    variance asks for, then simulated refinement where needed."""
+    """Surrogate-steered points (steered_points: a grid of `count` plus its predicted refinement) in
+    one batched engine, each with as many replicas as its predicted variance asks for, then
+    simulated refinement where still needed."""
     gap = 3 * tol if gap is None else gap
     dD_min = (D_hi - D_lo) / 64 if dD_min is None else dD_min
     timer = Timer()
-    Dg = np.linspace(D_lo, D_hi, 512)
-    Pm, _, _ = predicted_curve(s, Dg, rho, v0, n)
-    if count is None:  # enough points for the predicted total variation plus the end points
-        count = int(np.ceil(np.abs(np.diff(Pm)).sum() / gap)) + 3
+    if count is None:
+        count = grid_size(s, D_lo, D_hi, rho, v0, n, gap)
     D = steered_points(s, D_lo, D_hi, rho, v0, n, count, gap)
     pts = lambda Ds: [(float(x), rho, v0, n) for x in Ds]
     reps_fn = lambda Ds: replicas_for(s, Ds, rho, v0, n, tol, max_reps)
     reps = reps_fn(D)
     m, e = _converge(pts(D), tol, chunk, sample_every, max_steps, seed, timer, reps=reps)
-    log = [("steered", len(D), int(reps.sum()))]
+    log = [("steered", count, len(D), int(reps.sum()))]
     D, P, E, rounds = _refine(D, m, e, pts, tol, gap, dD_min, chunk, sample_every, max_steps, seed, timer, 1, log,
                               max_rounds, reps_fn)
-    return Curve(D, P, E, timer.seconds, timer.particle_steps, len(D), rounds, log)
+    return Curve(D, P, E, timer.seconds, timer.particle_steps, len(D), rounds, log, timer.device_seconds)
