@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cmath>
 #include <cstdarg>
@@ -65,6 +66,7 @@ struct Report {
     double maxdisp2;  // largest squared displacement at the interval's end
     int sort_fail;    // the interval's rebuild took the tile path and could not sort (redo it)
     int pad;
+    long long seq;    // the slot's report count, written last: the host polls it for completion
 };
 
 struct SlotBufs {
@@ -371,7 +373,10 @@ __global__ __launch_bounds__(kBlock) void k_rank_permute(PermuteArgs a, const in
 // ------------------------------------------------------------------------------------------------
 constexpr int kSortCap = 2048;     // particles one tile can hold (LDS)
 constexpr int kSortMaxW = 1024;    // cells per tile (LDS histogram; kBlock * kScanItems)
-constexpr int kSortTarget = 768;   // particles per tile the host aims for
+#ifndef SPP_SORT_TARGET
+#define SPP_SORT_TARGET 384
+#endif
+constexpr int kSortTarget = SPP_SORT_TARGET;  // particles per tile the host aims for
 constexpr int kSpinLimit = 1 << 20;
 
 struct SortArgs {
@@ -1201,8 +1206,12 @@ __global__ __launch_bounds__(kBlock) void k_pair_hist(int N, const double2* __re
 }
 
 // Reduces one interval's per-block partials into the host-mapped report (one block) and clears
-// the per-block bad steps for the slot's next interval.
-__global__ __launch_bounds__(kBlock) void k_report(int nblk, SlotBufs b, int with_list, int* sort_fail, Report* out)
+// the per-block bad steps for the slot's next interval.  The report's fields reach host memory
+// before its sequence number (the slot's count of reports, kept on the device so that replays of a
+// captured interval count too), which the host polls instead of waiting on an event: an event
+// record between two intervals idles the GPU for about 5 us on this device.
+__global__ __launch_bounds__(kBlock) void k_report(int nblk, SlotBufs b, int with_list, int* sort_fail, long long* count,
+                                                  Report* out)
 {
     __shared__ int r_bad[kBlock / 64], r_len[kBlock / 64];
     __shared__ long long r_nnz[kBlock / 64];
@@ -1252,6 +1261,10 @@ __global__ __launch_bounds__(kBlock) void k_report(int nblk, SlotBufs b, int wit
         out->maxdisp2 = mx;
         out->sort_fail = *sort_fail;
         *sort_fail = 0;  // the next interval's rebuild starts clear
+        const long long seq = *count + 1;
+        *count = seq;
+        __threadfence_system();
+        *(volatile long long*)&out->seq = seq;
         __threadfence_system();
     }
 }
@@ -1512,7 +1525,7 @@ struct Interval {
     int64_t since0;  // steps since the last rebuild at the interval's start (after its rebuild)
     double t0;
     bool rebuilt;
-    hipEvent_t done;
+    long long seq;   // the report count of its slot once the interval's report has been written
 };
 
 // Tangent-vector state of a Lyapunov engine (device arrays described in lyapunov_kernels.hpp).
@@ -1576,6 +1589,8 @@ struct spp_engine {
     SlotBufs slotb[2] = {};      // per-block partials of the (at most two) intervals in flight
     Report* hrep = nullptr;      // host-mapped reports, 2 slots
     Report* drep = nullptr;      // device view of hrep
+    long long* rep_count = nullptr;  // device: reports written per slot
+    long long rep_issued[2] = {0, 0};  // host: reports enqueued per slot (launched or replayed)
     double2* dpolar = nullptr;
     // batched engine: per-system constants and the (fixed) slot -> system map; null for one box
     int nsys = 1;
@@ -1587,7 +1602,6 @@ struct spp_engine {
     double2* dsamples = nullptr; // heading sums recorded during spp_multi_step_sampled
     int64_t samples_alloc = 0, sample_cap = 0, sample_base = 0;
     int sample_every = 0;
-    hipEvent_t slot_event[2] = {nullptr, nullptr};
     hipEvent_t dev_t0 = nullptr, dev_t1 = nullptr;  // device time of the outermost stepping call
     int dev_depth = 0;
     double device_ms = 0;
@@ -1788,7 +1802,34 @@ int launch_nlist(spp_engine* e, int slot)
 
 int launch_report(spp_engine* e, int slot, bool with_list)
 {
-    return launch(e, "report", k_report, 1, e->nstat, e->slotb[slot], with_list ? 1 : 0, e->sort_fail, e->drep + slot);
+    // counted also in dry mode: a dry interval is replayed from a captured graph, which runs it
+    e->rep_issued[slot]++;
+    return launch(e, "report", k_report, 1, e->nstat, e->slotb[slot], with_list ? 1 : 0, e->sort_fail,
+                  e->rep_count + slot, e->drep + slot);
+}
+
+// Waits until the slot's report number `want` has reached host memory.  The host spins on the
+// report's sequence number; every ~1024 spins it asks the stream whether it failed or ran dry
+// without writing the report (an error either way), so the wait always ends.
+int wait_report(spp_engine* e, int slot, long long want)
+{
+    volatile long long* seq = &e->hrep[slot].seq;
+    for (long spins = 1;; ++spins) {
+        if (*seq >= want) break;
+        if ((spins & 1023) == 0) {
+            const hipError_t q = hipStreamQuery(e->stream);
+            if (q != hipSuccess && q != hipErrorNotReady)
+                return fail(e, SPP_ERR_HIP, "interval: %s", hipGetErrorString(q));
+            if (q == hipSuccess) {
+                if (*seq >= want) break;
+                return fail(e, SPP_ERR_HIP, "interval report %lld of slot %d missing after the stream drained",
+                            want, slot);
+            }
+        }
+        __builtin_ia32_pause();
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    return SPP_OK;
 }
 
 // tile: the caller reads the rebuild's failure flag from the interval report (only the interval
@@ -2066,11 +2107,13 @@ int enqueue_interval(spp_engine* e, int m, bool rebuild, int protect, Interval& 
         if (!g) {
             // capture: the launches are recorded, not run; the bookkeeping advances as for a launch
             hipGraph_t graph = nullptr;
+            const long long issued0 = e->rep_issued[slot];  // a capture that fails runs no report
             HIPC(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
             e->graph_mode = true;
             int rc = interval_body(e, m, rebuild, protect, slot, iv);
             e->graph_mode = false;
             hipError_t ce = hipStreamEndCapture(e->stream, &graph);
+            if (rc != SPP_OK || ce != hipSuccess) e->rep_issued[slot] = issued0;
             if (rc != SPP_OK) {
                 if (graph) (void)hipGraphDestroy(graph);
                 return rc;
@@ -2079,7 +2122,10 @@ int enqueue_interval(spp_engine* e, int m, bool rebuild, int protect, Interval& 
             hipGraphExec_t exec = nullptr;
             hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
-            if (ie != hipSuccess) return fail(e, SPP_ERR_HIP, "interval graph: %s", hipGetErrorString(ie));
+            if (ie != hipSuccess) {
+                e->rep_issued[slot] = issued0;
+                return fail(e, SPP_ERR_HIP, "interval graph: %s", hipGetErrorString(ie));
+            }
             e->graphs.push_back(spp_engine::GraphEntry{cur0, protect, slot, m, e->alloc_gen, exec});
             g = &e->graphs.back();
             e->graph_captures++;
@@ -2094,8 +2140,7 @@ int enqueue_interval(spp_engine* e, int m, bool rebuild, int protect, Interval& 
         HIPC(e, hipGraphLaunch(g->exec, e->stream));
         e->graph_replays++;
     }
-    HIPC(e, hipEventRecord(e->slot_event[slot], e->stream));
-    iv.done = e->slot_event[slot];
+    iv.seq = e->rep_issued[slot];
     return SPP_OK;
 }
 
@@ -2171,7 +2216,7 @@ int run_intervals(spp_engine* e, int steps)
             continue;
         }
         Interval iv = q.front();
-        HIPC(e, hipEventSynchronize(iv.done));
+        TRY(wait_report(e, iv.slot, iv.seq));
         e->host_syncs++;
         const Report f = e->hrep[iv.slot];
         const bool sort_fail = iv.rebuilt && f.sort_fail != 0;
@@ -2832,11 +2877,14 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
             if (rc == SPP_OK && hipMemcpy(e->dsys1, &S, sizeof S, hipMemcpyHostToDevice) != hipSuccess) rc = SPP_ERR_HIP;
         }
     }
-    if (rc == SPP_OK && hipHostMalloc((void**)&e->hrep, 2 * sizeof(Report), hipHostMallocMapped) != hipSuccess)
+    // coherent host memory: the host polls the reports while kernels write them
+    if (rc == SPP_OK &&
+        hipHostMalloc((void**)&e->hrep, 2 * sizeof(Report), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
         rc = SPP_ERR_HIP;
+    if (rc == SPP_OK) std::memset((void*)e->hrep, 0, 2 * sizeof(Report));
     if (rc == SPP_OK && hipHostGetDevicePointer((void**)&e->drep, e->hrep, 0) != hipSuccess) rc = SPP_ERR_HIP;
-    for (int s = 0; s < 2 && rc == SPP_OK; ++s)
-        if (hipEventCreateWithFlags(&e->slot_event[s], hipEventDisableTiming) != hipSuccess) rc = SPP_ERR_HIP;
+    A(dalloc(e, &e->rep_count, 2));
+    if (rc == SPP_OK && hipMemset(e->rep_count, 0, 2 * sizeof(long long)) != hipSuccess) rc = SPP_ERR_HIP;
     if (hipEventCreate(&e->dev_t0) != hipSuccess || hipEventCreate(&e->dev_t1) != hipSuccess) rc = SPP_ERR_HIP;
     if (rc == SPP_OK && systems) {
         // systems occupy fixed slot ranges: cell keys are system-major, so every sort keeps them
@@ -2901,7 +2949,7 @@ void spp_destroy(spp_engine* e)
                     e->inj, e->kacc, e->force, e->old, e->aux_alt[0], e->aux_alt[1], e->aux_alt[2],
                     e->nbr32, e->ncount, e->cellxy, e->build, e->key, e->perm, e->perm2, e->inv, e->cell_count,
                     e->cell_start, e->block_sums, e->dpolar, e->dsys, e->sys_of, e->dpolar_sys, e->dsys1, e->dsamples,
-                    e->pstatus, e->cell_start_alt, e->tile_state, e->sort_fail};
+                    e->pstatus, e->cell_start_alt, e->tile_state, e->sort_fail, e->rep_count};
     for (void* b : bufs) dfree(b);
     for (auto& b : e->slotb) {
         dfree(b.blkbad);
@@ -2918,8 +2966,6 @@ void spp_destroy(spp_engine* e)
         (void)hipEventDestroy(r.b);
     }
     for (auto ev : e->event_pool) (void)hipEventDestroy(ev);
-    for (auto ev : e->slot_event)
-        if (ev) (void)hipEventDestroy(ev);
     for (auto ev : {e->dev_t0, e->dev_t1})
         if (ev) (void)hipEventDestroy(ev);
     if (e->hrep) (void)hipHostFree(e->hrep);
