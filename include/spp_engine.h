@@ -157,7 +157,10 @@ typedef struct spp_params {
        (particle.h:983-1011); swimmers move with the mobilities of Set_nb(nb_swimmer) (:856-876), ring
        beads with mob_par / mob_perp / mob_rot.                                                    */
     int32_t nb_swimmer;
-    int32_t reserved_i[3];
+    int32_t device_timing;  /* 1: HIP events on the engine's stream around each outermost stepping or
+                               measuring call, summed into spp_info.device_ms; 0 (default): none, and
+                               device_ms stays 0 (an event record idles the GPU ~5 us per call)    */
+    int32_t reserved_i[2];
     double bead_sep;        /* bead separation (reference ActiveBrownianChain::separation)          */
     double sigma_bead;      /* bead repulsion sigma (sigma_p)                                        */
     double rc_bead;         /* bead repulsion cutoff (repulsion_radius)                              */
@@ -191,10 +194,10 @@ typedef struct spp_info {
     double mean_neighbors;  /* mean full neighbour-list length at the last rebuild                 */
     int64_t tile_rebuilds;  /* rebuilds sorted by the one-launch tile path (SPP_REBUILD_SORT_AUTO)  */
     int64_t tile_fallbacks; /* tile sorts that failed and were redone by the general sort           */
-    double device_ms;       /* device time of the engine's stepping and measuring calls so far
-                               (spp_set_state, spp_multi_step, spp_multi_step_sampled,
-                               spp_batch_polarization, spp_lyapunov_*): HIP events on the engine's
-                               stream around each outermost call, summed                       */
+    double device_ms;       /* with spp_params.device_timing: device time of the engine's stepping
+                               and measuring calls so far (spp_set_state, spp_multi_step,
+                               spp_multi_step_sampled, spp_batch_polarization, spp_lyapunov_*): HIP
+                               events on the engine's stream around each outermost call, summed  */
 } spp_info;
 
 /* ---- lifetime ----------------------------------------------------------------------------------- */

@@ -73,7 +73,8 @@ class Params(C.Structure):
         ("graph_intervals", C.c_int32),
         ("rebuild_sort", C.c_int32),
         ("nb_swimmer", C.c_int32),
-        ("reserved_i", C.c_int32 * 3),
+        ("device_timing", C.c_int32),
+        ("reserved_i", C.c_int32 * 2),
         ("bead_sep", C.c_double),
         ("sigma_bead", C.c_double),
         ("rc_bead", C.c_double),

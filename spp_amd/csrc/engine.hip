@@ -2885,7 +2885,9 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
     if (rc == SPP_OK && hipHostGetDevicePointer((void**)&e->drep, e->hrep, 0) != hipSuccess) rc = SPP_ERR_HIP;
     A(dalloc(e, &e->rep_count, 2));
     if (rc == SPP_OK && hipMemset(e->rep_count, 0, 2 * sizeof(long long)) != hipSuccess) rc = SPP_ERR_HIP;
-    if (hipEventCreate(&e->dev_t0) != hipSuccess || hipEventCreate(&e->dev_t1) != hipSuccess) rc = SPP_ERR_HIP;
+    if (p->device_timing &&
+        (hipEventCreate(&e->dev_t0) != hipSuccess || hipEventCreate(&e->dev_t1) != hipSuccess))
+        rc = SPP_ERR_HIP;
     if (rc == SPP_OK && systems) {
         // systems occupy fixed slot ranges: cell keys are system-major, so every sort keeps them
         std::vector<int> so(N);
@@ -2979,7 +2981,9 @@ const char* spp_last_error(const spp_engine* e)
 }
 
 // Device time of a public call that steps or measures: HIP events on the engine's stream around
-// the outermost such call, accumulated into spp_info.device_ms (the calls end synchronised).
+// the outermost such call, accumulated into spp_info.device_ms (the calls end synchronised).  Only
+// engines created with spp_params.device_timing hold the events: each record idles the GPU for a
+// few microseconds, which a short stepping call would otherwise pay twice.
 struct DeviceClock {
     spp_engine* e;
     bool on;

@@ -43,7 +43,7 @@ def model_params(dt=0.01):
     """Continuous-time Vicsek: unit interaction radius, g = 1, RK2, Verlet radius 1.3."""
     p = default_params(MODEL_CONTINUOUS)
     return p.update(dt=dt, R_align=1.0, g=1.0, alpha=0.0, integrator=RK2, r_list=1.3,
-                    rebuild=REBUILD_DISPLACEMENT)
+                    rebuild=REBUILD_DISPLACEMENT, device_timing=1)
 
 
 def half_width(n, rho):
