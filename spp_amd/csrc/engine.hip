@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <atomic>
 #include <climits>
+#include <limits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -689,11 +690,23 @@ __device__ __forceinline__ int64_t noise_step(const StageArgs& a)
     return a.clock ? a.step + *a.clock : a.step;
 }
 
-// Slot j's record of the stage input: (x, y, theta) in a P4 (w unused).
+// Slot j's record of the stage input: (x, y, theta) in a P4 (w unused).  Addressed by 32-bit byte
+// offsets from the arrays' (uniform) bases, which the loads take as a VGPR offset to an SGPR base
+// (engines hold fewer than 2^27 particles).
 __device__ __forceinline__ P4 stage_record(const StageArgs& a, int j)
 {
-    const double2 p = a.gxy[j];
-    return P4{p.x, p.y, a.gth[j], 0.0};
+    const unsigned u = (unsigned)j;
+    const double2 p = *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(a.gxy) + (size_t)(u * 16u));
+    const double t = *reinterpret_cast<const double*>(reinterpret_cast<const char*>(a.gth) + (size_t)(u * 8u));
+    return P4{p.x, p.y, t, 0.0};
+}
+
+// Whether any live lane of the wavefront sits within img_m of a periodic edge (then every lane
+// takes the minimum image of its pair differences; otherwise none needs it).  Wave-uniform.
+__device__ __forceinline__ bool wave_needs_image(const Phys& P, bool live, double x, double y)
+{
+    const bool edge = live && !(fabs(x) <= P.Lx - P.img_m && fabs(y) <= P.Ly - P.img_m);
+    return __any(edge) != 0;
 }
 
 // Displacement trigger: marks the step (per workgroup, atomics stay inside one workgroup's slot)
@@ -748,6 +761,42 @@ struct NlistFlatArgs {
     int* blknnz;
 };
 
+// The candidates of slot i in flattened order (range k covers [p[k], p[k+1]), candidate t at slot
+// t + off[k]), eight requested per round; writes the list and returns its length before the cut.
+template <int kSeg, int kBatch, bool IMG>
+__device__ __forceinline__ int nlist_scan(const NlistFlatArgs& a, int i, double2 me, const Grid& G, const int* p,
+                                          const int* off)
+{
+    const int total = p[kSeg];
+    int c = 0;
+    for (int t0 = 0; t0 < total; t0 += kBatch) {
+        int jj[kBatch];
+        double2 q[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int t = t0 + u;
+            int o = off[0];
+#pragma unroll
+            for (int k = 1; k < kSeg; ++k) o = t >= p[k] ? off[k] : o;
+            jj[u] = t + o;
+            if (t < total) q[u] = a.pos[jj[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            if (t0 + u < total && jj[u] != i) {
+                double dx = spp::image_diff<IMG>(me.x - q[u].x, G.Lx2, G.invLx2);
+                double dy = spp::image_diff<IMG>(me.y - q[u].y, G.Ly2, G.invLy2);
+                double d2 = dx * dx + dy * dy;
+                if (d2 < a.T_list) {
+                    if (c < a.cap) a.nbr32[(size_t)c * a.N + i] = jj[u];
+                    ++c;
+                }
+            }
+        }
+    }
+    return c;
+}
+
 // Cell::Neighbor_List (shared/cell.h:74-103) for slot i over the 3x3 periodic stencil of grid G
 // (cell keys from koff).  Each stencil row is one or two contiguous slot ranges (the columns inside
 // the grid, then the column across the periodic edge), so the stencil is at most six ranges; their
@@ -790,32 +839,13 @@ __device__ __forceinline__ int nlist_particle(const NlistFlatArgs& a, int i, con
         }
     }
     const int total = p[kSeg];
-    int c = 0;
-    for (int t0 = 0; t0 < total; t0 += kBatch) {
-        int jj[kBatch];
-        double2 q[kBatch];
-#pragma unroll
-        for (int u = 0; u < kBatch; ++u) {
-            const int t = t0 + u;
-            int o = off[0];
-#pragma unroll
-            for (int k = 1; k < kSeg; ++k) o = t >= p[k] ? off[k] : o;
-            jj[u] = t + o;
-            if (t < total) q[u] = a.pos[jj[u]];
-        }
-#pragma unroll
-        for (int u = 0; u < kBatch; ++u) {
-            if (t0 + u < total && jj[u] != i) {
-                double dx = spp::min_image(me.x - q[u].x, G.Lx2, G.invLx2);
-                double dy = spp::min_image(me.y - q[u].y, G.Ly2, G.invLy2);
-                double d2 = dx * dx + dy * dy;
-                if (d2 < a.T_list) {
-                    if (c < a.cap) a.nbr32[(size_t)c * a.N + i] = jj[u];
-                    ++c;
-                }
-            }
-        }
-    }
+    // Cells away from the grid's edges (exact binning, five or more cells per side): every candidate
+    // sits in an adjacent cell on the same side of the periodic edges, closer than L, so the
+    // differences need no minimum image (image_diff).  Decided for the whole wavefront.
+    const bool inner = !G.quirk && G.ncx >= 5 && G.ncy >= 5 && cx >= 1 && cx <= G.ncx - 2 && cy >= 1 &&
+                       cy <= G.ncy - 2;
+    const int c = __all(inner) ? nlist_scan<kSeg, kBatch, false>(a, i, me, G, p, off)
+                               : nlist_scan<kSeg, kBatch, true>(a, i, me, G, p, off);
     a.ncount[i] = c < a.cap ? c : a.cap;
     a.build[i] = make_float2((float)me.x, (float)me.y);
     a.cellxy[i] = cx | (cy << 16);
@@ -876,7 +906,7 @@ __global__ __launch_bounds__(kBlock) void k_nlist_flat(NlistFlatArgs a)
 
 // GU: neighbour records in flight per lane (a block of the list loop).  Small systems, whose stage
 // time is one wavefront's chain of loads, take 8; large ones 4 (fewer VGPRs, same speed or better).
-template <int MODEL, int REP, int GU = 4>
+template <int MODEL, int REP, int GU = 4, bool IMG = true>
 __device__ __forceinline__ void gather_forces_flat(const StageArgs& a, const Phys& P, int i, const P4& me,
                                                    double mc, double ms, double& fx, double& fy, double& tq)
 {
@@ -896,7 +926,7 @@ __device__ __forceinline__ void gather_forces_flat(const StageArgs& a, const Phy
             if (k + u < cnt) q[u] = stage_record(a, j[u]);
 #pragma unroll
         for (int u = 0; u < GU; ++u)
-            if (k + u < cnt) spp::pair_accumulate<MODEL, REP>(P, me, mc, ms, q[u], fx, fy, s1, s2);
+            if (k + u < cnt) spp::pair_accumulate<MODEL, REP, IMG>(P, me, mc, ms, q[u], fx, fy, s1, s2);
     }
     tq = spp::drift_torque<MODEL>(P, s1, s2);
 }
@@ -909,6 +939,7 @@ __device__ __forceinline__ void stage_particle(const StageArgs& a, const Phys& P
                                                double& y)
 {
     P4 me = stage_record(a, i);
+    const bool img = wave_needs_image(P, true, me.x, me.y);  // over the lanes that run this
     double mc, ms;  // v = (cos, sin) of the stage input's heading
     spp::heading_unit(me.z, mc, ms);
     double dn = 0, th_in = 0, thold = 0;
@@ -929,7 +960,10 @@ __device__ __forceinline__ void stage_particle(const StageArgs& a, const Phys& P
         if (INTEG == SPP_RK4) kk = a.kacc[i];
     }
     double fx, fy, tq;
-    gather_forces_flat<MODEL, REP, GU>(a, P, i, me, mc, ms, fx, fy, tq);
+    if (img)
+        gather_forces_flat<MODEL, REP, GU, true>(a, P, i, me, mc, ms, fx, fy, tq);
+    else
+        gather_forces_flat<MODEL, REP, GU, false>(a, P, i, me, mc, ms, fx, fy, tq);
     double th;
     if (STAGE <= 1) {
         if (INTEG == SPP_EULER) {
@@ -2537,6 +2571,7 @@ Phys make_phys(const spp_params& p)
               (1. / (p.rc_rep * p.rc_rep) + 1. / (p.sigma_rep * p.rc_rep));
     P.noise_amp = std::sqrt(2 * p.D_rot * p.dt);  // BasicDynamicParticle::Set_Dr, particle.h:106-110
     P.seed = p.seed;
+    P.img_m = std::numeric_limits<double>::infinity();  // set by the engine for displacement lists
     return P;
 }
 
@@ -2694,7 +2729,8 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
         keys += (int64_t)d.ncx * d.ncy;
         Lmax = std::max(Lmax, std::max(d.Lx, d.Ly));
     }
-    if (ntot >= (int64_t)1 << 30 || keys >= (int64_t)1 << 30)
+    // (neighbour records are addressed by 32-bit byte offsets: 16 B per particle)
+    if (ntot >= (int64_t)1 << 27 || keys >= (int64_t)1 << 30)
         return fail(nullptr, SPP_ERR_ARG, "%lld particles / %lld cells: too many for one engine",
                     (long long)ntot, (long long)keys);
     int ndev = 0;
@@ -2809,6 +2845,11 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
     // float32 build positions: allow for their rounding (|x| <= L) when triggering
     double margin = 4.0 * Lmax * 5.960464477539063e-08;
     e->thresh = std::max(0.0, e->half_skin - margin);
+    // Displacement-triggered lists keep every listed neighbour of a kept step within r_list plus
+    // twice the trigger; a particle that far from the periodic edges has its neighbours on the same
+    // side (its differences need no minimum image).  A step whose check fails is discarded whole, so
+    // its pairs do not matter.  The fixed-period policy gives no such bound.
+    if (p->rebuild == SPP_REBUILD_DISPLACEMENT) e->P.img_m = p->r_list + 2 * e->half_skin + 1e-6;
     if (p->rebuild == SPP_REBUILD_DISPLACEMENT && e->thresh <= 0) {
         delete e;
         return fail(nullptr, SPP_ERR_ARG, "r_list leaves no skin for displacement-triggered rebuilds");

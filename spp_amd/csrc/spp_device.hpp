@@ -82,6 +82,10 @@ struct Phys {
     double inv_sigma;
     double noise_amp; // sqrt(2 D dt)
     uint64_t seed;
+    // A particle at most L - img_m from the centre along both axes has every listed neighbour on the
+    // same side of the periodic edges (displacement-triggered lists: neighbours are closer than
+    // r_list + the skin), so its pair differences need no minimum image.  Infinite: never assumed.
+    double img_m;
 };
 
 // ------------------------------------------------------------------------------------------------
@@ -108,6 +112,24 @@ __device__ __forceinline__ double min_image(double dx, double L2, double invL2)
 {
     double k = rint(dx * invL2);
     return fma(-k, L2, dx);
+}
+
+// The same difference when the caller knows that k = 0 (both coordinates away from the periodic
+// edges, closer than L to each other): fma(-0, 2L, dx) is dx, so skipping the image is exact.
+template <bool IMG>
+__device__ __forceinline__ double image_diff(double dx, double L2, double invL2)
+{
+    return IMG ? min_image(dx, L2, invL2) : dx;
+}
+
+// 1/sqrt(x) for a positive finite x: the hardware estimate and the correction the device library
+// applies to it (e = 1 - x y^2, y + y e (1/2 + 3/8 e)), without the library's guards for zero,
+// infinity and NaN, which pair distances never are; the same value as the library's rsqrt there.
+__device__ __forceinline__ double rsqrt_pos(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(y * -x, y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
 }
 
 // Theta wrap of the RK stages, shared/particle.h:599: th - floor(th/(2 pi) + 0.5)*2*pi, with the
@@ -175,12 +197,13 @@ __host__ __device__ __forceinline__ double noise_normal(uint64_t seed, uint32_t 
 //   CONTINUOUS (particle.h:296-338): for d2 < Ra^2, s1 += sin(th_j - th_i), s2 += (dr x v_i)/d
 // me holds i's record (x, y, theta); (mc, ms) = (cos, sin) of theta_i, needed by the CONTINUOUS
 // self-term only.  sin(th_j - th_i) is taken from the angles (sin_diff), like particle.h:750.
-template <int MODEL, int REP>
+// IMG = false: the caller guarantees that no pair crosses a periodic edge (image_diff).
+template <int MODEL, int REP, bool IMG = true>
 __device__ __forceinline__ void pair_accumulate(const Phys& P, const P4& me, double mc, double ms, const P4& pj,
                                                 double& fx, double& fy, double& s1, double& s2)
 {
-    double dx = min_image(me.x - pj.x, P.Lx2, P.invLx2);
-    double dy = min_image(me.y - pj.y, P.Ly2, P.invLy2);
+    double dx = image_diff<IMG>(me.x - pj.x, P.Lx2, P.invLx2);
+    double dy = image_diff<IMG>(me.y - pj.y, P.Ly2, P.invLy2);
     double d2 = dx * dx + dy * dy;
     if (MODEL == SPP_MODEL_CONTINUOUS) {
         if (d2 < P.Ra2) {
@@ -191,7 +214,7 @@ __device__ __forceinline__ void pair_accumulate(const Phys& P, const P4& me, dou
     } else {
         if (d2 >= P.T_max) return;
         if (REP != SPP_REP_NONE && d2 < P.T_rep) {
-            double ri = rsqrt(d2);  // 1/d
+            double ri = rsqrt_pos(d2);  // 1/d
             double s;
             if (REP == SPP_REP_HARMONIC) {  // Spring, force-fields.h:87-97: k (rc - d)/d
                 s = P.A * (P.rc * ri - 1.0);

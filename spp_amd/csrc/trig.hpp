@@ -25,6 +25,22 @@
 
 namespace spp {
 
+// fma(a, k, c) for a constant k, as one VOP3 instruction on the device.  Written as fma(k, z, c)
+// with two literal constants, the compiler keeps one of them in a VGPR and copies it into the
+// accumulator of a v_fmac before every term (an extra VALU operation per term); here k lives in an
+// SGPR and c in a VGPR the compiler hoists.  The product and the single rounding are those of fma,
+// so the value is bitwise the same on host and device.
+SPP_HD inline double fma_k(double a, double k, double c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(k), "v"(c));
+    return r;
+#else
+    return std::fma(a, k, c);
+#endif
+}
+
 SPP_HD inline void sincos_heading(double x, double* sn, double* cs)
 {
     constexpr double kTwoOverPi = 0.6366197723675814;
@@ -41,16 +57,16 @@ SPP_HD inline void sincos_heading(double x, double* sn, double* cs)
     // Estrin's scheme (pairs, then z^2, then z^4) for a short dependency chain
     const double z2 = z * z;
     const double z4 = z2 * z2;
-    const double p01 = fma(0.008333333333333333, z, -0.16666666666666666);
-    const double p23 = fma(2.7557319223985893e-06, z, -0.0001984126984126984);
-    const double p45 = fma(1.6059043836821613e-10, z, -2.505210838544172e-08);
-    const double p67 = fma(2.8114572543455206e-15, z, -7.647163731819816e-13);
+    const double p01 = fma_k(z, 0.008333333333333333, -0.16666666666666666);
+    const double p23 = fma_k(z, 2.7557319223985893e-06, -0.0001984126984126984);
+    const double p45 = fma_k(z, 1.6059043836821613e-10, -2.505210838544172e-08);
+    const double p67 = fma_k(z, 2.8114572543455206e-15, -7.647163731819816e-13);
     const double p = fma(fma(p67, z2, p45), z4, fma(p23, z2, p01));
     const double s = fma(r * z, p, r);
-    const double q01 = fma(-0.001388888888888889, z, 0.041666666666666664);
-    const double q23 = fma(-2.755731922398589e-07, z, 2.48015873015873e-05);
-    const double q45 = fma(-1.1470745597729725e-11, z, 2.08767569878681e-09);
-    const double q67 = fma(-1.5619206968586225e-16, z, 4.779477332387385e-14);
+    const double q01 = fma_k(z, -0.001388888888888889, 0.041666666666666664);
+    const double q23 = fma_k(z, -2.755731922398589e-07, 2.48015873015873e-05);
+    const double q45 = fma_k(z, -1.1470745597729725e-11, 2.08767569878681e-09);
+    const double q67 = fma_k(z, -1.5619206968586225e-16, 4.779477332387385e-14);
     const double q = fma(fma(q67, z2, q45), z4, fma(q23, z2, q01));
     const double hz = 0.5 * z;
     const double w = 1.0 - hz;
@@ -88,11 +104,11 @@ SPP_HD inline double sin_diff(double a, double b)
     const double z = r * r;
     const double z2 = z * z;
     const double z4 = z2 * z2;
-    const double p01 = fma(0.008333333333333333, z, -0.16666666666666666);
-    const double p23 = fma(2.7557319223985893e-06, z, -0.0001984126984126984);
-    const double p45 = fma(1.6059043836821613e-10, z, -2.505210838544172e-08);
-    const double p67 = fma(2.8114572543455206e-15, z, -7.647163731819816e-13);
-    const double p89 = fma(1.9572941063391263e-20, z, -8.22063524662433e-18);
+    const double p01 = fma_k(z, 0.008333333333333333, -0.16666666666666666);
+    const double p23 = fma_k(z, 2.7557319223985893e-06, -0.0001984126984126984);
+    const double p45 = fma_k(z, 1.6059043836821613e-10, -2.505210838544172e-08);
+    const double p67 = fma_k(z, 2.8114572543455206e-15, -7.647163731819816e-13);
+    const double p89 = fma_k(z, 1.9572941063391263e-20, -8.22063524662433e-18);
     const double q0 = fma(p23, z2, p01);
     const double q1 = fma(p67, z2, p45);
     const double p = fma(fma(p89, z4, q1), z4, q0);
