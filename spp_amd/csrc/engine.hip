@@ -763,12 +763,20 @@ struct NlistFlatArgs {
 
 // The candidates of slot i in flattened order (range k covers [p[k], p[k+1]), candidate t at slot
 // t + off[k]), eight requested per round; writes the list and returns its length before the cut.
+// Record j of a position array by a 32-bit byte offset from its (uniform) base (engines hold fewer
+// than 2^27 particles), which the load takes as a VGPR offset to an SGPR base.
+__device__ __forceinline__ double2 load_pos(const double2* base, int j)
+{
+    return *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(base) + (size_t)((unsigned)j * 16u));
+}
+
 template <int kSeg, int kBatch, bool IMG>
 __device__ __forceinline__ int nlist_scan(const NlistFlatArgs& a, int i, double2 me, const Grid& G, const int* p,
                                           const int* off)
 {
     const int total = p[kSeg];
     int c = 0;
+    int* w = a.nbr32 + i;  // the next list entry's address: row c of the slot-major list
     for (int t0 = 0; t0 < total; t0 += kBatch) {
         int jj[kBatch];
         double2 q[kBatch];
@@ -779,7 +787,7 @@ __device__ __forceinline__ int nlist_scan(const NlistFlatArgs& a, int i, double2
 #pragma unroll
             for (int k = 1; k < kSeg; ++k) o = t >= p[k] ? off[k] : o;
             jj[u] = t + o;
-            if (t < total) q[u] = a.pos[jj[u]];
+            if (t < total) q[u] = load_pos(a.pos, jj[u]);
         }
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
@@ -788,7 +796,10 @@ __device__ __forceinline__ int nlist_scan(const NlistFlatArgs& a, int i, double2
                 double dy = spp::image_diff<IMG>(me.y - q[u].y, G.Ly2, G.invLy2);
                 double d2 = dx * dx + dy * dy;
                 if (d2 < a.T_list) {
-                    if (c < a.cap) a.nbr32[(size_t)c * a.N + i] = jj[u];
+                    if (c < a.cap) {
+                        *w = jj[u];
+                        w += a.N;
+                    }
                     ++c;
                 }
             }
@@ -844,8 +855,14 @@ __device__ __forceinline__ int nlist_particle(const NlistFlatArgs& a, int i, con
     // differences need no minimum image (image_diff).  Decided for the whole wavefront.
     const bool inner = !G.quirk && G.ncx >= 5 && G.ncy >= 5 && cx >= 1 && cx <= G.ncx - 2 && cy >= 1 &&
                        cy <= G.ncy - 2;
-    const int c = __all(inner) ? nlist_scan<kSeg, kBatch, false>(a, i, me, G, p, off)
-                               : nlist_scan<kSeg, kBatch, true>(a, i, me, G, p, off);
+    int c;
+    if (__all(inner)) {
+        // no periodic column: the stencil is the three row ranges 0, 2, 4
+        const int p3[4] = {p[0], p[2], p[4], p[6]}, o3[3] = {off[0], off[2], off[4]};
+        c = nlist_scan<3, kBatch, false>(a, i, me, G, p3, o3);
+    } else {
+        c = nlist_scan<kSeg, kBatch, true>(a, i, me, G, p, off);
+    }
     a.ncount[i] = c < a.cap ? c : a.cap;
     a.build[i] = make_float2((float)me.x, (float)me.y);
     a.cellxy[i] = cx | (cy << 16);
