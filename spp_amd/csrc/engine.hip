@@ -90,13 +90,30 @@ __device__ __forceinline__ int xcd_block()
     return b;
 }
 
-__device__ __forceinline__ int cell_coord(double x, double L, double L2, int nc, int quirk)
+// Record j of a position array by a 32-bit byte offset from its (uniform) base (engines hold fewer
+// than 2^27 particles), which the load takes as a VGPR offset to an SGPR base.
+__device__ __forceinline__ double2 load_pos(const double2* base, int j)
+{
+    return *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(base) + (size_t)((unsigned)j * 16u));
+}
+
+// Cell index floor((x + L) nc / 2L) along one axis (quirk: the reference's truncating form).  The
+// quotient is formed as a product by the rounded reciprocal, which lies within 4 ulp of the rounded
+// quotient, so both have the same floor unless the product sits within 8 ulp of an integer; only
+// then (almost never) is the division itself evaluated.  The result is always the division's.
+__device__ __forceinline__ int cell_coord(double x, double L, double L2, double invL2, int nc, int quirk)
 {
     int c;
-    if (quirk)  // serial/box.h:116-117: the cast truncates (x + L) before scaling
+    if (quirk) {  // serial/box.h:116-117: the cast truncates (x + L) before scaling
         c = (int)((double)((int)(x + L) * nc) / L2);
-    else
-        c = (int)floor((x + L) * nc / L2);
+    } else {
+        const double q = (x + L) * nc;
+        const double a = q * invL2;
+        const double k = floor(a);
+        const double f = a - k;
+        const double m = 8.0 * 2.220446049250313e-16 * fabs(a) + 1e-300;
+        c = (f >= m && f <= 1.0 - m) ? (int)k : (int)floor(q / L2);
+    }
     return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
 }
 
@@ -187,8 +204,8 @@ __global__ __launch_bounds__(kBlock) void k_bin(int N, Grid G0, const double2* _
             koff = S.key_off;
         }
         const double2 p = pos[i];
-        int cx = cell_coord(p.x, G.Lx, G.Lx2, G.ncx, G.quirk);
-        int cy = cell_coord(p.y, G.Ly, G.Ly2, G.ncy, G.quirk);
+        int cx = cell_coord(p.x, G.Lx, G.Lx2, G.invLx2, G.ncx, G.quirk);
+        int cy = cell_coord(p.y, G.Ly, G.Ly2, G.invLy2, G.ncy, G.quirk);
         k = koff + cy * G.ncx + cx;
     }
     const int prev = __shfl_up(k, 1, 64);
@@ -499,16 +516,16 @@ __global__ __launch_bounds__(kBlock) void k_sort_tiles(SortArgs a)
             for (int k = 1; k < 6; ++k) o = q >= p[k] ? off[k] : o;
             sl[u] = q < total ? q + o : -1;
             if (sl[u] >= 0) {
-                xy[u] = a.in.xy[sl[u]];
+                xy[u] = load_pos(a.in.xy, sl[u]);
                 idv[u] = a.in.id[sl[u]];
             }
         }
 #pragma unroll
         for (int u = 0; u < kCollect; ++u) {
             if (sl[u] < 0) continue;
-            const int cy = cell_coord(xy[u].y, a.G.Ly, a.G.Ly2, H, a.G.quirk);
+            const int cy = cell_coord(xy[u].y, a.G.Ly, a.G.Ly2, a.G.invLy2, H, a.G.quirk);
             if (cy != y) continue;
-            const int cx = cell_coord(xy[u].x, a.G.Lx, a.G.Lx2, W, a.G.quirk);
+            const int cx = cell_coord(xy[u].x, a.G.Lx, a.G.Lx2, a.G.invLx2, W, a.G.quirk);
             if (cx < c0 || cx >= c1) continue;
             const int e = atomicAdd(&s_n, 1);
             if (e < kSortCap) {
@@ -763,13 +780,6 @@ struct NlistFlatArgs {
 
 // The candidates of slot i in flattened order (range k covers [p[k], p[k+1]), candidate t at slot
 // t + off[k]), eight requested per round; writes the list and returns its length before the cut.
-// Record j of a position array by a 32-bit byte offset from its (uniform) base (engines hold fewer
-// than 2^27 particles), which the load takes as a VGPR offset to an SGPR base.
-__device__ __forceinline__ double2 load_pos(const double2* base, int j)
-{
-    return *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(base) + (size_t)((unsigned)j * 16u));
-}
-
 template <int kSeg, int kBatch, bool IMG>
 __device__ __forceinline__ int nlist_scan(const NlistFlatArgs& a, int i, double2 me, const Grid& G, const int* p,
                                           const int* off)
@@ -818,8 +828,8 @@ __device__ __forceinline__ int nlist_particle(const NlistFlatArgs& a, int i, con
 {
     constexpr int kSeg = 6, kBatch = 8;
     const double2 me = a.pos[i];
-    const int cx = cell_coord(me.x, G.Lx, G.Lx2, G.ncx, G.quirk);
-    const int cy = cell_coord(me.y, G.Ly, G.Ly2, G.ncy, G.quirk);
+    const int cx = cell_coord(me.x, G.Lx, G.Lx2, G.invLx2, G.ncx, G.quirk);
+    const int cy = cell_coord(me.y, G.Ly, G.Ly2, G.invLy2, G.ncy, G.quirk);
     int x0 = cx - 1, x1 = cx + 1, xw = -1;
     if (x0 < 0) {
         xw = G.ncx - 1;

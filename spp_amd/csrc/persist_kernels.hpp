@@ -130,8 +130,8 @@ __device__ int rebuild_system(const Args& a, const SysInfo& S, const SetPtrs& fr
     const int koff = S.key_off, nk = G.ncx * G.ncy;
     for (int i = off + t; i < end; i += kBlock1024) {  // k_bin
         const double2 p = from.xy[i];
-        const int cx = cell_coord(p.x, G.Lx, G.Lx2, G.ncx, G.quirk);
-        const int cy = cell_coord(p.y, G.Ly, G.Ly2, G.ncy, G.quirk);
+        const int cx = cell_coord(p.x, G.Lx, G.Lx2, G.invLx2, G.ncx, G.quirk);
+        const int cy = cell_coord(p.y, G.Ly, G.Ly2, G.invLy2, G.ncy, G.quirk);
         const int k = koff + cy * G.ncx + cx;
         a.key[i] = k;
         atomicAdd(&a.count[k], 1);
