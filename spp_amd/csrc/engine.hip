@@ -38,6 +38,9 @@
 #include <vector>
 
 #include "spp_device.hpp"
+#include "binning.hpp"
+
+using spp_bin::cell_coord;
 
 using spp::P4;
 using spp::Phys;
@@ -95,26 +98,6 @@ __device__ __forceinline__ int xcd_block()
 __device__ __forceinline__ double2 load_pos(const double2* base, int j)
 {
     return *reinterpret_cast<const double2*>(reinterpret_cast<const char*>(base) + (size_t)((unsigned)j * 16u));
-}
-
-// Cell index floor((x + L) nc / 2L) along one axis (quirk: the reference's truncating form).  The
-// quotient is formed as a product by the rounded reciprocal, which lies within 4 ulp of the rounded
-// quotient, so both have the same floor unless the product sits within 8 ulp of an integer; only
-// then (almost never) is the division itself evaluated.  The result is always the division's.
-__device__ __forceinline__ int cell_coord(double x, double L, double L2, double invL2, int nc, int quirk)
-{
-    int c;
-    if (quirk) {  // serial/box.h:116-117: the cast truncates (x + L) before scaling
-        c = (int)((double)((int)(x + L) * nc) / L2);
-    } else {
-        const double q = (x + L) * nc;
-        const double a = q * invL2;
-        const double k = floor(a);
-        const double f = a - k;
-        const double m = 8.0 * 2.220446049250313e-16 * fabs(a) + 1e-300;
-        c = (f >= m && f <= 1.0 - m) ? (int)k : (int)floor(q / L2);
-    }
-    return c < 0 ? 0 : (c >= nc ? nc - 1 : c);
 }
 
 struct Grid {
