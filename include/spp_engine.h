@@ -198,6 +198,9 @@ typedef struct spp_info {
                                and measuring calls so far (spp_set_state, spp_multi_step,
                                spp_multi_step_sampled, spp_batch_polarization, spp_lyapunov_*): HIP
                                events on the engine's stream around each outermost call, summed  */
+    int64_t wide_rebuilds;  /* rebuilds that built the wide list (5x5-cell stencil, two cell widths):
+                               after a step moved a particle beyond the half skin right after a
+                               rebuild, until the particles slow down                              */
 } spp_info;
 
 /* ---- lifetime ----------------------------------------------------------------------------------- */

@@ -112,6 +112,7 @@ class Info(C.Structure):
         ("tile_rebuilds", C.c_int64),
         ("tile_fallbacks", C.c_int64),
         ("device_ms", C.c_double),
+        ("wide_rebuilds", C.c_int64),
     ]
 
 

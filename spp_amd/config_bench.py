@@ -127,14 +127,12 @@ def lyapunov64(scale):
         dt=0.005, R_align=1.0, repulsion=S.REP_HARMONIC, A_rep=10.0, sigma_rep=1.0, rc_rep=1.0, integrator=S.RK4,
         r_list=1.3, rebuild=S.REBUILD_DISPLACEMENT)
     systems = [S.system(n, L, v0=1.0, g=1.0, D_rot=0.05, seed=20 + r) for r in range(R) for _ in range(K + 1)]
-    # a triangular lattice (spacing 1.07 > the repulsion radius) with random headings: uniform random
-    # positions overlap so deeply somewhere among 5 million particles that a first step would move
-    # one farther than the half skin
-    lx, ly = S.triangle_lattice(n, L, L)
+    # uniform random positions and headings per replica: somewhere among 5 million particles the
+    # first steps push deep overlaps apart faster than the half skin per step, which the engine
+    # handles on its wide list (spp_info.wide_rebuilds) until the particles slow down
     xs, ys, ts = [], [], []
     for r in range(R):
-        _, _, t = S.random_state(50 + r, n, L, L)
-        x, y = lx, ly
+        x, y, t = S.random_state(50 + r, n, L, L)
         for _ in range(K + 1):
                   xs.append(x)
                   ys.append(y)

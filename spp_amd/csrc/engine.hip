@@ -751,6 +751,7 @@ struct NlistFlatArgs {
     const SysInfo* sys_tab;  // batched engine (null: single system)
     const int* sys_of;
     double T_list;
+    int wide;                // 1: the 5x5 stencil (the wide list after a deep overlap)
     const double2* pos;
     const int* start;
     int* nbr32;
@@ -802,23 +803,28 @@ __device__ __forceinline__ int nlist_scan(const NlistFlatArgs& a, int i, double2
 }
 
 // Cell::Neighbor_List (shared/cell.h:74-103) for slot i over the 3x3 periodic stencil of grid G
+// (R = 1; R = 2: the 5x5 stencil of the wide list, for pairs up to two cell widths apart)
 // (cell keys from koff).  Each stencil row is one or two contiguous slot ranges (the columns inside
 // the grid, then the column across the periodic edge), so the stencil is at most six ranges; their
 // bounds are read at once and the candidates are streamed eight at a time across range ends, in
 // range order, so the list keeps the row-by-row candidate order while a wave waits on a few rounds
 // of loads instead of one per range.  Returns the list length before the capacity cut.
+template <int R = 1>
 __device__ __forceinline__ int nlist_particle(const NlistFlatArgs& a, int i, const Grid& G, int koff)
 {
-    constexpr int kSeg = 6, kBatch = 8;
+    constexpr int kRows = 2 * R + 1, kSeg = 2 * kRows, kBatch = 8;
     const double2 me = a.pos[i];
     const int cx = cell_coord(me.x, G.Lx, G.Lx2, G.invLx2, G.ncx, G.quirk);
     const int cy = cell_coord(me.y, G.Ly, G.Ly2, G.invLy2, G.ncy, G.quirk);
-    int x0 = cx - 1, x1 = cx + 1, xw = -1;
+    // columns inside the grid [x0, x1], then the columns across the periodic edge [w0, w1)
+    int x0 = cx - R, x1 = cx + R, w0 = 0, w1 = 0;
     if (x0 < 0) {
-        xw = G.ncx - 1;
+        w0 = G.ncx + x0;
+        w1 = G.ncx;
         x0 = 0;
     } else if (x1 >= G.ncx) {
-        xw = 0;
+        w0 = 0;
+        w1 = x1 - G.ncx + 1;
         x1 = G.ncx - 1;
     }
     // range k covers flattened candidates [p[k], p[k+1]); candidate t sits at slot t + off[k]
@@ -826,14 +832,14 @@ __device__ __forceinline__ int nlist_particle(const NlistFlatArgs& a, int i, con
     {
         int s[kSeg], e[kSeg];
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            int y = cy + r - 1;
+        for (int r = 0; r < kRows; ++r) {
+            int y = cy + r - R;
             y = y < 0 ? y + G.ncy : (y >= G.ncy ? y - G.ncy : y);
             const int row = koff + y * G.ncx;
             s[2 * r] = a.start[row + x0];
             e[2 * r] = a.start[row + x1 + 1];
-            s[2 * r + 1] = xw < 0 ? 0 : a.start[row + xw];
-            e[2 * r + 1] = xw < 0 ? 0 : a.start[row + xw + 1];
+            s[2 * r + 1] = w1 > w0 ? a.start[row + w0] : 0;
+            e[2 * r + 1] = w1 > w0 ? a.start[row + w1] : 0;
         }
         p[0] = 0;
 #pragma unroll
@@ -842,16 +848,15 @@ __device__ __forceinline__ int nlist_particle(const NlistFlatArgs& a, int i, con
             off[k] = s[k] - p[k];
         }
     }
-    const int total = p[kSeg];
     // Cells away from the grid's edges (exact binning, five or more cells per side): every candidate
     // sits in an adjacent cell on the same side of the periodic edges, closer than L, so the
     // differences need no minimum image (image_diff).  Decided for the whole wavefront.
-    const bool inner = !G.quirk && G.ncx >= 5 && G.ncy >= 5 && cx >= 1 && cx <= G.ncx - 2 && cy >= 1 &&
+    const bool inner = R == 1 && !G.quirk && G.ncx >= 5 && G.ncy >= 5 && cx >= 1 && cx <= G.ncx - 2 && cy >= 1 &&
                        cy <= G.ncy - 2;
     int c;
     if (__all(inner)) {
         // no periodic column: the stencil is the three row ranges 0, 2, 4
-        const int p3[4] = {p[0], p[2], p[4], p[6]}, o3[3] = {off[0], off[2], off[4]};
+        const int p3[4] = {p[0], p[2], p[4], p[kSeg]}, o3[3] = {off[0], off[2], off[4]};
         c = nlist_scan<3, kBatch, false>(a, i, me, G, p3, o3);
     } else {
         c = nlist_scan<kSeg, kBatch, true>(a, i, me, G, p, off);
@@ -909,7 +914,7 @@ __global__ __launch_bounds__(kBlock) void k_nlist_flat(NlistFlatArgs a)
         Grid G;
         int koff;
         system_grid(a, i, G, koff);
-        c = nlist_particle(a, i, G, koff);
+        c = a.wide ? nlist_particle<2>(a, i, G, koff) : nlist_particle<1>(a, i, G, koff);
     }
     list_stats(a, b, c);
 }
@@ -1569,6 +1574,7 @@ struct Interval {
     int64_t since0;  // steps since the last rebuild at the interval's start (after its rebuild)
     double t0;
     bool rebuilt;
+    bool wide;       // its steps ran on the wide list
     long long seq;   // the report count of its slot once the interval's report has been written
 };
 
@@ -1592,6 +1598,13 @@ struct spp_engine {
     int cap = 32;
     Phys P{};
     double r_cut = 0, half_skin = 0, thresh = 0, T_list = 0;
+    // The wide list: a 5x5-cell stencil with radius r_wide (two cell widths), built instead of the
+    // ordinary one after a step moved a particle beyond the half skin right after a rebuild (a deep
+    // overlap), until the particles slow down.  wide: the current list is wide; wide_next: the next
+    // rebuild builds one.
+    bool wide_ok = false, wide = false, wide_next = false;
+    double r_wide = 0, T_wide = 0, thresh_wide = 0, img_m_wide = 0;
+    int64_t wide_rebuilds = 0;
     hipStream_t stream = nullptr;
     int device = 0;
 
@@ -1839,8 +1852,11 @@ void free_set(SetPtrs& s)
 // ------------------------------------------------------------------------------------------------
 int launch_nlist(spp_engine* e, int slot)
 {
-    NlistFlatArgs fa{e->N, e->cap, e->G, e->dsys, e->sys_of, e->T_list, e->set[e->cur].xy, e->cell_start, e->nbr32,
-                     e->ncount, e->build, e->cellxy, e->slotb[slot].blklen, e->slotb[slot].blknnz};
+    e->wide = e->wide_next && e->wide_ok;
+    if (e->wide) e->wide_rebuilds++;
+    NlistFlatArgs fa{e->N, e->cap, e->G, e->dsys, e->sys_of, e->wide ? e->T_wide : e->T_list, e->wide ? 1 : 0,
+                     e->set[e->cur].xy, e->cell_start, e->nbr32, e->ncount, e->build, e->cellxy,
+                     e->slotb[slot].blklen, e->slotb[slot].blknnz};
     return launch(e, "nlist", k_nlist_flat, e->nblk, fa);
 }
 
@@ -1997,6 +2013,9 @@ int take_injected(spp_engine* e)
 
 // Launch one fused step.  S = set holding the step's input state, W = set receiving its output
 // (S == W except for the first step of an interval).
+// The displacement trigger of the list in place (ordinary or wide).
+double cur_thresh(const spp_engine* e) { return e->wide ? e->thresh_wide : e->thresh; }
+
 int launch_step(spp_engine* e, int S, int W, int kstep, int slot, bool record)
 {
     int integ = e->p.integrator;
@@ -2016,7 +2035,8 @@ int launch_step(spp_engine* e, int S, int W, int kstep, int slot, bool record)
     a.step = e->graph_mode ? e->step - e->graph_base : e->step;
     a.clock = e->graph_mode ? e->dclock : nullptr;
     a.kstep = kstep;
-    a.thresh2 = e->thresh * e->thresh;
+    a.thresh2 = cur_thresh(e) * cur_thresh(e);
+    if (e->wide) a.P.img_m = e->img_m_wide;
     a.blkbad = e->slotb[slot].blkbad;
     a.blockmax = e->slotb[slot].blkmax;
     a.th0 = e->set[S].th;
@@ -2104,6 +2124,7 @@ int interval_body(spp_engine* e, int m, bool rebuild, int protect, int slot, Int
         TRY(launch_rebuild(e, dst, slot, false, true));
     }
     iv.start_set = e->cur;
+    iv.wide = e->wide;
     iv.step0 = e->step;
     iv.since0 = e->steps_since_build;
     iv.t0 = e->t;
@@ -2275,11 +2296,18 @@ int run_intervals(spp_engine* e, int steps)
             if (!period && iv.rebuilt) {
                 // linear extrapolation of the largest displacement to the trigger
                 double disp = std::sqrt(f.maxdisp2);
-                double est = disp > 0 ? e->thresh / disp * iv.m : 4096.0;
+                double est = disp > 0 ? (iv.wide ? e->thresh_wide : e->thresh) / disp * iv.m : 4096.0;
                 e->T_hist[2] = e->T_hist[1];
                 e->T_hist[1] = e->T_hist[0];
                 e->T_hist[0] = est;
                 derive_schedule(e);
+                // back to the ordinary list once the fastest particle covers less than half its
+                // trigger per step (the next rebuild builds it; the estimates start afresh)
+                if (iv.wide && e->wide_next && disp < 0.5 * e->thresh * iv.m) {
+                    e->wide_next = false;
+                    for (double& h : e->T_hist) h = 0;
+                    derive_schedule(e);
+                }
             }
             continue;
         }
@@ -2313,10 +2341,20 @@ int run_intervals(spp_engine* e, int steps)
             e->force_m = iv.m;
             continue;
         }
-        if (bad == 0 && iv.rebuilt)
+        if (bad == 0 && iv.rebuilt) {
+            // A step right after a rebuild moved a particle beyond the trigger (a deep overlap
+            // pushed apart): redo it on the wide list, whose skin is several times wider.
+            if (e->wide_ok && !iv.wide) {
+                e->wide_next = true;
+                e->force_m = 1;
+                for (double& h : e->T_hist) h = 0;
+                derive_schedule(e);
+                continue;
+            }
             return fail(e, SPP_ERR_STATE,
-                        "a particle moves farther than the half skin (%g) within one step: "
-                        "raise r_list or lower dt", e->half_skin);
+                        "a particle moves farther than %g within one step%s: raise r_list or lower dt",
+                        iv.wide ? e->thresh_wide : e->half_skin, iv.wide ? " (on the wide list)" : "");
+        }
         // The rebuild at the interval's start (identical, the state there is already sorted) is
         // redone; the valid prefix [0, bad) is replayed as its own interval and verified again.
         e->force_m = bad > 0 ? bad : 0;
@@ -2384,8 +2422,8 @@ int run_persist(spp_engine* e, int steps)
         st.nbr32 = e->nbr32;
         st.ncount = e->ncount;
         st.build = e->build;
-        a.nl = NlistFlatArgs{e->N, e->cap, e->G, e->dsys, e->sys_of, e->T_list, nullptr, e->cell_start, e->nbr32,
-                             e->ncount, e->build, e->cellxy, nullptr, nullptr};
+        a.nl = NlistFlatArgs{e->N, e->cap, e->G, e->dsys, e->sys_of, e->T_list, 0, nullptr, e->cell_start,
+                             e->nbr32, e->ncount, e->build, e->cellxy, nullptr, nullptr};
         a.sys_tab = e->dsys ? e->dsys : e->dsys1;
         a.cur = e->set[e->cur];
         a.spare = e->set[spare];
@@ -2860,6 +2898,24 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
     // side (its differences need no minimum image).  A step whose check fails is discarded whole, so
     // its pairs do not matter.  The fixed-period policy gives no such bound.
     if (p->rebuild == SPP_REBUILD_DISPLACEMENT) e->P.img_m = p->r_list + 2 * e->half_skin + 1e-6;
+    // The wide list: five or more cells per side in every system (the 5x5 stencil then holds every
+    // pair up to two cell widths apart, and two widths stay below L), exact binning, displacement
+    // policy, multi-launch kernels.
+    {
+        bool ok = p->rebuild == SPP_REBUILD_DISPLACEMENT && !persist && p->binning != SPP_BIN_REF_QUIRK;
+        double wmin = 1e300;
+        for (int s = 0; s < nsys; ++s) {
+            ok = ok && sd[s].ncx >= 5 && sd[s].ncy >= 5;
+            wmin = std::min(wmin, std::min(2 * sd[s].Lx / sd[s].ncx, 2 * sd[s].Ly / sd[s].ncy));
+        }
+        if (ok) {
+            e->wide_ok = true;
+            e->r_wide = 2 * wmin;
+            e->T_wide = sqrt_threshold(e->r_wide);
+            e->thresh_wide = std::max(0.0, 0.5 * (e->r_wide - rcut) - margin);
+            e->img_m_wide = e->r_wide + 2 * 0.5 * (e->r_wide - rcut) + 1e-6;
+        }
+    }
     if (p->rebuild == SPP_REBUILD_DISPLACEMENT && e->thresh <= 0) {
         delete e;
         return fail(nullptr, SPP_ERR_ARG, "r_list leaves no skin for displacement-triggered rebuilds");
@@ -3068,6 +3124,7 @@ int spp_get_info(spp_engine* e, spp_info* info)
     info->tile_rebuilds = e->tile_rebuilds;
     info->tile_fallbacks = e->tile_fallbacks;
     info->device_ms = e->device_ms;
+    info->wide_rebuilds = e->wide_rebuilds;
     return SPP_OK;
 }
 
@@ -3107,6 +3164,7 @@ int spp_set_state(spp_engine* e, const double* xw, const double* yw, const doubl
     e->have_state = true;
     e->unfused_stage = 0;
     e->inj_pending = false;
+    e->wide_next = false;  // a new state starts on the ordinary list
     return rebuild_sync(e, false);
 }
 
@@ -3764,7 +3822,8 @@ int spp_time_stage(spp_engine* e, int32_t stage, int32_t reps, double* avg_us)
     a.id = S.id;
     a.build = e->build;
     a.step = e->step;
-    a.thresh2 = e->thresh * e->thresh;
+    a.thresh2 = cur_thresh(e) * cur_thresh(e);
+    if (e->wide) a.P.img_m = e->img_m_wide;
     a.blkbad = e->slotb[0].blkbad;
     a.blockmax = e->slotb[0].blkmax;
     a.gxy = stage <= 1 ? S.xy : e->xy1;
