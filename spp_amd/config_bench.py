@@ -134,9 +134,9 @@ def lyapunov64(scale):
     for r in range(R):
         x, y, t = S.random_state(50 + r, n, L, L)
         for _ in range(K + 1):
-                  xs.append(x)
-                  ys.append(y)
-                  ts.append(t)
+            xs.append(x)
+            ys.append(y)
+            ts.append(t)
     x, y, th = np.concatenate(xs), np.concatenate(ys), np.concatenate(ts)
     # the autotuner picks four or eight neighbour records in flight per lane by timing plain steps
     # of this batch (about a second); the winner is reported with the line
