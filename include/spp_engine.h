@@ -104,7 +104,7 @@ enum {
                                   stream) runs a whole spp_multi_step in one launch: rebuilds when a
                                   system's own displacement (or period) requires; single or batched */
     SPP_KERNEL_SLOT_DEEP = 4   /* as SLOT with eight neighbour records in flight per lane at every
-                                  size (SLOT keeps eight below 131,072 particles, four above); same
+                                  size (SLOT keeps eight below 393,216 particles, four above); same
                                   results bit for bit; single or batched, Vicsek models            */
 };
 

@@ -6,7 +6,7 @@ frequency:
 * ``kernel_variant`` -- the interaction tile strategy: one lane per slot with L2 gathers (0), one
   workgroup per system running the whole call in one launch (3; systems of at most 4096
   particles), or the slot kernel with eight neighbour records in flight per lane at every size (4;
-  the slot kernel takes eight only below 131,072 particles);
+  the slot kernel takes eight only below 393,216 particles);
 * ``r_list`` -- the Verlet radius: a wider skin lengthens every list but rebuilds less often;
 * the cell size -- ``ncell_x/y`` below floor(2L / r_list) makes cells wider than the list radius
   (more candidates per list build, fewer empty cells).

@@ -48,9 +48,10 @@ using spp::Phys;
 namespace {
 
 constexpr int kBlock = 256;
-// below this many particles (two wavefronts per SIMD of the device) the stage kernels keep
-// eight neighbour records in flight per lane instead of four (SPP_KERNEL_SLOT_DEEP: at every size)
-constexpr int kSmallN = 131072;
+// below this many particles (six wavefronts per SIMD of the device) the stage kernels keep eight
+// neighbour records in flight per lane instead of four (SPP_KERNEL_SLOT_DEEP: at every size); at
+// 262,144 particles eight give 4 % shorter RK2 stages, at 1,048,576 four are 5 % faster
+constexpr int kSmallN = 393216;
 // up to this many particles the stage kernels write their outputs with streaming stores: the
 // launch is short, and ordinary stores would leave its outputs dirty in the L2s until the
 // write-back at its end.  Larger engines keep ordinary stores, which leave the outputs in the

@@ -269,10 +269,10 @@ def lyap_engine(p, R, K, n, L, D, eps, rescale=True):
 
 
 def test_lyapunov_deep_variant_bitwise_equals_slot():
-    """Ten 16,384-particle systems (2 replicas x 5) exceed 131,072 particles, where the slot
-    kernel keeps four neighbour records in flight and the deep variant eight; both add the pairs
-    in list order, so states and growth sums are bit-for-bit equal."""
-    R, K, n, L = 2, 4, 16384, 64.0
+    """Twenty-five 16,384-particle systems (5 replicas x 5) exceed 393,216 particles, where the
+    slot kernel keeps four neighbour records in flight and the deep variant eight; both add the
+    pairs in list order, so states and growth sums are bit-for-bit equal."""
+    R, K, n, L = 5, 4, 16384, 64.0
     lx, ly = S.triangle_lattice(n, L, L)
     out = []
     for v in (S.KERNEL_SLOT, S.KERNEL_SLOT_DEEP):

@@ -499,12 +499,12 @@ def test_kernel_variants_dense_clusters(oracle_mod, variant):
 
 @pytest.mark.parametrize("integ", [S.RK2, S.RK4])
 def test_deep_variant_bitwise_equals_slot(oracle_mod, integ):
-    """Above 131,072 particles the slot kernel keeps four neighbour records in flight per lane and
+    """From 393,216 particles the slot kernel keeps four neighbour records in flight per lane and
     the deep variant eight; both add the pairs in list order, so the trajectories are bit-for-bit
     equal."""
-    n, L = 147456, 192.0
+    n, L = 409600, 320.0
     th = oracle_mod.random_state(8, n, L, L)[2]
-    x, y = S.triangle_lattice(n, L, L)  # no deep overlaps among 147,456 particles
+    x, y = S.triangle_lattice(n, L, L)  # no deep overlaps on a lattice
     out = []
     for v in (S.KERNEL_SLOT, S.KERNEL_SLOT_DEEP):
         e = S.Engine(c2_params(n=n, L=L, kernel_variant=v, D=0.2).update(integrator=integ))
