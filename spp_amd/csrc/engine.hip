@@ -63,7 +63,8 @@ constexpr int kSets = 4;       // particle state sets (see spp_engine::set)
 constexpr int kScanTile = kBlock * kScanItems;
 
 // One interval's bookkeeping, reduced on the device from per-block partials (no same-address
-// atomics across the chip) and written straight into host-mapped memory.
+// atomics across the chip) and written straight into host-mapped memory as tagged words (k_report),
+// which the host decodes into this record.
 struct Report {
     int bad_step;     // first step index (within the interval) whose stage input left the skin
     int max_len;      // longest full neighbour list of the interval's rebuild
@@ -71,8 +72,14 @@ struct Report {
     double maxdisp2;  // largest squared displacement at the interval's end
     int sort_fail;    // the interval's rebuild took the tile path and could not sort (redo it)
     int pad;
-    long long seq;    // the slot's report count, written last: the host polls it for completion
+    long long seq;    // the slot's report count
 };
+
+// 64-bit words of one slot's report in host memory: the report number in the upper half of each,
+// one 32-bit value in the lower half (bad step, longest list, entries low / high, largest squared
+// displacement low / high bits, rebuild failure flag).
+constexpr int kRepWords = 8;
+constexpr int kRepUsed = 7;
 
 struct SlotBufs {
     int* blkbad;      // per-block first bad step (atomicMin inside a block only)
@@ -1255,13 +1262,15 @@ __global__ __launch_bounds__(kBlock) void k_pair_hist(int N, const double2* __re
         if (sh[k]) atomicAdd(&hist[k], (unsigned long long)sh[k]);
 }
 
-// Reduces one interval's per-block partials into the host-mapped report (one block) and clears
-// the per-block bad steps for the slot's next interval.  The report's fields reach host memory
-// before its sequence number (the slot's count of reports, kept on the device so that replays of a
-// captured interval count too), which the host polls instead of waiting on an event: an event
-// record between two intervals idles the GPU for about 5 us on this device.
+// Reduces one interval's per-block partials into the slot's host-mapped report words (one block)
+// and clears the per-block bad steps for the slot's next interval.  Each word carries the report
+// number (the slot's count of reports, kept on the device so that replays of a captured interval
+// count too) in its upper half and goes out as one system-scope store, so the host takes the
+// report once every word's tag matches, with no fence on the device between them.  The host polls
+// the words instead of waiting on an event: an event record between two intervals idles the GPU
+// for about 5 us on this device.
 __global__ __launch_bounds__(kBlock) void k_report(int nblk, SlotBufs b, int with_list, int* sort_fail, long long* count,
-                                                  Report* out)
+                                                  unsigned long long* out)
 {
     __shared__ int r_bad[kBlock / 64], r_len[kBlock / 64];
     __shared__ long long r_nnz[kBlock / 64];
@@ -1305,17 +1314,19 @@ __global__ __launch_bounds__(kBlock) void k_report(int nblk, SlotBufs b, int wit
         mx = fmax(mx, r_max[0]);
         nnz = 0;
         for (int k = 0; k < kBlock / 64; ++k) nnz += r_nnz[k];
-        out->bad_step = bad;
-        out->max_len = len;
-        out->nnz = nnz;
-        out->maxdisp2 = mx;
-        out->sort_fail = *sort_fail;
+        const int fail = *sort_fail;
         *sort_fail = 0;  // the next interval's rebuild starts clear
         const long long seq = *count + 1;
         *count = seq;
-        __threadfence_system();
-        *(volatile long long*)&out->seq = seq;
-        __threadfence_system();
+        const unsigned long long tag = (unsigned long long)(unsigned)seq << 32;
+        const unsigned long long n = (unsigned long long)nnz;
+        const unsigned long long bits = (unsigned long long)__double_as_longlong(mx);
+        const unsigned long long w[kRepUsed] = {tag | (unsigned)bad,       tag | (unsigned)len,
+                                                tag | (n & 0xffffffffull), tag | (n >> 32),
+                                                tag | (bits & 0xffffffffull), tag | (bits >> 32),
+                                                tag | (unsigned)fail};
+#pragma unroll
+        for (int k = 0; k < kRepUsed; ++k) __hip_atomic_store(out + k, w[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -1645,8 +1656,9 @@ struct spp_engine {
     int64_t tile_rebuilds = 0, tile_fallbacks = 0;
     AbpConst abp{};
     SlotBufs slotb[2] = {};      // per-block partials of the (at most two) intervals in flight
-    Report* hrep = nullptr;      // host-mapped reports, 2 slots
-    Report* drep = nullptr;      // device view of hrep
+    Report hrep[2] = {};         // the slots' last reports, decoded by wait_report
+    unsigned long long* hwords = nullptr;  // host-mapped report words, 2 slots x kRepWords
+    unsigned long long* dwords = nullptr;  // device view of hwords
     long long* rep_count = nullptr;  // device: reports written per slot
     long long rep_issued[2] = {0, 0};  // host: reports enqueued per slot (launched or replayed)
     double2* dpolar = nullptr;
@@ -1866,23 +1878,34 @@ int launch_report(spp_engine* e, int slot, bool with_list)
     // counted also in dry mode: a dry interval is replayed from a captured graph, which runs it
     e->rep_issued[slot]++;
     return launch(e, "report", k_report, 1, e->nstat, e->slotb[slot], with_list ? 1 : 0, e->sort_fail,
-                  e->rep_count + slot, e->drep + slot);
+                  e->rep_count + slot, e->dwords + slot * kRepWords);
 }
 
-// Waits until the slot's report number `want` has reached host memory.  The host spins on the
-// report's sequence number; every ~1024 spins it asks the stream whether it failed or ran dry
-// without writing the report (an error either way), so the wait always ends.
+// Waits until the slot's report number `want` has reached host memory, then decodes it into
+// e->hrep[slot].  The host spins until every word of the slot carries the number's tag (reports of
+// one slot are numbered 1, 2, ... and complete in order); every ~1024 spins it asks the stream
+// whether it failed or ran dry without the report (an error either way), so the wait always ends.
 int wait_report(spp_engine* e, int slot, long long want)
 {
-    volatile long long* seq = &e->hrep[slot].seq;
+    const volatile unsigned long long* w = e->hwords + slot * kRepWords;
+    const unsigned long long tag = (unsigned long long)(unsigned)want << 32;
+    auto done = [&]() {
+        for (int k = 0; k < kRepUsed; ++k)
+            if ((w[k] & 0xffffffff00000000ull) != tag) return false;
+        return true;
+    };
     for (long spins = 1;; ++spins) {
-        if (*seq >= want) break;
+        if (done()) break;
         if ((spins & 1023) == 0) {
             const hipError_t q = hipStreamQuery(e->stream);
             if (q != hipSuccess && q != hipErrorNotReady)
                 return fail(e, SPP_ERR_HIP, "interval: %s", hipGetErrorString(q));
             if (q == hipSuccess) {
-                if (*seq >= want) break;
+                // the stream has finished; the words may still be on their way to host memory, so
+                // allow them a grace period (about a millisecond) before calling the report lost
+                bool ok = false;
+                for (long g = 0; g < (1L << 14) && !(ok = done()); ++g) __builtin_ia32_pause();
+                if (ok) break;
                 return fail(e, SPP_ERR_HIP, "interval report %lld of slot %d missing after the stream drained",
                             want, slot);
             }
@@ -1890,6 +1913,17 @@ int wait_report(spp_engine* e, int slot, long long want)
         __builtin_ia32_pause();
     }
     std::atomic_thread_fence(std::memory_order_acquire);
+    unsigned long long v[kRepUsed];
+    for (int k = 0; k < kRepUsed; ++k) v[k] = w[k] & 0xffffffffull;
+    Report r{};
+    r.bad_step = (int)(unsigned)v[0];
+    r.max_len = (int)(unsigned)v[1];
+    r.nnz = (long long)(v[2] | v[3] << 32);
+    const unsigned long long bits = v[4] | v[5] << 32;
+    std::memcpy(&r.maxdisp2, &bits, sizeof r.maxdisp2);
+    r.sort_fail = (int)(unsigned)v[6];
+    r.seq = want;
+    e->hrep[slot] = r;
     return SPP_OK;
 }
 
@@ -1994,6 +2028,7 @@ int rebuild_sync(spp_engine* e, bool aux)
         TRY(launch(e, "fill", k_fill<double>, nblocks(e->nstat), e->slotb[0].blkmax, 0.0, e->nstat));
         TRY(launch_report(e, 0, true));
         HIPC(e, hipStreamSynchronize(e->stream));
+        TRY(wait_report(e, 0, e->rep_issued[0]));
         e->host_syncs++;
         int need = e->hrep[0].max_len;
         e->max_nbrs = need;
@@ -2985,12 +3020,13 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
             if (rc == SPP_OK && hipMemcpy(e->dsys1, &S, sizeof S, hipMemcpyHostToDevice) != hipSuccess) rc = SPP_ERR_HIP;
         }
     }
-    // coherent host memory: the host polls the reports while kernels write them
-    if (rc == SPP_OK &&
-        hipHostMalloc((void**)&e->hrep, 2 * sizeof(Report), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+    // coherent host memory: the host polls the report words while kernels write them (tag 0: no
+    // report yet)
+    const size_t wbytes = 2 * kRepWords * sizeof(unsigned long long);
+    if (rc == SPP_OK && hipHostMalloc((void**)&e->hwords, wbytes, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
         rc = SPP_ERR_HIP;
-    if (rc == SPP_OK) std::memset((void*)e->hrep, 0, 2 * sizeof(Report));
-    if (rc == SPP_OK && hipHostGetDevicePointer((void**)&e->drep, e->hrep, 0) != hipSuccess) rc = SPP_ERR_HIP;
+    if (rc == SPP_OK) std::memset((void*)e->hwords, 0, wbytes);
+    if (rc == SPP_OK && hipHostGetDevicePointer((void**)&e->dwords, e->hwords, 0) != hipSuccess) rc = SPP_ERR_HIP;
     A(dalloc(e, &e->rep_count, 2));
     if (rc == SPP_OK && hipMemset(e->rep_count, 0, 2 * sizeof(long long)) != hipSuccess) rc = SPP_ERR_HIP;
     if (p->device_timing &&
@@ -3078,7 +3114,7 @@ void spp_destroy(spp_engine* e)
     for (auto ev : e->event_pool) (void)hipEventDestroy(ev);
     for (auto ev : {e->dev_t0, e->dev_t1})
         if (ev) (void)hipEventDestroy(ev);
-    if (e->hrep) (void)hipHostFree(e->hrep);
+    if (e->hwords) (void)hipHostFree(e->hwords);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
