@@ -929,7 +929,8 @@ __global__ __launch_bounds__(kBlock) void k_nlist_flat(NlistFlatArgs a)
 
 // GU: neighbour records in flight per lane (a block of the list loop).  Small systems, whose stage
 // time is one wavefront's chain of loads, take 8; large ones 4 (fewer VGPRs, same speed or better).
-template <int MODEL, int REP, int GU = 4, bool IMG = true>
+// ONE: one cutoff for both pair laws (pair_accumulate).
+template <int MODEL, int REP, int GU = 4, bool IMG = true, bool ONE = false>
 __device__ __forceinline__ void gather_forces_flat(const StageArgs& a, const Phys& P, int i, const P4& me,
                                                    double mc, double ms, double& fx, double& fy, double& tq)
 {
@@ -949,7 +950,7 @@ __device__ __forceinline__ void gather_forces_flat(const StageArgs& a, const Phy
             if (k + u < cnt) q[u] = stage_record(a, j[u]);
 #pragma unroll
         for (int u = 0; u < GU; ++u)
-            if (k + u < cnt) spp::pair_accumulate<MODEL, REP, IMG>(P, me, mc, ms, q[u], fx, fy, s1, s2);
+            if (k + u < cnt) spp::pair_accumulate<MODEL, REP, IMG, ONE>(P, me, mc, ms, q[u], fx, fy, s1, s2);
     }
     tq = spp::drift_torque<MODEL>(P, s1, s2);
 }
@@ -957,7 +958,7 @@ __device__ __forceinline__ void gather_forces_flat(const StageArgs& a, const Phy
 // One particle's fused stage (interaction at the stage input + the move), for slot i of a system
 // whose constants P already hold and whose particle indices start at pid_off.  Returns the wrapped
 // output position for the displacement check.
-template <int MODEL, int REP, int INTEG, int STAGE, int GU = 4>
+template <int MODEL, int REP, int INTEG, int STAGE, int GU = 4, bool ONE = false>
 __device__ __forceinline__ void stage_particle(const StageArgs& a, const Phys& P, int i, int pid_off, double& x,
                                                double& y)
 {
@@ -984,9 +985,9 @@ __device__ __forceinline__ void stage_particle(const StageArgs& a, const Phys& P
     }
     double fx, fy, tq;
     if (img)
-        gather_forces_flat<MODEL, REP, GU, true>(a, P, i, me, mc, ms, fx, fy, tq);
+        gather_forces_flat<MODEL, REP, GU, true, ONE>(a, P, i, me, mc, ms, fx, fy, tq);
     else
-        gather_forces_flat<MODEL, REP, GU, false>(a, P, i, me, mc, ms, fx, fy, tq);
+        gather_forces_flat<MODEL, REP, GU, false, ONE>(a, P, i, me, mc, ms, fx, fy, tq);
     double th;
     if (STAGE <= 1) {
         if (INTEG == SPP_EULER) {
@@ -1030,7 +1031,7 @@ __device__ __forceinline__ void stage_particle(const StageArgs& a, const Phys& P
     spp::store_out(a.stream, &a.oth[i], th);
 }
 
-template <int MODEL, int REP, int INTEG, int STAGE, bool BATCH, int GU = 4>
+template <int MODEL, int REP, int INTEG, int STAGE, bool BATCH, int GU = 4, bool ONE = false>
 __global__ __launch_bounds__(kBlock) void k_stage_flat(StageArgs a)
 {
     int b = xcd_block();
@@ -1045,7 +1046,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_flat(StageArgs a)
     }
     constexpr bool final_stage = (INTEG == SPP_EULER) || (INTEG == SPP_RK2 && STAGE == 2) || (STAGE == 4);
     double x = 0, y = 0, dmax = 0;
-    if (live) stage_particle<MODEL, REP, INTEG, STAGE, GU>(a, P, i, pid_off, x, y);
+    if (live) stage_particle<MODEL, REP, INTEG, STAGE, GU, ONE>(a, P, i, pid_off, x, y);
     check_disp(a, P, b, live, i, x, y, final_stage ? a.kstep + 1 : a.kstep, dmax);
     finish_disp(a, b, dmax);
 }
@@ -1338,26 +1339,32 @@ using StageFn = void (*)(StageArgs);
 constexpr int kVariantBatch = -1;  // internal: slot-parallel kernels with per-system constants
 using InteractFn = void (*)(StageArgs, P4*);
 
+// one: the engine's repulsion and alignment cutoffs coincide (single engines of the repulsive model
+// with a repulsion law only; batched engines keep the general pair path).
 template <int MODEL, int REP, int I, int ST>
-StageFn variant_fn(int variant, bool small)
+StageFn variant_fn(int variant, bool small, bool one)
 {
     if (variant == kVariantBatch)
         return small ? k_stage_flat<MODEL, REP, I, ST, true, 8> : k_stage_flat<MODEL, REP, I, ST, true, 4>;
+    if constexpr (MODEL == SPP_MODEL_REPULSIVE && REP != SPP_REP_NONE) {
+        if (one)
+            return small ? k_stage_flat<MODEL, REP, I, ST, false, 8, true> : k_stage_flat<MODEL, REP, I, ST, false, 4, true>;
+    }
     return small ? k_stage_flat<MODEL, REP, I, ST, false, 8> : k_stage_flat<MODEL, REP, I, ST, false, 4>;
 }
 
 template <int MODEL, int REP>
-StageFn stage_fn(int integ, int stage, int variant, bool small)
+StageFn stage_fn(int integ, int stage, int variant, bool small, bool one)
 {
-    if (integ == SPP_EULER) return variant_fn<MODEL, REP, SPP_EULER, 0>(variant, small);
+    if (integ == SPP_EULER) return variant_fn<MODEL, REP, SPP_EULER, 0>(variant, small, one);
     if (integ == SPP_RK2)
-        return stage == 1 ? variant_fn<MODEL, REP, SPP_RK2, 1>(variant, small)
-                          : variant_fn<MODEL, REP, SPP_RK2, 2>(variant, small);
+        return stage == 1 ? variant_fn<MODEL, REP, SPP_RK2, 1>(variant, small, one)
+                          : variant_fn<MODEL, REP, SPP_RK2, 2>(variant, small, one);
     switch (stage) {
-    case 1: return variant_fn<MODEL, REP, SPP_RK4, 1>(variant, small);
-    case 2: return variant_fn<MODEL, REP, SPP_RK4, 2>(variant, small);
-    case 3: return variant_fn<MODEL, REP, SPP_RK4, 3>(variant, small);
-    default: return variant_fn<MODEL, REP, SPP_RK4, 4>(variant, small);
+    case 1: return variant_fn<MODEL, REP, SPP_RK4, 1>(variant, small, one);
+    case 2: return variant_fn<MODEL, REP, SPP_RK4, 2>(variant, small, one);
+    case 3: return variant_fn<MODEL, REP, SPP_RK4, 3>(variant, small, one);
+    default: return variant_fn<MODEL, REP, SPP_RK4, 4>(variant, small, one);
     }
 }
 
@@ -1368,7 +1375,7 @@ StageFn abp_stage_fn(int integ, int stage)
     return stage <= 1 ? k_stage_abp<REP, SPP_RK2, 1> : k_stage_abp<REP, SPP_RK2, 2>;
 }
 
-StageFn pick_stage(int model, int rep, int integ, int stage, int variant, bool small)
+StageFn pick_stage(int model, int rep, int integ, int stage, int variant, bool small, bool one)
 {
     if (model == SPP_MODEL_ABP) {
         switch (rep) {
@@ -1380,14 +1387,17 @@ StageFn pick_stage(int model, int rep, int integ, int stage, int variant, bool s
         default: return abp_stage_fn<SPP_REP_YUKAWA>(integ, stage);
         }
     }
-    if (model == SPP_MODEL_CONTINUOUS) return stage_fn<SPP_MODEL_CONTINUOUS, SPP_REP_NONE>(integ, stage, variant, small);
+    if (model == SPP_MODEL_CONTINUOUS)
+        return stage_fn<SPP_MODEL_CONTINUOUS, SPP_REP_NONE>(integ, stage, variant, small, false);
     switch (rep) {
-    case SPP_REP_NONE: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_NONE>(integ, stage, variant, small);
-    case SPP_REP_R12_TRUNCATED: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_R12_TRUNCATED>(integ, stage, variant, small);
-    case SPP_REP_HARMONIC: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_HARMONIC>(integ, stage, variant, small);
-    case SPP_REP_R12: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_R12>(integ, stage, variant, small);
-    case SPP_REP_YUKAWA_TRUNCATED: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_YUKAWA_TRUNCATED>(integ, stage, variant, small);
-    default: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_YUKAWA>(integ, stage, variant, small);
+    case SPP_REP_NONE: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_NONE>(integ, stage, variant, small, false);
+    case SPP_REP_R12_TRUNCATED:
+        return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_R12_TRUNCATED>(integ, stage, variant, small, one);
+    case SPP_REP_HARMONIC: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_HARMONIC>(integ, stage, variant, small, one);
+    case SPP_REP_R12: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_R12>(integ, stage, variant, small, one);
+    case SPP_REP_YUKAWA_TRUNCATED:
+        return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_YUKAWA_TRUNCATED>(integ, stage, variant, small, one);
+    default: return stage_fn<SPP_MODEL_REPULSIVE, SPP_REP_YUKAWA>(integ, stage, variant, small, one);
     }
 }
 
@@ -1636,6 +1646,7 @@ struct spp_engine {
     int *ncount = nullptr, *cellxy = nullptr;
     int variant = SPP_KERNEL_SLOT;
     bool deep = false;           // SPP_KERNEL_SLOT_DEEP: eight records in flight at every size
+    bool one_cut = false;        // repulsion and alignment share one cutoff (single repulsive engines)
     int nstat = 0;               // entries of the per-workgroup statistics arrays
     float2* build = nullptr;
     int *key = nullptr, *perm = nullptr, *cell_count = nullptr, *cell_start = nullptr,
@@ -2109,7 +2120,8 @@ int launch_step(spp_engine* e, int S, int W, int kstep, int slot, bool record)
             a.oth = ot[s];
         }
         a.stream = e->N <= kStreamMaxN;
-        StageFn fn = pick_stage(e->p.model, e->p.repulsion, integ, stage, e->variant, e->deep || e->N < kSmallN);
+        StageFn fn = pick_stage(e->p.model, e->p.repulsion, integ, stage, e->variant, e->deep || e->N < kSmallN,
+                                e->one_cut);
         TRY(launch(e, stage_name(integ, stage), fn, e->nblk, a));
         a.inj = nullptr;
         a.id_out = nullptr;
@@ -2876,6 +2888,9 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
         q.seed = sd[0].seed;
         q.n = sd[0].n;
         e->P = make_phys(q);
+        // the pair kernel's one-cutoff path: both laws end at the same exact threshold
+        e->one_cut = !systems && p->model == SPP_MODEL_REPULSIVE && p->repulsion != SPP_REP_NONE &&
+                     e->P.T_rep == e->P.T_align && e->P.T_align == e->P.T_max;
         if (systems) {
             e->nsys = nsys;
             e->hsys.resize(nsys);
@@ -3874,7 +3889,8 @@ int spp_time_stage(spp_engine* e, int32_t stage, int32_t reps, double* avg_us)
     a.dth = stage <= 1 ? e->dth_alt : e->dth;
     a.kacc = e->aux_alt[1];
     a.stream = e->N <= kStreamMaxN;
-    StageFn fn = pick_stage(e->p.model, e->p.repulsion, integ, stage, e->variant, e->deep || e->N < kSmallN);
+    StageFn fn = pick_stage(e->p.model, e->p.repulsion, integ, stage, e->variant, e->deep || e->N < kSmallN,
+                            e->one_cut);
     hipEvent_t t0, t1;
     HIPC(e, hipEventCreate(&t0));
     HIPC(e, hipEventCreate(&t1));

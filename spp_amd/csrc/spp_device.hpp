@@ -198,7 +198,32 @@ __host__ __device__ __forceinline__ double noise_normal(uint64_t seed, uint32_t 
 // me holds i's record (x, y, theta); (mc, ms) = (cos, sin) of theta_i, needed by the CONTINUOUS
 // self-term only.  sin(th_j - th_i) is taken from the angles (sin_diff), like particle.h:750.
 // IMG = false: the caller guarantees that no pair crosses a periodic edge (image_diff).
-template <int MODEL, int REP, bool IMG = true>
+// ONE = true (REPULSIVE with a repulsion law): the caller guarantees T_rep == T_align == T_max, so a
+// pair inside T_max feels both laws and the two inner tests (each a divergent branch) are skipped;
+// the arithmetic and its order are those of the general path, so the sums are bitwise equal.
+
+// The repulsion law's force on i, added to (fx, fy), for a pair at squared distance d2 > 0.
+template <int REP>
+__device__ __forceinline__ void repulsion_add(const Phys& P, double d2, double dx, double dy, double& fx, double& fy)
+{
+    double ri = rsqrt_pos(d2);  // 1/d
+    double s;
+    if (REP == SPP_REP_HARMONIC) {  // Spring, force-fields.h:87-97: k (rc - d)/d
+        s = P.A * (P.rc * ri - 1.0);
+    } else if (REP == SPP_REP_R12_TRUNCATED || REP == SPP_REP_R12) {  // :14-58
+        double q = P.sigma * ri, q2 = q * q, q4 = q2 * q2, q8 = q4 * q4;
+        double q13 = q8 * q4 * q;
+        s = (REP == SPP_REP_R12 ? P.A * q13 : P.A * (q13 - P.rc13)) * ri;
+    } else {  // Yukawa (truncated), :60-85
+        double d = d2 * ri;
+        double e = exp(-d * P.inv_sigma) * (ri * ri + ri * P.inv_sigma);
+        s = (REP == SPP_REP_YUKAWA ? P.A * e : P.A * (e - P.yuk_c)) * ri;
+    }
+    fx += s * dx;
+    fy += s * dy;
+}
+
+template <int MODEL, int REP, bool IMG = true, bool ONE = false>
 __device__ __forceinline__ void pair_accumulate(const Phys& P, const P4& me, double mc, double ms, const P4& pj,
                                                 double& fx, double& fy, double& s1, double& s2)
 {
@@ -213,23 +238,12 @@ __device__ __forceinline__ void pair_accumulate(const Phys& P, const P4& me, dou
         return;
     } else {
         if (d2 >= P.T_max) return;
-        if (REP != SPP_REP_NONE && d2 < P.T_rep) {
-            double ri = rsqrt_pos(d2);  // 1/d
-            double s;
-            if (REP == SPP_REP_HARMONIC) {  // Spring, force-fields.h:87-97: k (rc - d)/d
-                s = P.A * (P.rc * ri - 1.0);
-            } else if (REP == SPP_REP_R12_TRUNCATED || REP == SPP_REP_R12) {  // :14-58
-                double q = P.sigma * ri, q2 = q * q, q4 = q2 * q2, q8 = q4 * q4;
-                double q13 = q8 * q4 * q;
-                s = (REP == SPP_REP_R12 ? P.A * q13 : P.A * (q13 - P.rc13)) * ri;
-            } else {  // Yukawa (truncated), :60-85
-                double d = d2 * ri;
-                double e = exp(-d * P.inv_sigma) * (ri * ri + ri * P.inv_sigma);
-                s = (REP == SPP_REP_YUKAWA ? P.A * e : P.A * (e - P.yuk_c)) * ri;
-            }
-            fx += s * dx;
-            fy += s * dy;
+        if (ONE && REP != SPP_REP_NONE) {
+            repulsion_add<REP>(P, d2, dx, dy, fx, fy);
+            s1 += sin_diff(pj.z, me.z);
+            return;
         }
+        if (REP != SPP_REP_NONE && d2 < P.T_rep) repulsion_add<REP>(P, d2, dx, dy, fx, fy);
         if (d2 < P.T_align) s1 += sin_diff(pj.z, me.z);
     }
 }

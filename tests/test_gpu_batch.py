@@ -141,6 +141,33 @@ def test_batch_system_equals_single_engine_bitwise():
             assert np.array_equal(got[k][key], w[key]), (k, key)
 
 
+@pytest.mark.parametrize("law,integrator", [(S.REP_HARMONIC, S.RK2), (S.REP_HARMONIC, S.RK4),
+                                             (S.REP_HARMONIC, S.EULER), (S.REP_R12_TRUNCATED, S.RK2)])
+def test_one_cutoff_kernels_equal_the_general_pair_path_bitwise(law, integrator):
+    """Repulsion and alignment ending at the same radius (rc = R_a = 1, the benchmark's harmonic
+    workload): a single repulsive engine then runs the stage kernels without the two per-pair cutoff
+    tests, batched engines keep the general pair path.  Same rebuild steps, so the same lists and
+    summation order: the single engine must reproduce its system inside a batch bit for bit."""
+    p = S.default_params(S.MODEL_REPULSIVE).update(
+        dt=0.005, R_align=1.0, repulsion=law, A_rep=10.0 if law == S.REP_HARMONIC else 2.0, sigma_rep=1.0,
+        rc_rep=1.0, integrator=integrator, r_list=1.3, rebuild=S.REBUILD_PERIOD, rebuild_period=5)
+    systems = [S.system(n, 0.5 * math.sqrt(n), v0=1.0, g=1.0, D_rot=D, seed=30 + k)
+               for k, (n, D) in enumerate([(3000, 0.05), (1800, 0.2)])]
+    x, y, th = initial_states(systems, 90)
+    e = S.Engine(p, systems)
+    e.set_state(x, y, th)
+    e.multi_step(60)
+    got = split(e, e.get_state())
+    for k, s in enumerate(systems):
+        a, b = e.offsets[k], e.offsets[k + 1]
+        one = S.Engine(system_params(p, s))
+        one.set_state(x[a:b], y[a:b], th[a:b])
+        one.multi_step(60)
+        w = one.get_state()
+        for key in ("xw", "yw", "xu", "yu", "theta"):
+            assert np.array_equal(got[k][key], w[key]), (k, key)
+
+
 def c4a_ensemble(seed=4):
     """The ensemble workload: 256 Vicsek boxes, N log-uniform in [1024, 65536], density U[0.25, 4],
     D U[0, 0.5], v0 0.5 (about 3.9 million particles), uniform random starts."""
