@@ -16,6 +16,7 @@
 #pragma once
 
 #include <cmath>
+#include <cstring>
 
 #if defined(__HIPCC__)
 #define SPP_HD __host__ __device__
@@ -38,6 +39,21 @@ SPP_HD inline double fma_k(double a, double k, double c)
     return r;
 #else
     return std::fma(a, k, c);
+#endif
+}
+
+// (-1)^k x: the parity of k moved into x's sign bit (the same bits as k odd ? -x : x, without the
+// compare and select).
+SPP_HD inline double negate_if_odd(double x, int k)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __hiloint2double(__double2hiint(x) ^ (int)((unsigned)k << 31), __double2loint(x));
+#else
+    unsigned long long b;
+    std::memcpy(&b, &x, sizeof b);
+    b ^= (unsigned long long)((unsigned)k & 1u) << 63;
+    std::memcpy(&x, &b, sizeof x);
+    return x;
 #endif
 }
 
@@ -113,7 +129,7 @@ SPP_HD inline double sin_diff(double a, double b)
     const double q1 = fma(p67, z2, p45);
     const double p = fma(fma(p89, z4, q1), z4, q0);
     const double s = fma(r * z, p, r);
-    return ((int)kf & 1) ? -s : s;
+    return negate_if_odd(s, (int)kf);
 }
 
 }  // namespace spp
