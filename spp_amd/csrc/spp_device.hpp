@@ -187,7 +187,7 @@ __host__ __device__ __forceinline__ double noise_normal(uint64_t seed, uint32_t 
     double ub = ((double)(c[2] >> 5) * 67108864.0 + (double)(c[3] >> 6)) * (1.0 / 9007199254740992.0);
     double sn, cs;
     sincos_heading(kTwoPi * ub, &sn, &cs);  // cos(2 pi u_b), the heading routine (|arg| < 2 pi)
-    return sqrt(-2.0 * log(1.0 - ua)) * cs;
+    return sqrt(-2.0 * log_pos(1.0 - ua)) * cs;  // 1 - ua in [2^-53, 1]: positive, normal
 }
 
 // ------------------------------------------------------------------------------------------------

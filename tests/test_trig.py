@@ -57,3 +57,25 @@ def test_sin_diff_exact_values(sin_diff_out):
     z, t, u, one = (float(v) for v in sin_diff_out[1].split())
     assert z == 0.0 and t == 1e-12 and u == 1e-300
     assert abs(one - 1.0) <= 2.3e-16
+
+
+@pytest.fixture(scope="module")
+def log_out(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("trig") / "log_accuracy")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off",
+                    os.path.join(ROOT, "tests", "cpp", "log_accuracy.cpp"), "-o", exe], check=True)
+    return subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")
+
+
+def test_log_error_bound(log_out):
+    """The noise draw's logarithm: within one ulp of extended-precision logl on the draw's own
+    arguments 1 - u, next to 1 and to the reduction's boundaries, and over [2^-60, 2^10]."""
+    noise, edges, spread = (float(v) for v in log_out[0].split())
+    assert noise < 1.0 and edges < 1.0 and spread < 1.0
+
+
+def test_log_exact_values(log_out):
+    one, two, half, tiny = (float(v) for v in log_out[1].split())
+    assert one == 0.0
+    assert two == pytest.approx(math.log(2.0), rel=2.3e-16) and half == -two
+    assert tiny == pytest.approx(-53 * math.log(2.0), rel=2.3e-16)
