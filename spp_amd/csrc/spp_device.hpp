@@ -232,7 +232,7 @@ __device__ __forceinline__ void pair_accumulate(const Phys& P, const P4& me, dou
     double d2 = dx * dx + dy * dy;
     if (MODEL == SPP_MODEL_CONTINUOUS) {
         if (d2 < P.Ra2) {
-            s1 += sin_diff(pj.z, me.z);
+            s1 += sin_diff<false>(pj.z, me.z);
             if (P.c_self != 0.0) s2 += (dx * ms - dy * mc) * rsqrt(d2);
         }
         return;

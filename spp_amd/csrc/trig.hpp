@@ -105,7 +105,11 @@ SPP_HD inline void sincos_heading(double x, double* sn, double* cs)
 //     result keeps the relative accuracy of fl(a - b) itself;
 //   * sin r on |r| <= pi/2 by its Taylor series to r^21 (truncation below 1.3e-18), coefficients
 //     the doubles nearest (-1)^m/(2m+1)!, Estrin's scheme with explicit fma;
-//   * sin(k pi + r) = (-1)^k sin r.
+//   * sin(k pi + r) = (-1)^k sin r: XOR_SIGN moves k's parity into the sign bit (one fewer compare
+//     and select per pair), otherwise the sign is selected.  The two are bitwise equal; they differ
+//     in the registers the compiler allocates around them, which decides the occupancy of some
+//     stage kernels (the XOR form costs the continuous model's batched kernels a wave per SIMD).
+template <bool XOR_SIGN = true>
 SPP_HD inline double sin_diff(double a, double b)
 {
     constexpr double kInvPi = 0.3183098861837907;
@@ -129,7 +133,8 @@ SPP_HD inline double sin_diff(double a, double b)
     const double q1 = fma(p67, z2, p45);
     const double p = fma(fma(p89, z4, q1), z4, q0);
     const double s = fma(r * z, p, r);
-    return negate_if_odd(s, (int)kf);
+    if (XOR_SIGN) return negate_if_odd(s, (int)kf);
+    return ((int)kf & 1) ? -s : s;
 }
 
 // Natural logarithm of a positive normal double x, to about one ulp (the noise draw's log(1 - u),
