@@ -929,10 +929,12 @@ __global__ __launch_bounds__(kBlock) void k_nlist_flat(NlistFlatArgs a)
 
 // GU: neighbour records in flight per lane (a block of the list loop).  Small systems, whose stage
 // time is one wavefront's chain of loads, take 8; large ones 4 (fewer VGPRs, same speed or better).
-// ONE: one cutoff for both pair laws (pair_accumulate).
-template <int MODEL, int REP, int GU = 4, bool IMG = true, bool ONE = false>
+// ONE: one cutoff for both pair laws (pair_accumulate).  RAW: return the two torque sums in raw[]
+// instead of the torque (the caller forms it from constants it re-reads after the loop).
+template <int MODEL, int REP, int GU = 4, bool IMG = true, bool ONE = false, bool RAW = false>
 __device__ __forceinline__ void gather_forces_flat(const StageArgs& a, const Phys& P, int i, const P4& me,
-                                                   double mc, double ms, double& fx, double& fy, double& tq)
+                                                   double mc, double ms, double& fx, double& fy, double& tq,
+                                                   double* raw = nullptr)
 {
     const int cnt = a.ncount[i];
     const int* nb = a.nbr32 + i;
@@ -952,16 +954,29 @@ __device__ __forceinline__ void gather_forces_flat(const StageArgs& a, const Phy
         for (int u = 0; u < GU; ++u)
             if (k + u < cnt) spp::pair_accumulate<MODEL, REP, IMG, ONE>(P, me, mc, ms, q[u], fx, fy, s1, s2);
     }
-    tq = spp::drift_torque<MODEL>(P, s1, s2);
+    if (RAW) {
+        raw[0] = s1;
+        raw[1] = s2;
+    } else {
+        tq = spp::drift_torque<MODEL>(P, s1, s2);
+    }
 }
 
 // One particle's fused stage (interaction at the stage input + the move), for slot i of a system
 // whose constants P already hold and whose particle indices start at pid_off.  Returns the wrapped
 // output position for the displacement check.
-template <int MODEL, int REP, int INTEG, int STAGE, int GU = 4, bool ONE = false>
-__device__ __forceinline__ void stage_particle(const StageArgs& a, const Phys& P, int i, int pid_off, double& x,
+//
+// Stages after the first of large engines (four records in flight: many waves per SIMD, so
+// occupancy sets the speed) load their own inputs (theta_old, the noise, the unwrapped position,
+// RK4's accumulator) after the pair loop, and batched ones re-read their system's constants there,
+// instead of holding both in registers across the loop: the later stages' kernels fit one more
+// wave per SIMD (e.g. 102 -> 88 VGPRs for the batched continuous RK2 stage 2).  Same arithmetic.
+template <int MODEL, int REP, int INTEG, int STAGE, int GU = 4, bool ONE = false, bool BATCH = false>
+__device__ __forceinline__ void stage_particle(const StageArgs& a, const Phys& Pin, int i, int pid_off, double& x,
                                                double& y)
 {
+    constexpr bool kLate = STAGE > 1 && GU != 8;
+    Phys P = Pin;
     P4 me = stage_record(a, i);
     const bool img = wave_needs_image(P, true, me.x, me.y);  // over the lanes that run this
     double mc, ms;  // v = (cos, sin) of the stage input's heading
@@ -977,17 +992,35 @@ __device__ __forceinline__ void stage_particle(const StageArgs& a, const Phys& P
         // the stream is addressed by the particle's index within its system, so systems with
         // equal seeds (Lyapunov clones) draw identical noise
         dn = a.inj ? a.inj[i] : P.noise_amp * spp::noise_normal(P.seed, (uint32_t)(pid - pid_off), noise_step(a));
-    } else {
+    } else if (!kLate) {
         thold = a.th0[i];  // theta_old: the step's input set, which no stage has overwritten yet
         dn = a.dth[i];
         u = a.U_in[i];
         if (INTEG == SPP_RK4) kk = a.kacc[i];
     }
     double fx, fy, tq;
-    if (img)
-        gather_forces_flat<MODEL, REP, GU, true, ONE>(a, P, i, me, mc, ms, fx, fy, tq);
-    else
-        gather_forces_flat<MODEL, REP, GU, false, ONE>(a, P, i, me, mc, ms, fx, fy, tq);
+    if (BATCH && kLate) {
+        double raw[2];
+        if (img)
+            gather_forces_flat<MODEL, REP, GU, true, ONE, true>(a, P, i, me, mc, ms, fx, fy, tq, raw);
+        else
+            gather_forces_flat<MODEL, REP, GU, false, ONE, true>(a, P, i, me, mc, ms, fx, fy, tq, raw);
+        asm volatile("" ::: "memory");  // re-read from memory, not kept from before the loop
+        apply_sys(P, a.sys_tab[a.sys_of[i]]);
+        tq = spp::drift_torque<MODEL>(P, raw[0], raw[1]);
+    } else {
+        if (img)
+            gather_forces_flat<MODEL, REP, GU, true, ONE>(a, P, i, me, mc, ms, fx, fy, tq);
+        else
+            gather_forces_flat<MODEL, REP, GU, false, ONE>(a, P, i, me, mc, ms, fx, fy, tq);
+    }
+    if (kLate) {
+        asm volatile("" ::: "memory");  // issued here, after the pair loop
+        thold = a.th0[i];
+        dn = a.dth[i];
+        u = a.U_in[i];
+        if (INTEG == SPP_RK4) kk = a.kacc[i];
+    }
     double th;
     if (STAGE <= 1) {
         if (INTEG == SPP_EULER) {
@@ -1046,7 +1079,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_flat(StageArgs a)
     }
     constexpr bool final_stage = (INTEG == SPP_EULER) || (INTEG == SPP_RK2 && STAGE == 2) || (STAGE == 4);
     double x = 0, y = 0, dmax = 0;
-    if (live) stage_particle<MODEL, REP, INTEG, STAGE, GU, ONE>(a, P, i, pid_off, x, y);
+    if (live) stage_particle<MODEL, REP, INTEG, STAGE, GU, ONE, BATCH>(a, P, i, pid_off, x, y);
     check_disp(a, P, b, live, i, x, y, final_stage ? a.kstep + 1 : a.kstep, dmax);
     finish_disp(a, b, dmax);
 }
