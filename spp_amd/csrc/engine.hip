@@ -88,18 +88,108 @@ struct SlotBufs {
     int* blknnz;      // per-block list entries
 };
 
+// One interval's report: its slot's partials, the slot's rebuild failure flag, the slot's report
+// count and the host-mapped words it is written to.  An interval's report rides on the next
+// interval's first launch (the tile sort or the first stage kernel) as one extra workgroup after the
+// launch's own, which saves a kernel launch per interval; the host learns the result while that
+// launch runs, and launches the report on its own (k_report) when no such launch follows.
+struct ReportArgs {
+    int nblk;                  // partials to reduce
+    SlotBufs b;
+    int with_list;             // the interval rebuilt: list statistics too
+    int* sort_fail;            // the slot's tile-sort failure flag (read, then cleared)
+    long long* count;          // the slot's report count on the device
+    unsigned long long* out;   // the slot's host-mapped words
+};
+
+// Reduces one interval's per-block partials into the slot's host-mapped report words (one
+// workgroup of kBlock threads) and clears the per-block bad steps for the slot's next interval.
+// Each word carries the report
+// number (the slot's count of reports, kept on the device so that replays of a captured interval
+// count too) in its upper half and goes out as one system-scope store, so the host takes the
+// report once every word's tag matches, with no fence on the device between them.  The host polls
+// the words instead of waiting on an event: an event record between two intervals idles the GPU
+// for about 5 us on this device.
+__device__ void report_body(const ReportArgs& r)
+{
+    const int nblk = r.nblk;
+    const SlotBufs& b = r.b;
+    const int with_list = r.with_list;
+    int* sort_fail = r.sort_fail;
+    long long* count = r.count;
+    unsigned long long* out = r.out;
+    __shared__ int r_bad[kBlock / 64], r_len[kBlock / 64];
+    __shared__ long long r_nnz[kBlock / 64];
+    __shared__ double r_max[kBlock / 64];
+    int bad = INT_MAX, len = 0;
+    long long nnz = 0;
+    double mx = 0;
+    for (int k = threadIdx.x; k < nblk; k += kBlock) {
+        bad = min(bad, b.blkbad[k]);
+        b.blkbad[k] = INT_MAX;  // the slot's next interval starts with no bad step
+        mx = fmax(mx, b.blkmax[k]);
+        if (with_list) {
+            len = max(len, b.blklen[k]);
+            nnz += b.blknnz[k];
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        bad = min(bad, __shfl_xor(bad, o));
+        len = max(len, __shfl_xor(len, o));
+        nnz += __shfl_xor(nnz, o);
+        mx = fmax(mx, __shfl_xor(mx, o));
+    }
+    int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        r_bad[w] = bad;
+        r_len[w] = len;
+        r_nnz[w] = nnz;
+        r_max[w] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlock / 64; ++k) {
+            bad = min(bad, r_bad[k]);
+            len = max(len, r_len[k]);
+            nnz += r_nnz[k];
+            mx = fmax(mx, r_max[k]);
+        }
+        bad = min(bad, r_bad[0]);
+        len = max(len, r_len[0]);
+        mx = fmax(mx, r_max[0]);
+        nnz = 0;
+        for (int k = 0; k < kBlock / 64; ++k) nnz += r_nnz[k];
+        const int fail = *sort_fail;
+        *sort_fail = 0;  // the next interval's rebuild starts clear
+        const long long seq = *count + 1;
+        *count = seq;
+        const unsigned long long tag = (unsigned long long)(unsigned)seq << 32;
+        const unsigned long long n = (unsigned long long)nnz;
+        const unsigned long long bits = (unsigned long long)__double_as_longlong(mx);
+        const unsigned long long w[kRepUsed] = {tag | (unsigned)bad,       tag | (unsigned)len,
+                                                tag | (n & 0xffffffffull), tag | (n >> 32),
+                                                tag | (bits & 0xffffffffull), tag | (bits >> 32),
+                                                tag | (unsigned)fail};
+#pragma unroll
+        for (int k = 0; k < kRepUsed; ++k) __hip_atomic_store(out + k, w[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 // ------------------------------------------------------------------------------------------------
 // Small device utilities
 // ------------------------------------------------------------------------------------------------
 
 // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of
 // slots (a band of the box) so its L2 keeps that band's particles.
-__device__ __forceinline__ int xcd_block()
+__device__ __forceinline__ int xcd_block_of(int nblk)
 {
-    int nblk = gridDim.x, b = blockIdx.x;
+    const int b = blockIdx.x;
     if ((nblk & 7) == 0) return (b & 7) * (nblk >> 3) + (b >> 3);
     return b;
 }
+
+__device__ __forceinline__ int xcd_block() { return xcd_block_of(gridDim.x); }
 
 // Record j of a position array by a 32-bit byte offset from its (uniform) base (engines hold fewer
 // than 2^27 particles), which the load takes as a VGPR offset to an SGPR base.
@@ -399,6 +489,8 @@ struct SortArgs {
     int* cs_new;
     unsigned long long* tstate;    // per tile: (epoch << 2 | 1) << 32 | particle count
     int* fail;
+    int rep_on;                    // workgroup ntiles reduces the previous interval's report
+    ReportArgs rep;
 };
 
 __device__ __forceinline__ unsigned long long tile_load(const unsigned long long* p)
@@ -442,6 +534,10 @@ __device__ __forceinline__ int tile_put(const SortArgs& a, int slot, int mine, d
 
 __global__ __launch_bounds__(kBlock) void k_sort_tiles(SortArgs a)
 {
+    if (a.rep_on && (int)blockIdx.x == a.ntiles) {  // the previous interval's report (dispatched last)
+        report_body(a.rep);
+        return;
+    }
     __shared__ int e_old[kSortCap], e_id[kSortCap];
     __shared__ uint16_t e_col[kSortCap], order[kSortCap];
     __shared__ int hist[kSortMaxW + 1], fillc[kSortMaxW];
@@ -689,6 +785,9 @@ struct StageArgs {
     double* blockmax;      // per-workgroup max displacement^2 (last step of an interval)
     int record;
     bool stream;           // write the stage outputs with streaming stores (kStreamMaxN)
+    int grid;              // the launch's workgroups for the particles (one more carries rep)
+    int rep_on;            // workgroup grid reduces the previous interval's report
+    ReportArgs rep;
 };
 
 // The noise stream's step of a stage launch: absolute, or relative to the interval's first step
@@ -1067,7 +1166,11 @@ __device__ __forceinline__ void stage_particle(const StageArgs& a, const Phys& P
 template <int MODEL, int REP, int INTEG, int STAGE, bool BATCH, int GU = 4, bool ONE = false>
 __global__ __launch_bounds__(kBlock) void k_stage_flat(StageArgs a)
 {
-    int b = xcd_block();
+    if (a.rep_on && (int)blockIdx.x == a.grid) {  // the previous interval's report (dispatched last)
+        report_body(a.rep);
+        return;
+    }
+    int b = xcd_block_of(a.grid);
     int i = b * kBlock + threadIdx.x;
     bool live = i < a.N;
     Phys P = a.P;
@@ -1296,72 +1399,11 @@ __global__ __launch_bounds__(kBlock) void k_pair_hist(int N, const double2* __re
         if (sh[k]) atomicAdd(&hist[k], (unsigned long long)sh[k]);
 }
 
-// Reduces one interval's per-block partials into the slot's host-mapped report words (one block)
-// and clears the per-block bad steps for the slot's next interval.  Each word carries the report
-// number (the slot's count of reports, kept on the device so that replays of a captured interval
-// count too) in its upper half and goes out as one system-scope store, so the host takes the
-// report once every word's tag matches, with no fence on the device between them.  The host polls
-// the words instead of waiting on an event: an event record between two intervals idles the GPU
-// for about 5 us on this device.
-__global__ __launch_bounds__(kBlock) void k_report(int nblk, SlotBufs b, int with_list, int* sort_fail, long long* count,
-                                                  unsigned long long* out)
+// One interval's report as a kernel of its own (the interval's report could not ride on the next
+// interval's first launch, see ReportArgs).
+__global__ __launch_bounds__(kBlock) void k_report(ReportArgs r)
 {
-    __shared__ int r_bad[kBlock / 64], r_len[kBlock / 64];
-    __shared__ long long r_nnz[kBlock / 64];
-    __shared__ double r_max[kBlock / 64];
-    int bad = INT_MAX, len = 0;
-    long long nnz = 0;
-    double mx = 0;
-    for (int k = threadIdx.x; k < nblk; k += kBlock) {
-        bad = min(bad, b.blkbad[k]);
-        b.blkbad[k] = INT_MAX;  // the slot's next interval starts with no bad step
-        mx = fmax(mx, b.blkmax[k]);
-        if (with_list) {
-            len = max(len, b.blklen[k]);
-            nnz += b.blknnz[k];
-        }
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        bad = min(bad, __shfl_xor(bad, o));
-        len = max(len, __shfl_xor(len, o));
-        nnz += __shfl_xor(nnz, o);
-        mx = fmax(mx, __shfl_xor(mx, o));
-    }
-    int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        r_bad[w] = bad;
-        r_len[w] = len;
-        r_nnz[w] = nnz;
-        r_max[w] = mx;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 1; k < kBlock / 64; ++k) {
-            bad = min(bad, r_bad[k]);
-            len = max(len, r_len[k]);
-            nnz += r_nnz[k];
-            mx = fmax(mx, r_max[k]);
-        }
-        bad = min(bad, r_bad[0]);
-        len = max(len, r_len[0]);
-        mx = fmax(mx, r_max[0]);
-        nnz = 0;
-        for (int k = 0; k < kBlock / 64; ++k) nnz += r_nnz[k];
-        const int fail = *sort_fail;
-        *sort_fail = 0;  // the next interval's rebuild starts clear
-        const long long seq = *count + 1;
-        *count = seq;
-        const unsigned long long tag = (unsigned long long)(unsigned)seq << 32;
-        const unsigned long long n = (unsigned long long)nnz;
-        const unsigned long long bits = (unsigned long long)__double_as_longlong(mx);
-        const unsigned long long w[kRepUsed] = {tag | (unsigned)bad,       tag | (unsigned)len,
-                                                tag | (n & 0xffffffffull), tag | (n >> 32),
-                                                tag | (bits & 0xffffffffull), tag | (bits >> 32),
-                                                tag | (unsigned)fail};
-#pragma unroll
-        for (int k = 0; k < kRepUsed; ++k) __hip_atomic_store(out + k, w[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
+    report_body(r);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1705,6 +1747,11 @@ struct spp_engine {
     unsigned long long* dwords = nullptr;  // device view of hwords
     long long* rep_count = nullptr;  // device: reports written per slot
     long long rep_issued[2] = {0, 0};  // host: reports enqueued per slot (launched or replayed)
+    // the last enqueued interval's report, not launched yet: it rides on the next interval's first
+    // launch, or is launched on its own before the host waits for it
+    bool rep_pending = false;
+    int rep_pending_slot = 0;
+    bool rep_pending_list = false;
     double2* dpolar = nullptr;
     // batched engine: per-system constants and the (fixed) slot -> system map; null for one box
     int nsys = 1;
@@ -1917,12 +1964,36 @@ int launch_nlist(spp_engine* e, int slot)
     return launch(e, "nlist", k_nlist_flat, e->nblk, fa);
 }
 
+// The report of the slot's interval, counted as issued (also in dry mode: a dry interval is replayed
+// from a captured graph, which runs it).
+ReportArgs report_args(spp_engine* e, int slot, bool with_list)
+{
+    e->rep_issued[slot]++;
+    return ReportArgs{e->nstat, e->slotb[slot], with_list ? 1 : 0, e->sort_fail + slot, e->rep_count + slot,
+                      e->dwords + slot * kRepWords};
+}
+
 int launch_report(spp_engine* e, int slot, bool with_list)
 {
-    // counted also in dry mode: a dry interval is replayed from a captured graph, which runs it
-    e->rep_issued[slot]++;
-    return launch(e, "report", k_report, 1, e->nstat, e->slotb[slot], with_list ? 1 : 0, e->sort_fail,
-                  e->rep_count + slot, e->dwords + slot * kRepWords);
+    return launch(e, "report", k_report, 1, report_args(e, slot, with_list));
+}
+
+// The pending report for a launch that can carry it as its extra workgroup (false: none pending).
+bool take_pending_report(spp_engine* e, ReportArgs* r)
+{
+    if (!e->rep_pending) return false;
+    e->rep_pending = false;
+    *r = report_args(e, e->rep_pending_slot, e->rep_pending_list);
+    return true;
+}
+
+// Launches the pending report on its own (before a launch that cannot carry it, or before the
+// host waits for it).
+int flush_pending_report(spp_engine* e)
+{
+    if (!e->rep_pending) return SPP_OK;
+    e->rep_pending = false;
+    return launch_report(e, e->rep_pending_slot, e->rep_pending_list);
 }
 
 // Waits until the slot's report number `want` has reached host memory, then decodes it into
@@ -1990,10 +2061,11 @@ int launch_rebuild(spp_engine* e, int dst, int slot, bool aux, bool tile)
         sa.cs_old = e->cell_start;
         sa.cs_new = e->cell_start_alt;
         sa.tstate = e->tile_state;
-        sa.fail = e->sort_fail;
+        sa.fail = e->sort_fail + slot;
         e->sort_epoch = e->sort_epoch % ((1u << 30) - 1) + 1;  // 1 .. 2^30 - 1, never the cleared 0
         sa.epoch = e->sort_epoch;
-        TRY(launch(e, "sort_tiles", k_sort_tiles, e->sort_ntiles, sa));
+        sa.rep_on = take_pending_report(e, &sa.rep) ? 1 : 0;
+        TRY(launch(e, "sort_tiles", k_sort_tiles, e->sort_ntiles + sa.rep_on, sa));
         std::swap(e->cell_start, e->cell_start_alt);
         e->tile_rebuilds++;
         e->cur = dst;
@@ -2004,6 +2076,7 @@ int launch_rebuild(spp_engine* e, int dst, int slot, bool aux, bool tile)
         e->list_valid = true;
         return SPP_OK;
     }
+    TRY(flush_pending_report(e));
     e->force_general = false;
     e->order_valid = true;  // the general path sorts any order
     // the arrival ranks live in perm2 until the per-cell ordering overwrites it
@@ -2155,7 +2228,18 @@ int launch_step(spp_engine* e, int S, int W, int kstep, int slot, bool record)
         a.stream = e->N <= kStreamMaxN;
         StageFn fn = pick_stage(e->p.model, e->p.repulsion, integ, stage, e->variant, e->deep || e->N < kSmallN,
                                 e->one_cut);
-        TRY(launch(e, stage_name(integ, stage), fn, e->nblk, a));
+        a.grid = e->nblk;
+        a.rep_on = 0;
+        if (e->rep_pending) {
+            // the previous interval's report rides on this launch (the stage kernels of the slot
+            // family carry it; the active-Brownian stages do not)
+            if (e->p.model == SPP_MODEL_ABP)
+                TRY(flush_pending_report(e));
+            else
+                a.rep_on = take_pending_report(e, &a.rep) ? 1 : 0;
+        }
+        TRY(launch(e, stage_name(integ, stage), fn, e->nblk + a.rep_on, a));
+        a.rep_on = 0;
         a.inj = nullptr;
         a.id_out = nullptr;
     }
@@ -2218,7 +2302,13 @@ int interval_body(spp_engine* e, int m, bool rebuild, int protect, int slot, Int
         TRY(launch_step(e, e->cur, k == 0 ? W : e->cur, k, slot, k == m - 1));
         if (k == 0) e->inj_pending = false;
     }
-    return launch_report(e, slot, rebuild);
+    if (e->graph_mode || e->dry) return launch_report(e, slot, rebuild);
+    TRY(flush_pending_report(e));  // (this interval's first launch has taken the previous one)
+    // the report rides on the next interval's first launch (or is launched before the host waits)
+    e->rep_pending = true;
+    e->rep_pending_slot = slot;
+    e->rep_pending_list = rebuild;
+    return SPP_OK;
 }
 
 __global__ void k_set_i64(long long* p, long long v)
@@ -2245,6 +2335,7 @@ int enqueue_interval(spp_engine* e, int m, bool rebuild, int protect, Interval& 
     if (!graphable(e, m, rebuild)) {
         TRY(interval_body(e, m, rebuild, protect, slot, iv));
     } else {
+        TRY(flush_pending_report(e));  // a captured launch must not carry another interval's report
         if (!e->dclock) TRY(dalloc(e, &e->dclock, 1));
         spp_engine::GraphEntry* g = nullptr;
         for (auto& x : e->graphs)
@@ -2286,7 +2377,7 @@ int enqueue_interval(spp_engine* e, int m, bool rebuild, int protect, Interval& 
         HIPC(e, hipGraphLaunch(g->exec, e->stream));
         e->graph_replays++;
     }
-    iv.seq = e->rep_issued[slot];
+    iv.seq = e->rep_issued[slot] + (e->rep_pending && e->rep_pending_slot == slot ? 1 : 0);
     return SPP_OK;
 }
 
@@ -2362,6 +2453,7 @@ int run_intervals(spp_engine* e, int steps)
             continue;
         }
         Interval iv = q.front();
+        if (e->rep_pending && e->rep_pending_slot == iv.slot) TRY(flush_pending_report(e));
         TRY(wait_report(e, iv.slot, iv.seq));
         e->host_syncs++;
         const Report f = e->hrep[iv.slot];
@@ -2394,6 +2486,9 @@ int run_intervals(spp_engine* e, int steps)
         }
         // Roll back to the interval's start; it and everything enqueued after it are redone.
         HIPC(e, hipStreamSynchronize(e->stream));
+        // a discarded interval's report still runs: it clears its slot's per-block bad steps and
+        // failure flag for the slot's next interval (its result is not read)
+        TRY(flush_pending_report(e));
         int redo = 0;
         for (auto& x : q) redo += x.m;
         q.clear();
@@ -3038,7 +3133,7 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
         A(dalloc(e, &e->cell_start_alt, (size_t)e->nkeys + 1));
         A(dalloc(e, &e->tile_state, (size_t)e->sort_ntiles));
     }
-    A(dalloc(e, &e->sort_fail, 1));
+    A(dalloc(e, &e->sort_fail, 2));  // one per report slot
     A(dalloc(e, &e->block_sums, (size_t)e->nscan_blocks));
     for (int sl = 0; sl < 2; ++sl) {
         A(dalloc(e, &e->slotb[sl].blkbad, (size_t)e->nstat));
@@ -3093,7 +3188,7 @@ int create_engine(const spp_params* p, const spp_system* systems, int nsys, spp_
         // cell counts start at zero and every rebuild's scatter returns them to zero; the tile
         // path's ticket counters and look-back states start at zero and every launch clears them
         if (launch(e, "fill", k_fill<int>, nblocks(e->nkeys), e->cell_count, 0, e->nkeys) != SPP_OK ||
-            launch(e, "fill", k_fill<int>, 1, e->sort_fail, 0, 1) != SPP_OK)
+            launch(e, "fill", k_fill<int>, 1, e->sort_fail, 0, 2) != SPP_OK)
             rc = SPP_ERR_HIP;
         if (rc == SPP_OK && e->sort_ok &&
             launch(e, "fill", k_fill<unsigned long long>, (e->sort_ntiles + 255) / 256, e->tile_state, 0ull,
@@ -3924,6 +4019,7 @@ int spp_time_stage(spp_engine* e, int32_t stage, int32_t reps, double* avg_us)
     a.stream = e->N <= kStreamMaxN;
     StageFn fn = pick_stage(e->p.model, e->p.repulsion, integ, stage, e->variant, e->deep || e->N < kSmallN,
                             e->one_cut);
+    a.grid = e->nblk;
     hipEvent_t t0, t1;
     HIPC(e, hipEventCreate(&t0));
     HIPC(e, hipEventCreate(&t1));
