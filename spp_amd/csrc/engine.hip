@@ -104,12 +104,11 @@ struct ReportArgs {
 
 // Reduces one interval's per-block partials into the slot's host-mapped report words (one
 // workgroup of kBlock threads) and clears the per-block bad steps for the slot's next interval.
-// Each word carries the report
-// number (the slot's count of reports, kept on the device so that replays of a captured interval
-// count too) in its upper half and goes out as one system-scope store, so the host takes the
-// report once every word's tag matches, with no fence on the device between them.  The host polls
-// the words instead of waiting on an event: an event record between two intervals idles the GPU
-// for about 5 us on this device.
+// Each word carries the report number (the slot's count of reports, kept on the device so that
+// replays of a captured interval count too) in its upper half and goes out as one system-scope
+// store, so the host takes the report once every word's tag matches, with no fence on the device
+// between them.  The host polls the words instead of waiting on an event: an event record between
+// two intervals idles the GPU for about 5 us on this device.
 __device__ void report_body(const ReportArgs& r)
 {
     const int nblk = r.nblk;
